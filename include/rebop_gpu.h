@@ -1,0 +1,166 @@
+/*
+ * rebop_gpu.h -- C ABI of the MI355X batched direct-method SSA.
+ *
+ * This is the drop-in boundary for rebop's run-time Gillespie path: the
+ * entry points below are what a Rust FFI crate (or the Python extension)
+ * binds in place of calling rebop's `gillespie::Gillespie` directly.  Each
+ * declaration names the reference interface it replaces (paths relative to
+ * the rebop source tree).  Plain C types only; every pointer argument is a
+ * host pointer unless the function name ends in `_device`, in which case it
+ * is a device pointer on the current HIP device.
+ *
+ * Semantics are those of rebop's `Gillespie::advance_until` driven by the
+ * `Gillespie.run` grid loop, for N independent trajectories: trajectory j
+ * (global id traj_begin + j) draws from the Philox4x32-10 stream keyed by
+ * (seed, traj_begin + j); counts and grid samples are bit-identical to the
+ * reference algorithm driven by that stream.
+ *
+ * Errors: every function returning int returns 0 (REBOP_OK) or one of the
+ * REBOP_ERR_* codes; rebop_last_error() then describes the failure (the
+ * message is thread-local and valid until the next failing call on that
+ * thread).  Where the reference panics (`assert!`), the ABI returns an error.
+ */
+#ifndef REBOP_GPU_H
+#define REBOP_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define REBOP_GPU_ABI_VERSION 1
+
+enum rebop_status {
+  REBOP_OK = 0,
+  REBOP_ERR_SHAPE = 1,  /* length mismatch: assert_eq! at src/gillespie.rs:228, :237, :265 */
+  REBOP_ERR_INDEX = 2,  /* species index out of range: assert! at src/gillespie.rs:229-233 */
+  REBOP_ERR_VALUE = 3,  /* invalid argument value (e.g. nb_steps == 0, counts too large) */
+  REBOP_ERR_DEVICE = 4, /* HIP runtime failure, or no usable gfx950 device */
+  REBOP_ERR_ALLOC = 5,  /* host or device allocation failed */
+  REBOP_ERR_RANGE = 6,  /* a count left the exactly simulated range during a run */
+  REBOP_ERR_UNSUPPORTED = 7 /* network shape beyond the compiled kernels */
+};
+
+/* Counters of one run, summed over its trajectories. */
+typedef struct rebop_stats {
+  uint64_t events;       /* applied reactions (Jump::affect calls) */
+  uint64_t draws;        /* Philox blocks drawn (events + discarded overshoot draws) */
+  uint64_t trajectories; /* trajectories completed */
+  uint64_t kernel;       /* which kernel ran: REBOP_KERNEL_* */
+  double kernel_ms;      /* device time of the SSA kernel (0 if not timed) */
+} rebop_stats;
+
+enum rebop_kernel_kind {
+  REBOP_KERNEL_AUTO = 0,
+  REBOP_KERNEL_PLAIN = 1,      /* one trajectory per lane, lanes retire when done */
+  REBOP_KERNEL_PERSISTENT = 2, /* lanes refill from an atomic trajectory queue */
+  REBOP_KERNEL_LDS = 3         /* counts in LDS, for networks beyond the register kernels */
+};
+
+/* Opaque network + initial state: rebop's `gillespie::Gillespie` (src/gillespie.rs:157-163). */
+typedef struct rebop_gillespie rebop_gillespie;
+
+/* Gillespie::new(species, sparse) -- src/gillespie.rs:168-176.  The seed is
+ * drawn from the OS entropy source, as rand::make_rng() does. */
+int rebop_gillespie_new(const int64_t *species, uint32_t nb_species, int sparse,
+                        rebop_gillespie **out);
+
+/* Gillespie::new_with_seed(species, sparse, seed) -- src/gillespie.rs:179-187. */
+int rebop_gillespie_new_with_seed(const int64_t *species, uint32_t nb_species, int sparse,
+                                  uint64_t seed, rebop_gillespie **out);
+
+/* Drop for Gillespie.  Accepts NULL. */
+void rebop_gillespie_free(rebop_gillespie *g);
+
+/* Gillespie::seed(seed) -- src/gillespie.rs:189-191. */
+int rebop_gillespie_seed(rebop_gillespie *g, uint64_t seed);
+
+/* Gillespie::nb_species / nb_reactions -- src/gillespie.rs:199-211. */
+uint32_t rebop_gillespie_nb_species(const rebop_gillespie *g);
+uint32_t rebop_gillespie_nb_reactions(const rebop_gillespie *g);
+
+/* add_reaction(Rate::lma(rate, orders), differences) -- src/gillespie.rs:225-244
+ * with Rate::lma at :24-26.  n_orders and n_differences must equal nb_species. */
+int rebop_gillespie_add_reaction_lma(rebop_gillespie *g, double rate, const uint32_t *orders,
+                                     uint32_t n_orders, const int64_t *differences,
+                                     uint32_t n_differences);
+
+/* add_reaction(Rate::lma_sparse(rate, [(species_idx[i], orders[i])]), differences)
+ * -- src/gillespie.rs:225-244 with Rate::lma_sparse at :29-31.  The pairs are
+ * evaluated in the given order (duplicates allowed), as Rate::rate does. */
+int rebop_gillespie_add_reaction_lma_sparse(rebop_gillespie *g, double rate,
+                                            const uint32_t *species_idx, const uint32_t *orders,
+                                            uint32_t n_pairs, const int64_t *differences,
+                                            uint32_t n_differences);
+
+/* Gillespie::set_species / get_species on the initial state -- src/gillespie.rs:260-267. */
+int rebop_gillespie_set_species(rebop_gillespie *g, const int64_t *species, uint32_t n);
+int rebop_gillespie_get_species(const rebop_gillespie *g, int64_t *species, uint32_t n);
+
+/*
+ * The grid loop of rebop's Python `Gillespie.run(tmax, nb_steps, seed=...)`
+ * (src/pyo3_gillespie.rs:197-208: for i in 0..=nb_steps,
+ * advance_until(tmax * i / nb_steps), record the counts), run for n_traj
+ * independent trajectories on the GPU.
+ *   x0      [n_traj][nb_species] per-trajectory initial counts, or NULL to
+ *           start every trajectory from the handle's species vector.
+ *   out     [nb_steps+1][nb_species][n_traj] int64, trajectory fastest (may be NULL).
+ *   times   [nb_steps+1] sample times tmax*i/nb_steps (may be NULL).
+ *   stats   counters of the run (may be NULL).
+ * nb_steps must be > 0 (the nb_steps == 0 all-events mode is
+ * rebop_run_all_events).  Host buffers; blocks until the results are copied.
+ */
+int rebop_run(const rebop_gillespie *g, double tmax, uint64_t nb_steps, uint64_t traj_begin,
+              uint64_t n_traj, const int64_t *x0, int64_t *out, double *times,
+              rebop_stats *stats);
+
+/*
+ * rebop_run with device-resident buffers, enqueued on `stream` (a
+ * hipStream_t, NULL for the default stream).  d_x0 may be NULL; d_out may be
+ * NULL (then only d_final and the counters are produced).  d_final
+ * (nullable): [nb_species][n_traj] int64 counts after the last grid point.
+ * kernel: REBOP_KERNEL_* (AUTO lets the tuner choose).  When stats is
+ * non-NULL the call synchronises the stream to read the counters back and
+ * to time the kernel; with stats == NULL it returns as soon as the work is
+ * enqueued.
+ */
+int rebop_run_device(const rebop_gillespie *g, double tmax, uint64_t nb_steps,
+                     uint64_t traj_begin, uint64_t n_traj, const int64_t *d_x0, int64_t *d_out,
+                     int64_t *d_final, int kernel, void *stream, rebop_stats *stats);
+
+/* Batched resumable state: N trajectories of one network, each with its own
+ * (species, t, rng) -- the fields of gillespie::Gillespie (src/gillespie.rs:157-163)
+ * for every trajectory.  Device-resident. */
+typedef struct rebop_batch rebop_batch;
+
+/* N copies of Gillespie::new_with_seed: trajectory j starts from x0[j] (or
+ * the handle's species if x0 is NULL), t = 0, stream (seed, traj_begin + j). */
+int rebop_batch_new(const rebop_gillespie *g, uint64_t traj_begin, uint64_t n_traj,
+                    const int64_t *x0, rebop_batch **out);
+void rebop_batch_free(rebop_batch *b);
+
+/* Gillespie::advance_until(tmax) on every trajectory -- src/gillespie.rs:315-344. */
+int rebop_batch_advance_until(rebop_batch *b, double tmax, rebop_stats *stats);
+
+/* Gillespie::get_species (src/gillespie.rs:260-262) for every trajectory:
+ * species [n_traj][nb_species]. */
+int rebop_batch_get_species(const rebop_batch *b, int64_t *species);
+/* Gillespie::set_species (src/gillespie.rs:264-267) for every trajectory. */
+int rebop_batch_set_species(rebop_batch *b, const int64_t *species);
+/* Gillespie::get_time / set_time (src/gillespie.rs:246-252) for every trajectory. */
+int rebop_batch_get_time(const rebop_batch *b, double *t);
+int rebop_batch_set_time(rebop_batch *b, const double *t);
+
+/* Description of the last error on this thread ("" if none). */
+const char *rebop_last_error(void);
+
+/* Name of the device the library runs on, and its compute-unit count. */
+int rebop_device_info(char *name, size_t name_len, int *cu_count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* REBOP_GPU_H */

@@ -1,0 +1,456 @@
+/*
+ * oracle.c -- CPU restatement of rebop's run-time direct-method SSA.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker; see oracle.h).  Counts are int64
+ * (Rust isize) and every count enters floating point through the same
+ * int64 -> f64 conversion the reference performs, so this file follows the
+ * reference's arithmetic literally; only the random source differs (the
+ * Philox stream of rebop_rng.h, or xoshiro256++ for independent streams).
+ *
+ * Build: oracle/Makefile (gcc -O3 -ffp-contract=off; contraction would
+ * change the rounding of the propensity products and the cumulative sum).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#include "../rebop_amd/csrc/rebop_rng.h"
+
+/* ------------------------------------------------------------------ */
+/* network: a list of (Rate, Jump) as Gillespie.reactions (gillespie.rs:160) */
+
+struct orc_net {
+  uint32_t S, R, cap;
+  int sparse;         /* Gillespie.sparse (gillespie.rs:162) */
+  /* Rate: LMA (dense, rate_sparse=0) holds S orders at rate_off[r]..+S;
+   *       LMASparse holds (idx, order) pairs rate_off[r]..rate_off[r+1]. */
+  int *rate_sparse;
+  double *k;
+  uint32_t *rate_off; /* R+1 */
+  uint32_t *rate_idx, *rate_ord;
+  uint32_t rate_len, rate_cap;
+  /* Jump: Flat holds S diffs at jump_off[r]; Sparse holds (idx, diff) pairs. */
+  int *jump_sparse;
+  uint32_t *jump_off; /* R+1 */
+  uint32_t *jump_idx;
+  int64_t *jump_d;
+  uint32_t jump_len, jump_cap;
+};
+
+orc_net *orc_net_new(uint32_t nb_species, int sparse) {
+  orc_net *n = (orc_net *)calloc(1, sizeof *n);
+  if (!n) return NULL;
+  n->S = nb_species;
+  n->sparse = sparse ? 1 : 0;
+  n->rate_off = (uint32_t *)calloc(1, sizeof(uint32_t));
+  n->jump_off = (uint32_t *)calloc(1, sizeof(uint32_t));
+  if (!n->rate_off || !n->jump_off) {
+    orc_net_free(n);
+    return NULL;
+  }
+  return n;
+}
+
+void orc_net_free(orc_net *n) {
+  if (!n) return;
+  free(n->rate_sparse);
+  free(n->k);
+  free(n->rate_off);
+  free(n->rate_idx);
+  free(n->rate_ord);
+  free(n->jump_sparse);
+  free(n->jump_off);
+  free(n->jump_idx);
+  free(n->jump_d);
+  free(n);
+}
+
+uint32_t orc_net_nb_species(const orc_net *n) { return n->S; }
+uint32_t orc_net_nb_reactions(const orc_net *n) { return n->R; }
+
+static int grow(void **p, size_t elem, uint32_t *cap, uint32_t need) {
+  if (need <= *cap) return 0;
+  uint32_t nc = *cap ? *cap : 16;
+  while (nc < need) nc *= 2;
+  void *q = realloc(*p, (size_t)nc * elem);
+  if (!q) return -1;
+  *p = q;
+  *cap = nc;
+  return 0;
+}
+
+static int push_rate_entry(orc_net *n, uint32_t idx, uint32_t ord) {
+  uint32_t c1 = n->rate_cap, c2 = n->rate_cap;
+  if (grow((void **)&n->rate_idx, sizeof(uint32_t), &c1, n->rate_len + 1)) return -1;
+  if (grow((void **)&n->rate_ord, sizeof(uint32_t), &c2, n->rate_len + 1)) return -1;
+  n->rate_cap = c1;
+  n->rate_idx[n->rate_len] = idx;
+  n->rate_ord[n->rate_len] = ord;
+  n->rate_len++;
+  return 0;
+}
+
+static int push_jump_entry(orc_net *n, uint32_t idx, int64_t d) {
+  uint32_t c1 = n->jump_cap, c2 = n->jump_cap;
+  if (grow((void **)&n->jump_idx, sizeof(uint32_t), &c1, n->jump_len + 1)) return -1;
+  if (grow((void **)&n->jump_d, sizeof(int64_t), &c2, n->jump_len + 1)) return -1;
+  n->jump_cap = c1;
+  n->jump_idx[n->jump_len] = idx;
+  n->jump_d[n->jump_len] = d;
+  n->jump_len++;
+  return 0;
+}
+
+/* Gillespie::add_reaction (gillespie.rs:225-244) with Rate::sparse (:55-69)
+ * and Jump::sparse (:126-140) applied when the net is sparse. */
+int orc_net_add_reaction(orc_net *n, int rate_sparse, double k, const uint32_t *idx,
+                         const uint32_t *orders, uint32_t cnt, const int64_t *diffs,
+                         uint32_t ndiffs) {
+  /* gillespie.rs:227-235: the rate must operate on existing species */
+  if (!rate_sparse) {
+    if (cnt != n->S) return ORC_ERR_SHAPE;
+  } else {
+    for (uint32_t i = 0; i < cnt; ++i)
+      if (idx[i] >= n->S) return ORC_ERR_INDEX;
+  }
+  /* gillespie.rs:237: the jump must have one entry per species */
+  if (ndiffs != n->S) return ORC_ERR_SHAPE;
+
+  /* every per-reaction array is sized for R + 2 entries (the offsets need R + 1) */
+  uint32_t r = n->R;
+  if (r + 2 > n->cap) {
+    uint32_t c1 = n->cap, c2 = n->cap, c3 = n->cap, c4 = n->cap, c5 = n->cap;
+    if (grow((void **)&n->rate_sparse, sizeof(int), &c1, r + 2) ||
+        grow((void **)&n->k, sizeof(double), &c2, r + 2) ||
+        grow((void **)&n->jump_sparse, sizeof(int), &c3, r + 2) ||
+        grow((void **)&n->rate_off, sizeof(uint32_t), &c4, r + 2) ||
+        grow((void **)&n->jump_off, sizeof(uint32_t), &c5, r + 2))
+      return ORC_ERR_ALLOC;
+    n->cap = c1;
+  }
+
+  n->k[r] = k;
+  if (!rate_sparse && !n->sparse) {
+    /* Rate::LMA kept dense: one order per species, zeros included */
+    n->rate_sparse[r] = 0;
+    for (uint32_t s = 0; s < cnt; ++s)
+      if (push_rate_entry(n, s, orders[s])) return ORC_ERR_ALLOC;
+  } else if (!rate_sparse) {
+    /* Rate::sparse: (index, exponent) for exponent > 0, ascending index */
+    n->rate_sparse[r] = 1;
+    for (uint32_t s = 0; s < cnt; ++s)
+      if (orders[s] > 0 && push_rate_entry(n, s, orders[s])) return ORC_ERR_ALLOC;
+  } else {
+    /* Rate::LMASparse as given: order and duplicates preserved */
+    n->rate_sparse[r] = 1;
+    for (uint32_t i = 0; i < cnt; ++i)
+      if (push_rate_entry(n, idx[i], orders[i])) return ORC_ERR_ALLOC;
+  }
+  n->rate_off[r + 1] = n->rate_len;
+
+  if (!n->sparse) {
+    n->jump_sparse[r] = 0;
+    for (uint32_t s = 0; s < ndiffs; ++s)
+      if (push_jump_entry(n, s, diffs[s])) return ORC_ERR_ALLOC;
+  } else {
+    /* Jump::sparse: (index, difference) for difference != 0 */
+    n->jump_sparse[r] = 1;
+    for (uint32_t s = 0; s < ndiffs; ++s)
+      if (diffs[s] != 0 && push_jump_entry(n, s, diffs[s])) return ORC_ERR_ALLOC;
+  }
+  n->jump_off[r + 1] = n->jump_len;
+  n->R = r + 1;
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Rate::rate (gillespie.rs:71-90).  Dense: fold over species ascending,
+ * each (n+1-e ..= n) multiplied in ascending order as f64.  Sparse: the same
+ * over the stored pairs, in stored order.  Both are the same loop here since
+ * the dense form stores every species in ascending order. */
+double orc_rate(const orc_net *net, uint32_t r, const int64_t *x) {
+  double acc = net->k[r];
+  for (uint32_t i = net->rate_off[r]; i < net->rate_off[r + 1]; ++i) {
+    const int64_t n = x[net->rate_idx[i]];
+    const int64_t e = (int64_t)net->rate_ord[i];
+    for (int64_t f = n + 1 - e; f <= n; ++f) acc = acc * (double)f;
+  }
+  return acc;
+}
+
+/* Jump::affect (gillespie.rs:142-152) */
+void orc_affect(const orc_net *net, uint32_t r, int64_t *x) {
+  for (uint32_t i = net->jump_off[r]; i < net->jump_off[r + 1]; ++i)
+    x[net->jump_idx[i]] += net->jump_d[i];
+}
+
+/* make_cumrates (gillespie.rs:357-364): cum[r] = total + rate_r; total = cum[r] */
+double orc_make_cumrates(const orc_net *net, const int64_t *x, double *cum) {
+  double total = 0.0;
+  for (uint32_t r = 0; r < net->R; ++r) {
+    cum[r] = total + orc_rate(net, r, x);
+    total = cum[r];
+  }
+  return total;
+}
+
+/* choose_cumrate_sum (gillespie.rs:402-407): #{r : cum[r] < chosen} */
+uint64_t orc_choose_cumrate_sum(double chosen, const double *cum, uint64_t n) {
+  uint64_t k = 0;
+  for (uint64_t r = 0; r < n; ++r) k += (cum[r] < chosen) ? 1u : 0u;
+  return k;
+}
+
+/* ------------------------------------------------------------------ */
+/* random sources */
+
+static inline uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+static uint64_t splitmix64(uint64_t *s) {
+  uint64_t z = (*s += 0x9e3779b97f4a7c15ull);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+static uint64_t xoshiro_next(uint64_t *s) {
+  const uint64_t result = rotl64(s[0] + s[3], 23) + s[0];
+  const uint64_t t = s[1] << 17;
+  s[2] ^= s[0];
+  s[3] ^= s[1];
+  s[1] ^= s[2];
+  s[0] ^= s[3];
+  s[2] ^= t;
+  s[3] = rotl64(s[3], 45);
+  return result;
+}
+
+void orc_state_init(orc_state *st, int64_t *x, uint64_t seed, uint64_t traj, int rng_kind) {
+  st->x = x;
+  st->t = 0.0;
+  st->seed = seed;
+  st->traj = traj;
+  st->draw = 0;
+  st->rng_kind = rng_kind;
+  st->events = 0;
+  uint64_t sm = seed ^ (traj * 0xd1342543de82ef95ull) ^ 0x5851f42d4c957f2dull;
+  for (int i = 0; i < 4; ++i) st->xs[i] = splitmix64(&sm);
+}
+
+/* one draw: the exponential, and (for Philox) the uniform of the same block */
+static inline double draw_exp1(orc_state *st, double *u_out) {
+  if (st->rng_kind == ORC_RNG_PHILOX) {
+    const rebop_u32x4 b = rebop_draw_block(st->seed, st->traj, st->draw);
+    st->draw++;
+    *u_out = rebop_unif_of(b);
+    return rebop_exp1_of(b);
+  }
+  const uint64_t a = xoshiro_next(st->xs);
+  const uint64_t c = xoshiro_next(st->xs);
+  st->draw++;
+  *u_out = (double)(c >> 11) * 0x1p-53;
+  return -log((double)((a >> 11) + 1) * 0x1p-53);
+}
+
+/* Gillespie::advance_until (gillespie.rs:315-344) */
+void orc_advance_until(const orc_net *net, orc_state *st, double tmax, double *cum) {
+  for (;;) {
+    const double total = orc_make_cumrates(net, st->x, cum);
+    if (!(0.0 < total)) { /* :323-326 absorbing: no draw */
+      st->t = tmax;
+      return;
+    }
+    double u;
+    const double e = draw_exp1(st, &u);
+    st->t = st->t + e / total; /* :327 */
+    if (st->t > tmax) {        /* :328-331 the draw is consumed */
+      st->t = tmax;
+      return;
+    }
+    const double chosen = total * u; /* :332 */
+    const uint64_t k = orc_choose_cumrate_sum(chosen, cum, net->R);
+    orc_affect(net, (uint32_t)k, st->x); /* :342 */
+    st->events++;
+  }
+}
+
+/* Gillespie::_advance_one_reaction (gillespie.rs:275-297) */
+void orc_advance_one_reaction(const orc_net *net, orc_state *st, double *cum) {
+  const double total = orc_make_cumrates(net, st->x, cum);
+  if (!(0.0 < total)) {
+    st->t = INFINITY;
+    return;
+  }
+  double u;
+  const double e = draw_exp1(st, &u);
+  st->t = st->t + e / total;
+  const double chosen = total * u;
+  const uint64_t k = orc_choose_cumrate_sum(chosen, cum, net->R);
+  orc_affect(net, (uint32_t)k, st->x);
+  st->events++;
+}
+
+/* pyo3_gillespie.rs:202 */
+double orc_grid_time(double tmax, uint64_t i, uint64_t nb_steps) {
+  return tmax * (double)i / (double)nb_steps;
+}
+
+/* ------------------------------------------------------------------ */
+/* the grid driver, pooled over threads */
+
+typedef struct {
+  const orc_net *net;
+  uint64_t seed, traj_begin, n_traj;
+  const int64_t *x0;
+  int x0_per_traj;
+  double tmax;
+  uint64_t nb_steps;
+  int64_t *out;
+  int64_t *final_x;
+  int rng_kind;
+  atomic_uint_fast64_t next;
+  atomic_uint_fast64_t events;
+  atomic_uint_fast64_t draws;
+  atomic_int err;
+} grid_job;
+
+#define ORC_CHUNK 64u
+
+static void *grid_worker(void *arg) {
+  grid_job *J = (grid_job *)arg;
+  const uint32_t S = J->net->S;
+  const uint32_t R = J->net->R;
+  int64_t *x = (int64_t *)malloc(sizeof(int64_t) * (S ? S : 1));
+  double *cum = (double *)malloc(sizeof(double) * (R ? R : 1));
+  if (!x || !cum) {
+    atomic_store(&J->err, ORC_ERR_ALLOC);
+    free(x);
+    free(cum);
+    return NULL;
+  }
+  uint64_t ev = 0, dr = 0;
+  for (;;) {
+    const uint64_t b = atomic_fetch_add(&J->next, ORC_CHUNK);
+    if (b >= J->n_traj) break;
+    const uint64_t e = b + ORC_CHUNK < J->n_traj ? b + ORC_CHUNK : J->n_traj;
+    for (uint64_t j = b; j < e; ++j) {
+      const int64_t *src = J->x0_per_traj ? J->x0 + j * S : J->x0;
+      memcpy(x, src, sizeof(int64_t) * S);
+      orc_state st;
+      orc_state_init(&st, x, J->seed, J->traj_begin + j, J->rng_kind);
+      for (uint64_t i = 0; i <= J->nb_steps; ++i) {
+        const double t = orc_grid_time(J->tmax, i, J->nb_steps);
+        orc_advance_until(J->net, &st, t, cum);
+        if (J->out)
+          for (uint32_t s = 0; s < S; ++s) J->out[(i * S + s) * J->n_traj + j] = x[s];
+      }
+      if (J->final_x) memcpy(J->final_x + j * S, x, sizeof(int64_t) * S);
+      ev += st.events;
+      dr += st.draw;
+    }
+  }
+  atomic_fetch_add(&J->events, ev);
+  atomic_fetch_add(&J->draws, dr);
+  free(x);
+  free(cum);
+  return NULL;
+}
+
+int orc_run_grid(const orc_net *net, uint64_t seed, uint64_t traj_begin, uint64_t n_traj,
+                 const int64_t *x0, int x0_per_traj, double tmax, uint64_t nb_steps,
+                 int64_t *out, int64_t *final_x, double *times, uint64_t *events,
+                 uint64_t *draws, int rng_kind, int nthreads) {
+  if (nb_steps == 0) return ORC_ERR_VALUE;
+  if (times)
+    for (uint64_t i = 0; i <= nb_steps; ++i) times[i] = orc_grid_time(tmax, i, nb_steps);
+  grid_job J;
+  J.net = net;
+  J.seed = seed;
+  J.traj_begin = traj_begin;
+  J.n_traj = n_traj;
+  J.x0 = x0;
+  J.x0_per_traj = x0_per_traj;
+  J.tmax = tmax;
+  J.nb_steps = nb_steps;
+  J.out = out;
+  J.final_x = final_x;
+  J.rng_kind = rng_kind;
+  atomic_init(&J.next, 0);
+  atomic_init(&J.events, 0);
+  atomic_init(&J.draws, 0);
+  atomic_init(&J.err, 0);
+  if (nthreads <= 1) {
+    grid_worker(&J);
+  } else {
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    if (!th) return ORC_ERR_ALLOC;
+    int started = 0;
+    for (int i = 0; i < nthreads; ++i) {
+      if (pthread_create(&th[i], NULL, grid_worker, &J) != 0) break;
+      started++;
+    }
+    if (started == 0) grid_worker(&J);
+    for (int i = 0; i < started; ++i) pthread_join(th[i], NULL);
+    free(th);
+  }
+  if (events) *events = atomic_load(&J.events);
+  if (draws) *draws = atomic_load(&J.draws);
+  return atomic_load(&J.err);
+}
+
+/* pyo3_gillespie.rs:209-223 for one trajectory */
+int orc_run_all_events(const orc_net *net, uint64_t seed, uint64_t traj, const int64_t *x0,
+                       double tmax, uint64_t cap, double *times, int64_t *xs,
+                       uint64_t *n_out, int rng_kind) {
+  const uint32_t S = net->S;
+  int64_t *x = (int64_t *)malloc(sizeof(int64_t) * (S ? S : 1));
+  double *cum = (double *)malloc(sizeof(double) * (net->R ? net->R : 1));
+  if (!x || !cum) {
+    free(x);
+    free(cum);
+    return ORC_ERR_ALLOC;
+  }
+  memcpy(x, x0, sizeof(int64_t) * S);
+  orc_state st;
+  orc_state_init(&st, x, seed, traj, rng_kind);
+  uint64_t n = 0;
+#define REC()                                                        \
+  do {                                                               \
+    if (n < cap) {                                                   \
+      times[n] = st.t;                                               \
+      for (uint32_t s = 0; s < S; ++s) xs[n * S + s] = x[s];         \
+    }                                                                \
+    n++;                                                             \
+  } while (0)
+  REC();
+  while (st.t < tmax) {
+    orc_advance_one_reaction(net, &st, cum);
+    REC();
+  }
+#undef REC
+  *n_out = n;
+  free(x);
+  free(cum);
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ */
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  const rebop_u32x4 r = rebop_philox4x32_10(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1]);
+  for (int i = 0; i < 4; ++i) out[i] = r.w[i];
+}
+
+double orc_ln64(double x) { return rebop_ln64(x); }
+double orc_u01_open_closed(uint32_t lo, uint32_t hi) { return rebop_u01_open_closed(lo, hi); }
+double orc_u01_closed_open(uint32_t lo, uint32_t hi) { return rebop_u01_closed_open(lo, hi); }
+
+int orc_hw_threads(void) {
+  long n = sysconf(_SC_NPROCESSORS_ONLN);
+  return n > 0 ? (int)n : 1;
+}

@@ -1,0 +1,121 @@
+/*
+ * oracle.h -- CPU restatement of rebop's run-time direct-method SSA.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (rebop_amd/, include/rebop_gpu.h) never links or calls it.
+ *
+ * It restates, function by function (reference paths relative to the rebop
+ * source tree):
+ *   Rate::rate            src/gillespie.rs:71-90    orc_rate()
+ *   Rate::sparse / dense  src/gillespie.rs:37-69    orc_net_add_reaction() packing
+ *   Jump::affect          src/gillespie.rs:142-152  orc_affect()
+ *   Gillespie::add_reaction asserts  :225-244       orc_net_add_reaction()
+ *   make_cumrates         src/gillespie.rs:357-364  orc_make_cumrates()
+ *   choose_cumrate_sum    src/gillespie.rs:402-407  orc_choose_cumrate_sum()
+ *   advance_until         src/gillespie.rs:315-344  orc_advance_until()
+ *   _advance_one_reaction src/gillespie.rs:275-297  orc_advance_one_reaction()
+ *   Gillespie.run grid    src/pyo3_gillespie.rs:200-208  orc_run_grid()
+ *   Gillespie.run events  src/pyo3_gillespie.rs:209-223  orc_run_all_events()
+ * with the random source swapped for the Philox stream of rebop_rng.h
+ * (rng kind ORC_RNG_PHILOX) or for an independent xoshiro256++ + libm log
+ * stream (ORC_RNG_XOSHIRO) used for two-sample statistical tests and as the
+ * second CPU-baseline variant.
+ */
+#ifndef REBOP_ORACLE_H
+#define REBOP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_OK = 0, ORC_ERR_SHAPE = 1, ORC_ERR_INDEX = 2, ORC_ERR_VALUE = 3, ORC_ERR_ALLOC = 4 };
+enum { ORC_RNG_PHILOX = 0, ORC_RNG_XOSHIRO = 1 };
+
+typedef struct orc_net orc_net;
+
+/* Gillespie::new(species, sparse) without the species vector (x0 is given per run). */
+orc_net *orc_net_new(uint32_t nb_species, int sparse);
+void orc_net_free(orc_net *net);
+uint32_t orc_net_nb_species(const orc_net *net);
+uint32_t orc_net_nb_reactions(const orc_net *net);
+
+/*
+ * add_reaction(Rate::lma(k, orders), diffs) when rate_sparse == 0
+ *   (orders has n entries, must equal nb_species), or
+ * add_reaction(Rate::lma_sparse(k, [(idx[i], orders[i])]), diffs) when rate_sparse == 1
+ *   (n pairs in the given order; every idx must be < nb_species).
+ * diffs has ndiffs entries, must equal nb_species.  If the net was created
+ * sparse, dense rates and the jump are converted as Rate::sparse / Jump::sparse do.
+ */
+int orc_net_add_reaction(orc_net *net, int rate_sparse, double k, const uint32_t *idx,
+                         const uint32_t *orders, uint32_t n, const int64_t *diffs,
+                         uint32_t ndiffs);
+
+/* Rate::rate of reaction r at counts x (for the known-answer tables). */
+double orc_rate(const orc_net *net, uint32_t r, const int64_t *x);
+/* make_cumrates: fills cum[R], returns the total. */
+double orc_make_cumrates(const orc_net *net, const int64_t *x, double *cum);
+/* choose_cumrate_sum over cum[0..n). */
+uint64_t orc_choose_cumrate_sum(double chosen, const double *cum, uint64_t n);
+/* Jump::affect of reaction r on x. */
+void orc_affect(const orc_net *net, uint32_t r, int64_t *x);
+
+/* One trajectory's resumable state (Gillespie's species, t and rng). */
+typedef struct {
+  int64_t *x;      /* [nb_species], caller-owned */
+  double t;
+  uint64_t seed;
+  uint64_t traj;
+  uint64_t draw;   /* Philox: number of blocks drawn so far */
+  uint64_t xs[4];  /* xoshiro256++ state */
+  int rng_kind;
+  uint64_t events; /* applied reactions */
+} orc_state;
+
+void orc_state_init(orc_state *st, int64_t *x, uint64_t seed, uint64_t traj, int rng_kind);
+void orc_advance_until(const orc_net *net, orc_state *st, double tmax, double *scratch_cum);
+void orc_advance_one_reaction(const orc_net *net, orc_state *st, double *scratch_cum);
+
+/*
+ * The pyo3 grid loop for trajectories traj_begin .. traj_begin+n_traj-1:
+ *   for i in 0..=nb_steps: t_i = tmax*i/nb_steps; advance_until(t_i); record.
+ * x0: [nb_species] shared (x0_per_traj == 0) or [n_traj][nb_species].
+ * out: [nb_steps+1][nb_species][n_traj] (trajectory fastest); may be NULL.
+ * final_x (nullable): [n_traj][nb_species] counts after the last grid point.
+ * times: [nb_steps+1] (nullable).  events (nullable): total applied reactions.
+ * nthreads <= 1 runs on the calling thread; otherwise a pool hands out
+ * chunks of 64 trajectories from an atomic counter.
+ */
+int orc_run_grid(const orc_net *net, uint64_t seed, uint64_t traj_begin, uint64_t n_traj,
+                 const int64_t *x0, int x0_per_traj, double tmax, uint64_t nb_steps,
+                 int64_t *out, int64_t *final_x, double *times, uint64_t *events,
+                 uint64_t *draws, int rng_kind, int nthreads);
+
+/*
+ * The pyo3 nb_steps == 0 loop for one trajectory: records (t, x) before the
+ * first event and after each event while t < tmax (an absorbing state sets
+ * t = +inf).  Writes at most cap records; *n_out receives the number of
+ * records the run produced (which may exceed cap: then the output is cut).
+ */
+int orc_run_all_events(const orc_net *net, uint64_t seed, uint64_t traj, const int64_t *x0,
+                       double tmax, uint64_t cap, double *times, int64_t *xs,
+                       uint64_t *n_out, int rng_kind);
+
+/* RNG primitives, exported so tests can pin them. */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double orc_ln64(double x);
+double orc_u01_open_closed(uint32_t lo, uint32_t hi);
+double orc_u01_closed_open(uint32_t lo, uint32_t hi);
+double orc_grid_time(double tmax, uint64_t i, uint64_t nb_steps);
+
+/* Number of hardware threads seen by the process. */
+int orc_hw_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,62 @@
+"""Builds the gfx950 HIP library ``rebop_amd/librebop_gpu.so`` in-tree.
+
+Usage: ``python -m rebop_amd.build`` (or ``build()`` from Python).  hipcc
+cross-compiles for gfx950 without a GPU present.
+"""
+
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+CSRC = HERE / "csrc"
+LIB = HERE / "librebop_gpu.so"
+ARCH = "gfx950"
+
+SOURCES = ["ssa_kernels.hip", "rebop_gpu.cpp"]
+
+# -ffp-contract=off: the SSA's FP64 arithmetic must round exactly as the
+# reference's separate multiplies and adds do (no fused multiply-add).
+# -packed-fp32-ops is disabled for every HIP compile of this project.
+FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-ffp-contract=off",
+    "-fno-fast-math",
+    "-fPIC",
+    "-Wall",
+    "-Xclang",
+    "-target-feature",
+    "-Xclang",
+    "-packed-fp32-ops",
+]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found; the rebop_amd HIP library cannot be built")
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile every HIP source into ``librebop_gpu.so`` (always recompiles)."""
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [hipcc(), *FLAGS, "-shared", "-o", str(tmp), *[str(CSRC / s) for s in SOURCES]]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print(LIB)

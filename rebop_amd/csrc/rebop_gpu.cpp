@@ -1,0 +1,574 @@
+// rebop_gpu.cpp -- host side of the C ABI declared in include/rebop_gpu.h.
+//
+// Owns the network description (the Rust `gillespie::Gillespie` fields,
+// src/gillespie.rs:157-163), packs it for the device (ssa_layout.h), chooses
+// a kernel and launches it.  All device work of one call is enqueued on one
+// HIP stream.
+#include "../../include/rebop_gpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "ssa_layout.h"
+
+namespace rebop {
+int reg_shape_count();
+int reg_shape_ms(int c);
+int reg_shape_mr(int c);
+hipError_t reg_occupancy(int c, int threads, size_t lds, int *blocks_per_cu);
+hipError_t launch_reset_counters(unsigned long long *counters, hipStream_t stream);
+hipError_t launch_reg(int c, const rebop_dev_net &net, const rebop_dev_run &run, int blocks,
+                      int threads, size_t lds, hipStream_t stream);
+} // namespace rebop
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(REBOP_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kThreads = 256;
+
+struct DeviceInfo {
+  bool probed = false;
+  int status = REBOP_OK;
+  std::string error;
+  int device = -1;
+  int cus = 0;
+  std::string name;
+};
+
+DeviceInfo g_dev;
+
+// The library runs only on gfx950; anything else is an error, never a
+// silent fallback.
+int ensure_device() {
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess) {
+    return fail(REBOP_ERR_DEVICE, "no HIP device available");
+  }
+  if (g_dev.probed && g_dev.device == cur) {
+    if (g_dev.status != REBOP_OK) return fail(g_dev.status, g_dev.error);
+    return REBOP_OK;
+  }
+  g_dev = DeviceInfo();
+  g_dev.probed = true;
+  g_dev.device = cur;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, cur) != hipSuccess) {
+    g_dev.status = REBOP_ERR_DEVICE;
+    g_dev.error = "hipGetDeviceProperties failed";
+    return fail(g_dev.status, g_dev.error);
+  }
+  g_dev.cus = prop.multiProcessorCount;
+  g_dev.name = prop.gcnArchName;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    g_dev.status = REBOP_ERR_DEVICE;
+    g_dev.error = std::string("device ") + prop.gcnArchName +
+                  " is not gfx950; this library carries gfx950 code only";
+    return fail(g_dev.status, g_dev.error);
+  }
+  return REBOP_OK;
+}
+
+struct Reaction {
+  double k;
+  std::vector<std::pair<uint32_t, uint32_t>> entries; // (species, order) in evaluation order
+  std::vector<int64_t> diff;                          // dense [S]
+};
+
+// device buffer owned by RAII
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  hipError_t ensure(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(&p, n ? n : 8);
+    if (e == hipSuccess) bytes = n ? n : 8;
+    return e;
+  }
+};
+
+} // namespace
+
+struct rebop_gillespie {
+  uint32_t S = 0;
+  bool sparse = false;
+  uint64_t seed = 0;
+  std::vector<int64_t> x0;
+  std::vector<Reaction> rx;
+  // packed device copy of the network, rebuilt when reactions change
+  bool dirty = true;
+  int dev_id = -1;
+  DevBuf dnet;
+  DevBuf dx0;
+  rebop_dev_net view{};
+  uint32_t F = 0;
+  DevBuf counters;
+};
+
+struct rebop_batch {
+  const rebop_gillespie *g = nullptr;
+  uint64_t traj_begin = 0, n = 0;
+  DevBuf x, t, draw;
+};
+
+namespace {
+
+int check_x0(const int64_t *x, size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    const int64_t v = x[i];
+    if (v > (int64_t)REBOP_EXACT_LIMIT || v < -(int64_t)REBOP_EXACT_LIMIT)
+      return fail(REBOP_ERR_VALUE, "initial count " + std::to_string(v) +
+                                       " is beyond the exactly simulated range |n| <= 2^52");
+  }
+  return REBOP_OK;
+}
+
+// Packs the network into one device buffer (layout in ssa_layout.h).
+int upload_net(rebop_gillespie *g) {
+  int cur = -1;
+  HIP_TRY(hipGetDevice(&cur));
+  if (!g->dirty && g->dev_id == cur) return REBOP_OK;
+  const uint32_t S = g->S, R = (uint32_t)g->rx.size();
+  std::vector<double> k(R);
+  std::vector<uint32_t> fb(R + 1, 0), fs;
+  std::vector<double> fc, diff((size_t)R * S);
+  for (uint32_t r = 0; r < R; ++r) {
+    const Reaction &x = g->rx[r];
+    k[r] = x.k;
+    for (auto &e : x.entries) {
+      // Rate::rate multiplies (n+1-e), ..., n: factor x[s] - c, c = e-1 .. 0
+      for (uint32_t j = 0; j < e.second; ++j) {
+        fs.push_back(e.first);
+        fc.push_back((double)(e.second - 1u - j));
+      }
+      if (fs.size() > (1u << 24))
+        return fail(REBOP_ERR_UNSUPPORTED, "more than 2^24 propensity factors in total");
+    }
+    fb[r + 1] = (uint32_t)fs.size();
+    for (uint32_t s = 0; s < S; ++s) diff[(size_t)r * S + s] = (double)x.diff[s];
+  }
+  const uint32_t F = (uint32_t)fs.size();
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t ok = 0;
+  const size_t ofb = ok + al(sizeof(double) * R);
+  const size_t ofs = ofb + al(sizeof(uint32_t) * (R + 1));
+  const size_t ofc = ofs + al(sizeof(uint32_t) * F);
+  const size_t odf = ofc + al(sizeof(double) * F);
+  const size_t total = odf + al(sizeof(double) * R * S);
+  std::vector<unsigned char> host(total, 0);
+  if (R) std::memcpy(host.data() + ok, k.data(), sizeof(double) * R);
+  std::memcpy(host.data() + ofb, fb.data(), sizeof(uint32_t) * (R + 1));
+  if (F) std::memcpy(host.data() + ofs, fs.data(), sizeof(uint32_t) * F);
+  if (F) std::memcpy(host.data() + ofc, fc.data(), sizeof(double) * F);
+  if (R * S) std::memcpy(host.data() + odf, diff.data(), sizeof(double) * R * S);
+  HIP_TRY(g->dnet.ensure(total));
+  HIP_TRY(hipMemcpy(g->dnet.p, host.data(), total, hipMemcpyHostToDevice));
+  HIP_TRY(g->dx0.ensure(sizeof(int64_t) * (S ? S : 1)));
+  if (S) HIP_TRY(hipMemcpy(g->dx0.p, g->x0.data(), sizeof(int64_t) * S, hipMemcpyHostToDevice));
+  HIP_TRY(g->counters.ensure(sizeof(unsigned long long) * REBOP_CTR_COUNT));
+  unsigned char *base = (unsigned char *)g->dnet.p;
+  g->view.S = S;
+  g->view.R = R;
+  g->view.F = F;
+  g->view.k = (const double *)(base + ok);
+  g->view.fac_begin = (const uint32_t *)(base + ofb);
+  g->view.fac_s = (const uint32_t *)(base + ofs);
+  g->view.fac_c = (const double *)(base + ofc);
+  g->view.diff = (const double *)(base + odf);
+  g->F = F;
+  g->dirty = false;
+  g->dev_id = cur;
+  return REBOP_OK;
+}
+
+int pick_reg_shape(uint32_t S, uint32_t R) {
+  for (int c = 0; c < rebop::reg_shape_count(); ++c)
+    if ((int)S <= rebop::reg_shape_ms(c) && (int)R <= rebop::reg_shape_mr(c)) return c;
+  return -1;
+}
+
+// Enqueues one SSA launch (plus the counter reset) on `stream`.  When
+// `stats` is set, synchronises and fills it.
+int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t stream,
+               rebop_stats *stats) {
+  int rc = ensure_device();
+  if (rc) return rc;
+  rc = upload_net(g);
+  if (rc) return rc;
+  const uint32_t S = g->S, R = (uint32_t)g->rx.size();
+  const int shape = pick_reg_shape(S, R);
+  if (shape < 0)
+    return fail(REBOP_ERR_UNSUPPORTED, "network with " + std::to_string(S) + " species and " +
+                                           std::to_string(R) +
+                                           " reactions exceeds the register kernels");
+  if (kernel == REBOP_KERNEL_LDS)
+    return fail(REBOP_ERR_UNSUPPORTED, "the LDS kernel is not built in this library");
+  const bool persistent = kernel != REBOP_KERNEL_PLAIN;
+  run.persistent = persistent ? 1 : 0;
+  run.counters = (unsigned long long *)g->counters.p;
+  const size_t lds = sizeof(double) * (size_t)R * S;
+  const uint64_t need_blocks = (run.n_traj + kThreads - 1) / kThreads;
+  uint64_t blocks = need_blocks;
+  if (persistent) {
+    int per_cu = 0;
+    HIP_TRY(rebop::reg_occupancy(shape, kThreads, lds, &per_cu));
+    if (per_cu < 1) per_cu = 1;
+    const uint64_t resident = (uint64_t)per_cu * (uint64_t)g_dev.cus;
+    if (blocks > resident) blocks = resident;
+  }
+  if (blocks == 0) blocks = 1;
+  if (blocks > 0x7fffffffull) return fail(REBOP_ERR_VALUE, "too many trajectories for one launch");
+
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  if (stats) {
+    HIP_TRY(hipEventCreate(&ev0));
+    HIP_TRY(hipEventCreate(&ev1));
+  }
+  HIP_TRY(rebop::launch_reset_counters(run.counters, stream));
+  if (stats) HIP_TRY(hipEventRecord(ev0, stream));
+  if (run.n_traj > 0)
+    HIP_TRY(rebop::launch_reg(shape, g->view, run, (int)blocks, kThreads, lds, stream));
+  if (stats) HIP_TRY(hipEventRecord(ev1, stream));
+
+  unsigned long long ctr[REBOP_CTR_COUNT] = {0};
+  if (stats) {
+    HIP_TRY(hipMemcpyAsync(ctr, run.counters, sizeof ctr, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    stats->events = ctr[REBOP_CTR_EVENTS];
+    stats->draws = ctr[REBOP_CTR_DRAWS];
+    stats->trajectories = ctr[REBOP_CTR_TRAJ];
+    stats->kernel = persistent ? REBOP_KERNEL_PERSISTENT : REBOP_KERNEL_PLAIN;
+    stats->kernel_ms = ms;
+    if (ctr[REBOP_CTR_FLAGS] & REBOP_FLAG_RANGE)
+      return fail(REBOP_ERR_RANGE, "a count left the exactly simulated range |n| <= 2^52");
+  }
+  return REBOP_OK;
+}
+
+int check_handle(const void *p, const char *what) {
+  if (!p) return fail(REBOP_ERR_VALUE, std::string(what) + " is NULL");
+  return REBOP_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *rebop_last_error(void) { return g_last_error.c_str(); }
+
+int rebop_device_info(char *name, size_t name_len, int *cu_count) {
+  int rc = ensure_device();
+  if (rc) return rc;
+  if (name && name_len) {
+    std::snprintf(name, name_len, "%s", g_dev.name.c_str());
+  }
+  if (cu_count) *cu_count = g_dev.cus;
+  return REBOP_OK;
+}
+
+int rebop_gillespie_new_with_seed(const int64_t *species, uint32_t nb_species, int sparse,
+                                  uint64_t seed, rebop_gillespie **out) {
+  if (!out) return fail(REBOP_ERR_VALUE, "out is NULL");
+  *out = nullptr;
+  if (nb_species && !species) return fail(REBOP_ERR_VALUE, "species is NULL");
+  rebop_gillespie *g = new (std::nothrow) rebop_gillespie();
+  if (!g) return fail(REBOP_ERR_ALLOC, "out of host memory");
+  g->S = nb_species;
+  g->sparse = sparse != 0;
+  g->seed = seed;
+  g->x0.assign(species, species + nb_species);
+  *out = g;
+  return REBOP_OK;
+}
+
+int rebop_gillespie_new(const int64_t *species, uint32_t nb_species, int sparse,
+                        rebop_gillespie **out) {
+  std::random_device rd;
+  const uint64_t seed = ((uint64_t)rd() << 32) ^ (uint64_t)rd();
+  return rebop_gillespie_new_with_seed(species, nb_species, sparse, seed, out);
+}
+
+void rebop_gillespie_free(rebop_gillespie *g) { delete g; }
+
+int rebop_gillespie_seed(rebop_gillespie *g, uint64_t seed) {
+  if (int rc = check_handle(g, "gillespie")) return rc;
+  g->seed = seed;
+  return REBOP_OK;
+}
+
+uint32_t rebop_gillespie_nb_species(const rebop_gillespie *g) { return g ? g->S : 0; }
+uint32_t rebop_gillespie_nb_reactions(const rebop_gillespie *g) {
+  return g ? (uint32_t)g->rx.size() : 0;
+}
+
+static int add_reaction_common(rebop_gillespie *g, double rate, bool rate_sparse,
+                               const uint32_t *idx, const uint32_t *orders, uint32_t n,
+                               const int64_t *diffs, uint32_t nd) {
+  if (int rc = check_handle(g, "gillespie")) return rc;
+  if (n && !orders) return fail(REBOP_ERR_VALUE, "orders is NULL");
+  if (rate_sparse && n && !idx) return fail(REBOP_ERR_VALUE, "species_idx is NULL");
+  if (nd && !diffs) return fail(REBOP_ERR_VALUE, "differences is NULL");
+  // src/gillespie.rs:227-235: the rate must operate on existing species
+  if (!rate_sparse) {
+    if (n != g->S)
+      return fail(REBOP_ERR_SHAPE, "assertion `left == right` failed: rate has " +
+                                       std::to_string(n) + " orders for " +
+                                       std::to_string(g->S) + " species");
+  } else {
+    for (uint32_t i = 0; i < n; ++i)
+      if (idx[i] >= g->S)
+        return fail(REBOP_ERR_INDEX, "assertion failed: species index " + std::to_string(idx[i]) +
+                                         " out of range for " + std::to_string(g->S) +
+                                         " species");
+  }
+  // src/gillespie.rs:237: one difference per species
+  if (nd != g->S)
+    return fail(REBOP_ERR_SHAPE, "assertion `left == right` failed: " + std::to_string(nd) +
+                                     " differences for " + std::to_string(g->S) + " species");
+  // an infinite constant makes E/total == 0 forever (time never advances);
+  // NaN is accepted: it makes the state absorbing, as in the reference
+  if (std::isinf(rate))
+    return fail(REBOP_ERR_VALUE, "rate constant must be finite (or NaN)");
+  for (uint32_t s = 0; s < nd; ++s)
+    if (diffs[s] > REBOP_DIFF_LIMIT || diffs[s] < -REBOP_DIFF_LIMIT)
+      return fail(REBOP_ERR_VALUE, "stoichiometric difference " + std::to_string(diffs[s]) +
+                                       " is beyond |d| <= 2^20");
+  Reaction r;
+  r.k = rate;
+  if (!rate_sparse) {
+    // dense LMA: ascending species; order 0 contributes no factor
+    for (uint32_t s = 0; s < n; ++s)
+      if (orders[s] > 0) r.entries.emplace_back(s, orders[s]);
+  } else {
+    // LMASparse: the given pair order, duplicates kept (Rate::rate :79-86);
+    // a sparse net converts dense rates to this same ascending list (Rate::sparse)
+    for (uint32_t i = 0; i < n; ++i)
+      if (orders[i] > 0) r.entries.emplace_back(idx[i], orders[i]);
+  }
+  r.diff.assign(diffs, diffs + nd);
+  g->rx.push_back(std::move(r));
+  g->dirty = true;
+  return REBOP_OK;
+}
+
+int rebop_gillespie_add_reaction_lma(rebop_gillespie *g, double rate, const uint32_t *orders,
+                                     uint32_t n_orders, const int64_t *differences,
+                                     uint32_t n_differences) {
+  return add_reaction_common(g, rate, false, nullptr, orders, n_orders, differences,
+                             n_differences);
+}
+
+int rebop_gillespie_add_reaction_lma_sparse(rebop_gillespie *g, double rate,
+                                            const uint32_t *species_idx, const uint32_t *orders,
+                                            uint32_t n_pairs, const int64_t *differences,
+                                            uint32_t n_differences) {
+  return add_reaction_common(g, rate, true, species_idx, orders, n_pairs, differences,
+                             n_differences);
+}
+
+int rebop_gillespie_set_species(rebop_gillespie *g, const int64_t *species, uint32_t n) {
+  if (int rc = check_handle(g, "gillespie")) return rc;
+  if (n != g->S)
+    return fail(REBOP_ERR_SHAPE, "assertion `left == right` failed: " + std::to_string(n) +
+                                     " counts for " + std::to_string(g->S) + " species");
+  if (n && !species) return fail(REBOP_ERR_VALUE, "species is NULL");
+  g->x0.assign(species, species + n);
+  g->dirty = true;
+  return REBOP_OK;
+}
+
+int rebop_gillespie_get_species(const rebop_gillespie *g, int64_t *species, uint32_t n) {
+  if (int rc = check_handle(g, "gillespie")) return rc;
+  if (n != g->S) return fail(REBOP_ERR_SHAPE, "length mismatch");
+  if (n) std::memcpy(species, g->x0.data(), sizeof(int64_t) * n);
+  return REBOP_OK;
+}
+
+int rebop_run_device(const rebop_gillespie *gc, double tmax, uint64_t nb_steps,
+                     uint64_t traj_begin, uint64_t n_traj, const int64_t *d_x0, int64_t *d_out,
+                     int64_t *d_final, int kernel, void *stream, rebop_stats *stats) {
+  if (int rc = check_handle(gc, "gillespie")) return rc;
+  rebop_gillespie *g = const_cast<rebop_gillespie *>(gc);
+  if (nb_steps == 0)
+    return fail(REBOP_ERR_VALUE, "nb_steps must be > 0 for a grid run");
+  if (nb_steps > (1ull << 40)) return fail(REBOP_ERR_VALUE, "nb_steps too large");
+  if (!d_x0) {
+    if (int rc = check_x0(g->x0.data(), g->x0.size())) return rc;
+  }
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  int rc = ensure_device();
+  if (rc) return rc;
+  rc = upload_net(g);
+  if (rc) return rc;
+  rebop_dev_run run{};
+  run.seed = g->seed;
+  run.traj_begin = traj_begin;
+  run.n_traj = n_traj;
+  run.tmax = tmax;
+  run.nb_steps = nb_steps;
+  run.mode = REBOP_MODE_GRID;
+  run.x0 = d_x0 ? d_x0 : (const int64_t *)g->dx0.p;
+  run.x0_per_traj = d_x0 ? 1 : 0;
+  run.out = d_out;
+  run.final_x = d_final;
+  return launch_run(g, run, kernel, (hipStream_t)stream, stats);
+}
+
+int rebop_run(const rebop_gillespie *gc, double tmax, uint64_t nb_steps, uint64_t traj_begin,
+              uint64_t n_traj, const int64_t *x0, int64_t *out, double *times,
+              rebop_stats *stats) {
+  if (int rc = check_handle(gc, "gillespie")) return rc;
+  if (nb_steps == 0) return fail(REBOP_ERR_VALUE, "nb_steps must be > 0 for a grid run");
+  const uint32_t S = gc->S;
+  if (x0) {
+    if (int rc = check_x0(x0, (size_t)n_traj * S)) return rc;
+  }
+  if (times)
+    for (uint64_t i = 0; i <= nb_steps; ++i) times[i] = tmax * (double)i / (double)nb_steps;
+  int rc = ensure_device();
+  if (rc) return rc;
+  DevBuf dx0, dout;
+  const size_t out_bytes = sizeof(int64_t) * (size_t)(nb_steps + 1) * S * n_traj;
+  if (x0 && n_traj && S) {
+    HIP_TRY(dx0.ensure(sizeof(int64_t) * (size_t)n_traj * S));
+    HIP_TRY(hipMemcpy(dx0.p, x0, sizeof(int64_t) * (size_t)n_traj * S, hipMemcpyHostToDevice));
+  }
+  if (out && out_bytes) HIP_TRY(dout.ensure(out_bytes));
+  rebop_stats local{};
+  rc = rebop_run_device(gc, tmax, nb_steps, traj_begin, n_traj,
+                        (x0 && n_traj && S) ? (const int64_t *)dx0.p : nullptr,
+                        (out && out_bytes) ? (int64_t *)dout.p : nullptr, nullptr,
+                        REBOP_KERNEL_AUTO, nullptr, &local);
+  if (rc) return rc;
+  if (out && out_bytes) HIP_TRY(hipMemcpy(out, dout.p, out_bytes, hipMemcpyDeviceToHost));
+  if (stats) *stats = local;
+  return REBOP_OK;
+}
+
+int rebop_batch_new(const rebop_gillespie *g, uint64_t traj_begin, uint64_t n_traj,
+                    const int64_t *x0, rebop_batch **out) {
+  if (!out) return fail(REBOP_ERR_VALUE, "out is NULL");
+  *out = nullptr;
+  if (int rc = check_handle(g, "gillespie")) return rc;
+  if (int rc = ensure_device()) return rc;
+  const uint32_t S = g->S;
+  const int64_t *src = x0 ? x0 : g->x0.data();
+  if (int rc = check_x0(src, x0 ? (size_t)n_traj * S : S)) return rc;
+  rebop_batch *b = new (std::nothrow) rebop_batch();
+  if (!b) return fail(REBOP_ERR_ALLOC, "out of host memory");
+  b->g = g;
+  b->traj_begin = traj_begin;
+  b->n = n_traj;
+  std::vector<double> hx((size_t)S * n_traj);
+  for (uint64_t j = 0; j < n_traj; ++j)
+    for (uint32_t s = 0; s < S; ++s)
+      hx[(size_t)s * n_traj + j] = (double)(x0 ? x0[j * S + s] : g->x0[s]);
+  std::vector<double> ht(n_traj, 0.0);
+  std::vector<uint64_t> hd(n_traj, 0);
+  hipError_t e = b->x.ensure(sizeof(double) * hx.size());
+  if (e == hipSuccess) e = b->t.ensure(sizeof(double) * n_traj);
+  if (e == hipSuccess) e = b->draw.ensure(sizeof(uint64_t) * n_traj);
+  if (e == hipSuccess && !hx.empty())
+    e = hipMemcpy(b->x.p, hx.data(), sizeof(double) * hx.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && n_traj)
+    e = hipMemcpy(b->t.p, ht.data(), sizeof(double) * n_traj, hipMemcpyHostToDevice);
+  if (e == hipSuccess && n_traj)
+    e = hipMemcpy(b->draw.p, hd.data(), sizeof(uint64_t) * n_traj, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    delete b;
+    return fail(REBOP_ERR_DEVICE, std::string("batch allocation: ") + hipGetErrorString(e));
+  }
+  *out = b;
+  return REBOP_OK;
+}
+
+void rebop_batch_free(rebop_batch *b) { delete b; }
+
+int rebop_batch_advance_until(rebop_batch *b, double tmax, rebop_stats *stats) {
+  if (int rc = check_handle(b, "batch")) return rc;
+  rebop_gillespie *g = const_cast<rebop_gillespie *>(b->g);
+  rebop_stats local{};
+  rebop_dev_run run{};
+  run.seed = g->seed;
+  run.traj_begin = b->traj_begin;
+  run.n_traj = b->n;
+  run.tmax = tmax;
+  run.nb_steps = 0;
+  run.mode = REBOP_MODE_TARGET;
+  run.st_x = (double *)b->x.p;
+  run.st_t = (double *)b->t.p;
+  run.st_draw = (uint64_t *)b->draw.p;
+  int rc = launch_run(g, run, REBOP_KERNEL_AUTO, nullptr, &local);
+  if (stats) *stats = local;
+  return rc;
+}
+
+int rebop_batch_get_species(const rebop_batch *b, int64_t *species) {
+  if (int rc = check_handle(b, "batch")) return rc;
+  const uint32_t S = b->g->S;
+  std::vector<double> hx((size_t)S * b->n);
+  if (!hx.empty())
+    HIP_TRY(hipMemcpy(hx.data(), b->x.p, sizeof(double) * hx.size(), hipMemcpyDeviceToHost));
+  for (uint64_t j = 0; j < b->n; ++j)
+    for (uint32_t s = 0; s < S; ++s) species[j * S + s] = (int64_t)hx[(size_t)s * b->n + j];
+  return REBOP_OK;
+}
+
+int rebop_batch_set_species(rebop_batch *b, const int64_t *species) {
+  if (int rc = check_handle(b, "batch")) return rc;
+  const uint32_t S = b->g->S;
+  if (int rc = check_x0(species, (size_t)S * b->n)) return rc;
+  std::vector<double> hx((size_t)S * b->n);
+  for (uint64_t j = 0; j < b->n; ++j)
+    for (uint32_t s = 0; s < S; ++s) hx[(size_t)s * b->n + j] = (double)species[j * S + s];
+  if (!hx.empty())
+    HIP_TRY(hipMemcpy(b->x.p, hx.data(), sizeof(double) * hx.size(), hipMemcpyHostToDevice));
+  return REBOP_OK;
+}
+
+int rebop_batch_get_time(const rebop_batch *b, double *t) {
+  if (int rc = check_handle(b, "batch")) return rc;
+  if (b->n) HIP_TRY(hipMemcpy(t, b->t.p, sizeof(double) * b->n, hipMemcpyDeviceToHost));
+  return REBOP_OK;
+}
+
+int rebop_batch_set_time(rebop_batch *b, const double *t) {
+  if (int rc = check_handle(b, "batch")) return rc;
+  if (b->n) HIP_TRY(hipMemcpy(b->t.p, t, sizeof(double) * b->n, hipMemcpyHostToDevice));
+  return REBOP_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,78 @@
+/*
+ * ssa_layout.h -- device-side layout of a packed reaction network and of one
+ * SSA launch, shared by the host library (rebop_gpu.cpp) and the kernels
+ * (ssa_kernels.hip).
+ *
+ * Packed network (one contiguous device buffer, read by every lane at
+ * wave-uniform addresses, so loads go through the scalar cache into SGPRs):
+ *   k[R]            LMA rate constant of reaction r
+ *   fac_begin[R+1]  factor list of reaction r is fac_begin[r] .. fac_begin[r+1]
+ *   fac_s[F]        species index of each factor
+ *   fac_c[F]        factor value is x[fac_s] - fac_c, as a double
+ *   diff[R][S]      net stoichiometry, as doubles (staged into LDS per workgroup)
+ * The factor list is Rate::rate's multiplication sequence (src/gillespie.rs:71-90):
+ * for each (species s, order e) in evaluation order, the factors
+ * (n+1-e), ..., n, i.e. c = e-1, ..., 0.  Evaluation order is ascending
+ * species for a dense Rate::LMA and the stored pair order for Rate::LMASparse.
+ *
+ * Counts are carried in FP64 registers.  Integers up to 2^53 are exact in
+ * binary64, so x - c, x + diff and the products see exactly the values the
+ * reference's int64 counts produce (its `x as f64` conversions are exact in
+ * that range).  The host rejects |x0| > 2^52 and |diff| > 2^20; the kernel
+ * checks |x| <= 2^52 at every sample and that fewer than 2^31 draws happened
+ * since the previous check (so no count can have passed 2^53 in between),
+ * and raises REBOP_FLAG_RANGE otherwise.
+ */
+#ifndef REBOP_SSA_LAYOUT_H
+#define REBOP_SSA_LAYOUT_H
+
+#include <stdint.h>
+
+#define REBOP_EXACT_LIMIT 4503599627370496.0 /* 2^52 */
+#define REBOP_DIFF_LIMIT (1ll << 20)
+#define REBOP_DRAWS_PER_CHECK (1ull << 31)
+
+/* error flags raised by kernels */
+#define REBOP_FLAG_RANGE 1u
+
+/* counters[] slots */
+#define REBOP_CTR_EVENTS 0
+#define REBOP_CTR_DRAWS 1
+#define REBOP_CTR_TRAJ 2
+#define REBOP_CTR_FLAGS 3
+#define REBOP_CTR_QUEUE 4
+#define REBOP_CTR_COUNT 8
+
+typedef struct {
+  uint32_t S, R, F, pad;
+  const double *k;
+  const uint32_t *fac_begin;
+  const uint32_t *fac_s;
+  const double *fac_c;
+  const double *diff;
+} rebop_dev_net;
+
+/* modes of one launch */
+#define REBOP_MODE_GRID 0   /* sample at tmax*i/nb_steps, i = 0..nb_steps */
+#define REBOP_MODE_TARGET 1 /* one advance_until(tmax) from the stored state */
+
+typedef struct {
+  uint64_t seed;
+  uint64_t traj_begin;
+  uint64_t n_traj;
+  double tmax;
+  uint64_t nb_steps;
+  int32_t mode;
+  int32_t persistent;
+  const int64_t *x0;   /* [S] shared (x0_per_traj == 0) or [n][S] */
+  int32_t x0_per_traj;
+  int32_t pad;
+  int64_t *out;        /* [nb+1][S][n] or NULL */
+  int64_t *final_x;    /* [S][n] or NULL */
+  double *st_x;        /* resumable state [S][n], or NULL */
+  double *st_t;        /* [n] */
+  uint64_t *st_draw;   /* [n] */
+  unsigned long long *counters; /* [REBOP_CTR_COUNT] */
+} rebop_dev_run;
+
+#endif

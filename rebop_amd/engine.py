@@ -1,0 +1,286 @@
+"""Low-level mirror of rebop's Rust run-time API over the device library.
+
+``Rate.lma`` / ``Rate.lma_sparse`` and ``GillespieNet`` follow
+``src/gillespie.rs`` (``Rate`` :10-69, ``Gillespie`` :157-344), batched over
+N independent trajectories: trajectory j of a run uses the Philox stream
+(seed, traj_begin + j).  ``Batch`` holds N resumable trajectory states and
+offers ``advance_until`` on all of them at once.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_int64, c_uint32, c_void_p
+from dataclasses import dataclass
+from typing import Sequence
+
+import numpy as np
+
+from . import _native
+from ._native import Stats, check, load
+
+
+def _i64(a: np.ndarray):
+    return a.ctypes.data_as(POINTER(c_int64))
+
+
+def _u32(a: np.ndarray):
+    return a.ctypes.data_as(POINTER(c_uint32))
+
+
+def _f64(a: np.ndarray):
+    return a.ctypes.data_as(POINTER(c_double))
+
+
+@dataclass(frozen=True)
+class Rate:
+    """An LMA rate: ``Rate::lma`` (dense orders) or ``Rate::lma_sparse`` (pairs)."""
+
+    rate: float
+    orders: tuple
+    sparse: bool = False
+
+    @staticmethod
+    def lma(rate: float, orders: Sequence[int]) -> "Rate":
+        """``Rate::lma(rate, orders)`` -- src/gillespie.rs:24-26."""
+        return Rate(float(rate), tuple(int(o) for o in orders), False)
+
+    @staticmethod
+    def lma_sparse(rate: float, pairs: Sequence[tuple[int, int]]) -> "Rate":
+        """``Rate::lma_sparse(rate, [(species, order)])`` -- src/gillespie.rs:29-31."""
+        return Rate(float(rate), tuple((int(i), int(o)) for i, o in pairs), True)
+
+    def dense(self) -> "Rate":
+        """``Rate::dense`` (src/gillespie.rs:37-53): length max index + 1, orders summed."""
+        if not self.sparse:
+            return self
+        n = max((i + 1 for i, _ in self.orders), default=0)
+        d = [0] * n
+        for i, o in self.orders:
+            d[i] += o
+        return Rate(self.rate, tuple(d), False)
+
+    def sparse_form(self) -> "Rate":
+        """``Rate::sparse`` (src/gillespie.rs:55-69): (index, order) for order > 0, ascending."""
+        if self.sparse:
+            return self
+        return Rate(self.rate, tuple((i, o) for i, o in enumerate(self.orders) if o > 0), True)
+
+
+@dataclass(frozen=True)
+class Jump:
+    """Net stoichiometry of a reaction: ``Jump::Flat`` or ``Jump::Sparse`` (src/gillespie.rs:93-100)."""
+
+    diffs: tuple
+    sparse: bool = False
+
+    @staticmethod
+    def new(differences: Sequence[int]) -> "Jump":
+        return Jump(tuple(int(d) for d in differences), False)
+
+    @staticmethod
+    def new_sparse(pairs: Sequence[tuple[int, int]]) -> "Jump":
+        return Jump(tuple((int(i), int(d)) for i, d in pairs), True)
+
+    def dense(self) -> "Jump":
+        """``Jump::dense`` (src/gillespie.rs:112-124)."""
+        if not self.sparse:
+            return self
+        n = max((i + 1 for i, _ in self.diffs), default=0)
+        d = [0] * n
+        for i, v in self.diffs:
+            d[i] += v
+        return Jump(tuple(d), False)
+
+    def sparse_form(self) -> "Jump":
+        """``Jump::sparse`` (src/gillespie.rs:126-140): (index, diff) for diff != 0."""
+        if self.sparse:
+            return self
+        return Jump(tuple((i, d) for i, d in enumerate(self.diffs) if d != 0), True)
+
+
+class GillespieNet:
+    """``gillespie::Gillespie`` (src/gillespie.rs:157-163) on the device.
+
+    ``species`` is the initial state every trajectory starts from unless a
+    per-trajectory ``x0`` is passed to ``run``/``batch``.
+    """
+
+    def __init__(self, species: Sequence[int], sparse: bool = False, seed: int | None = None):
+        lib = load()
+        x = np.ascontiguousarray(np.asarray(species, dtype=np.int64).reshape(-1))
+        h = c_void_p()
+        if seed is None:
+            check(lib.rebop_gillespie_new(_i64(x), len(x), int(bool(sparse)), ctypes.byref(h)))
+        else:
+            check(
+                lib.rebop_gillespie_new_with_seed(
+                    _i64(x), len(x), int(bool(sparse)), int(seed) & (2**64 - 1), ctypes.byref(h)
+                )
+            )
+        self._h = h
+        self._lib = lib
+
+    @classmethod
+    def new_with_seed(cls, species: Sequence[int], sparse: bool, seed: int) -> "GillespieNet":
+        """``Gillespie::new_with_seed`` -- src/gillespie.rs:179-187."""
+        return cls(species, sparse, seed)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.rebop_gillespie_free(h)
+            self._h = None
+
+    @property
+    def handle(self) -> c_void_p:
+        return self._h
+
+    def seed(self, seed: int) -> None:
+        """``Gillespie::seed`` -- src/gillespie.rs:189-191."""
+        check(self._lib.rebop_gillespie_seed(self._h, int(seed) & (2**64 - 1)))
+
+    def nb_species(self) -> int:
+        return int(self._lib.rebop_gillespie_nb_species(self._h))
+
+    def nb_reactions(self) -> int:
+        return int(self._lib.rebop_gillespie_nb_reactions(self._h))
+
+    def add_reaction(self, rate: Rate, differences: Sequence[int]) -> None:
+        """``Gillespie::add_reaction`` -- src/gillespie.rs:225-244."""
+        d = np.ascontiguousarray(np.asarray(differences, dtype=np.int64).reshape(-1))
+        if rate.sparse:
+            idx = np.ascontiguousarray([p[0] for p in rate.orders], dtype=np.uint32)
+            orde = np.ascontiguousarray([p[1] for p in rate.orders], dtype=np.uint32)
+            check(
+                self._lib.rebop_gillespie_add_reaction_lma_sparse(
+                    self._h, rate.rate, _u32(idx), _u32(orde), len(idx), _i64(d), len(d)
+                )
+            )
+        else:
+            orde = np.ascontiguousarray(rate.orders, dtype=np.uint32)
+            check(
+                self._lib.rebop_gillespie_add_reaction_lma(
+                    self._h, rate.rate, _u32(orde), len(orde), _i64(d), len(d)
+                )
+            )
+
+    def get_species(self) -> np.ndarray:
+        out = np.zeros(self.nb_species(), dtype=np.int64)
+        check(self._lib.rebop_gillespie_get_species(self._h, _i64(out), len(out)))
+        return out
+
+    def set_species(self, species: Sequence[int]) -> None:
+        x = np.ascontiguousarray(np.asarray(species, dtype=np.int64).reshape(-1))
+        check(self._lib.rebop_gillespie_set_species(self._h, _i64(x), len(x)))
+
+    def run(
+        self,
+        tmax: float,
+        nb_steps: int,
+        n_traj: int = 1,
+        traj_begin: int = 0,
+        x0: np.ndarray | None = None,
+        with_stats: bool = False,
+    ):
+        """The pyo3 grid loop (src/pyo3_gillespie.rs:197-208) for ``n_traj`` trajectories.
+
+        Returns ``(times[nb_steps+1], counts[nb_steps+1, S, n_traj])`` (plus a
+        ``Stats`` if ``with_stats``).  Host buffers.
+        """
+        S = self.nb_species()
+        nb = int(nb_steps)
+        out = np.zeros((nb + 1, S, int(n_traj)), dtype=np.int64)
+        times = np.zeros(nb + 1, dtype=np.float64)
+        st = Stats()
+        xp = None
+        if x0 is not None:
+            x0 = np.ascontiguousarray(np.asarray(x0, dtype=np.int64).reshape(int(n_traj), S))
+            xp = _i64(x0)
+        check(
+            self._lib.rebop_run(
+                self._h, float(tmax), nb, int(traj_begin), int(n_traj), xp, _i64(out),
+                _f64(times), ctypes.byref(st),
+            )
+        )
+        if with_stats:
+            return times, out, st
+        return times, out
+
+    def run_device(
+        self,
+        tmax: float,
+        nb_steps: int,
+        n_traj: int,
+        out_ptr: int | None,
+        traj_begin: int = 0,
+        x0_ptr: int | None = None,
+        final_ptr: int | None = None,
+        kernel: int = _native.REBOP_KERNEL_AUTO,
+        stream: int | None = None,
+        with_stats: bool = True,
+    ) -> Stats | None:
+        """``rebop_run_device``: device pointers (e.g. ``torch.Tensor.data_ptr()``)."""
+        st = Stats()
+        check(
+            self._lib.rebop_run_device(
+                self._h, float(tmax), int(nb_steps), int(traj_begin), int(n_traj),
+                c_void_p(x0_ptr) if x0_ptr else None,
+                c_void_p(out_ptr) if out_ptr else None,
+                c_void_p(final_ptr) if final_ptr else None,
+                int(kernel),
+                c_void_p(stream) if stream else None,
+                ctypes.byref(st) if with_stats else None,
+            )
+        )
+        return st if with_stats else None
+
+    def batch(self, n_traj: int, traj_begin: int = 0, x0: np.ndarray | None = None) -> "Batch":
+        return Batch(self, n_traj, traj_begin, x0)
+
+
+class Batch:
+    """N resumable trajectories of one network (device-resident state)."""
+
+    def __init__(self, net: GillespieNet, n_traj: int, traj_begin: int = 0, x0=None):
+        self._net = net
+        self._lib = net._lib
+        self.n = int(n_traj)
+        self.S = net.nb_species()
+        h = c_void_p()
+        xp = None
+        if x0 is not None:
+            x0 = np.ascontiguousarray(np.asarray(x0, dtype=np.int64).reshape(self.n, self.S))
+            xp = _i64(x0)
+        check(self._lib.rebop_batch_new(net.handle, int(traj_begin), self.n, xp, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.rebop_batch_free(h)
+            self._h = None
+
+    def advance_until(self, tmax: float) -> Stats:
+        """``Gillespie::advance_until`` on every trajectory -- src/gillespie.rs:315-344."""
+        st = Stats()
+        check(self._lib.rebop_batch_advance_until(self._h, float(tmax), ctypes.byref(st)))
+        return st
+
+    def get_species(self) -> np.ndarray:
+        out = np.zeros((self.n, self.S), dtype=np.int64)
+        check(self._lib.rebop_batch_get_species(self._h, _i64(out)))
+        return out
+
+    def set_species(self, species) -> None:
+        x = np.ascontiguousarray(np.asarray(species, dtype=np.int64).reshape(self.n, self.S))
+        check(self._lib.rebop_batch_set_species(self._h, _i64(x)))
+
+    def get_time(self) -> np.ndarray:
+        out = np.zeros(self.n, dtype=np.float64)
+        check(self._lib.rebop_batch_get_time(self._h, _f64(out)))
+        return out
+
+    def set_time(self, t) -> None:
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(t, dtype=np.float64), (self.n,)))
+        check(self._lib.rebop_batch_set_time(self._h, _f64(a)))

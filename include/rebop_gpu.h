@@ -56,7 +56,10 @@ enum rebop_kernel_kind {
   REBOP_KERNEL_AUTO = 0,
   REBOP_KERNEL_PLAIN = 1,      /* one trajectory per lane, lanes retire when done */
   REBOP_KERNEL_PERSISTENT = 2, /* lanes refill from an atomic trajectory queue */
-  REBOP_KERNEL_LDS = 3         /* counts in LDS, for networks beyond the register kernels */
+  REBOP_KERNEL_LDS = 3,        /* counts in LDS, for networks beyond the register kernels */
+  /* flag: the network compiled into the kernel (run-time compilation,
+   * cached); combine with PLAIN or PERSISTENT */
+  REBOP_KERNEL_SPECIALIZED = 16
 };
 
 /* Opaque network + initial state: rebop's `gillespie::Gillespie` (src/gillespie.rs:157-163). */
@@ -94,6 +97,12 @@ int rebop_gillespie_add_reaction_lma_sparse(rebop_gillespie *g, double rate,
                                             const uint32_t *species_idx, const uint32_t *orders,
                                             uint32_t n_pairs, const int64_t *differences,
                                             uint32_t n_differences);
+
+/* Compiles and caches the kernel `kernel` (REBOP_KERNEL_*) will use for this
+ * network, so that the first run does not pay for it.  Without a device it
+ * only compiles, which checks the specialised kernel for gfx950.  No
+ * counterpart in the reference (which has no device code to prepare). */
+int rebop_gillespie_prepare(const rebop_gillespie *g, int kernel);
 
 /* Gillespie::set_species / get_species on the initial state -- src/gillespie.rs:260-267. */
 int rebop_gillespie_set_species(rebop_gillespie *g, const int64_t *species, uint32_t n);

@@ -27,6 +27,7 @@ REBOP_KERNEL_AUTO = 0
 REBOP_KERNEL_PLAIN = 1
 REBOP_KERNEL_PERSISTENT = 2
 REBOP_KERNEL_LDS = 3
+REBOP_KERNEL_SPECIALIZED = 16
 
 # every entry point of include/rebop_gpu.h: name -> (restype, argtypes)
 SIGNATURES: dict[str, tuple] = {
@@ -47,6 +48,7 @@ SIGNATURES: dict[str, tuple] = {
         c_int,
         [c_void_p, c_double, POINTER(c_uint32), POINTER(c_uint32), c_uint32, POINTER(c_int64), c_uint32],
     ),
+    "rebop_gillespie_prepare": (c_int, [c_void_p, c_int]),
     "rebop_gillespie_set_species": (c_int, [c_void_p, POINTER(c_int64), c_uint32]),
     "rebop_gillespie_get_species": (c_int, [c_void_p, POINTER(c_int64), c_uint32]),
     "rebop_run": (

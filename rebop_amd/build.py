@@ -10,6 +10,7 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
@@ -17,7 +18,10 @@ CSRC = HERE / "csrc"
 LIB = HERE / "librebop_gpu.so"
 ARCH = "gfx950"
 
-SOURCES = ["ssa_kernels.hip", "rebop_gpu.cpp"]
+SOURCES = ["ssa_kernels.hip", "rebop_gpu.cpp", "jit.cpp"]
+# headers whose text is embedded in the library for run-time compilation
+EMBEDDED_HEADERS = ["rebop_rng.h", "ssa_layout.h", "ssa_lane.h"]
+DELIM = "REBOPSRC"
 
 # -ffp-contract=off: the SSA's FP64 arithmetic must round exactly as the
 # reference's separate multiplies and adds do (no fused multiply-add).
@@ -44,13 +48,36 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found; the rebop_amd HIP library cannot be built")
 
 
+def embedded_headers_cpp() -> str:
+    """C++ source defining the text of the headers the run-time compiler needs."""
+    names, bodies = [], []
+    for h in EMBEDDED_HEADERS:
+        text = (CSRC / h).read_text()
+        if f"){DELIM}\"" in text:
+            raise RuntimeError(f"{h} contains the raw-string delimiter")
+        names.append(f'"{h}"')
+        bodies.append(f'R"{DELIM}({text}){DELIM}"')
+    sources = ",\n".join(bodies)
+    return (
+        "namespace rebop {\n"
+        f"extern const int kEmbeddedHeaderCount = {len(names)};\n"
+        f"extern const char *const kEmbeddedHeaderNames[] = {{{', '.join(names)}}};\n"
+        f"extern const char *const kEmbeddedHeaderSources[] = {{\n{sources}\n}};\n"
+        "}\n"
+    )
+
+
 def build(verbose: bool = False) -> Path:
     """Compile every HIP source into ``librebop_gpu.so`` (always recompiles)."""
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *FLAGS, "-shared", "-o", str(tmp), *[str(CSRC / s) for s in SOURCES]]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
+    with tempfile.TemporaryDirectory() as td:
+        emb = Path(td) / "embedded_headers.cpp"
+        emb.write_text(embedded_headers_cpp())
+        cmd = [hipcc(), *FLAGS, "-shared", "-o", str(tmp), *[str(CSRC / s) for s in SOURCES],
+               str(emb), "-lhiprtc"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <random>
@@ -26,6 +27,22 @@ hipError_t reg_occupancy(int c, int threads, size_t lds, int *blocks_per_cu);
 hipError_t launch_reset_counters(unsigned long long *counters, hipStream_t stream);
 hipError_t launch_reg(int c, const rebop_dev_net &net, const rebop_dev_run &run, int blocks,
                       int threads, size_t lds, hipStream_t stream);
+
+struct JitReaction {
+  double k;
+  std::vector<std::pair<uint32_t, uint32_t>> factors;
+  std::vector<int64_t> diff;
+};
+struct JitKernel {
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+};
+std::string jit_source(uint32_t S, const std::vector<JitReaction> &rx);
+int jit_get(const std::string &src, JitKernel *out, std::string *err);
+int jit_compile_only(const std::string &src, size_t *code_size, std::string *err);
+hipError_t jit_occupancy(const JitKernel &k, int threads, int *blocks_per_cu);
+hipError_t jit_launch(const JitKernel &k, const rebop_dev_run &run, int blocks, int threads,
+                      hipStream_t stream);
 } // namespace rebop
 
 namespace {
@@ -132,6 +149,12 @@ struct rebop_gillespie {
   rebop_dev_net view{};
   uint32_t F = 0;
   DevBuf counters;
+  // network-specialised kernel (run-time compiled), rebuilt when reactions change
+  bool jit_dirty = true;
+  std::string jit_src;
+  int jit_dev = -1;
+  rebop::JitKernel jit{};
+  std::string jit_err;
 };
 
 struct rebop_batch {
@@ -216,6 +239,58 @@ int pick_reg_shape(uint32_t S, uint32_t R) {
   return -1;
 }
 
+// Networks small enough to compile into a kernel (registers hold x[S] and cum[R]).
+bool jit_eligible(const rebop_gillespie *g) {
+  if (g->S > 32 || g->rx.size() > 64) return false;
+  size_t F = 0;
+  for (auto &r : g->rx)
+    for (auto &e : r.entries) F += e.second;
+  return F <= 4096;
+}
+
+bool jit_disabled() {
+  const char *v = std::getenv("REBOP_NO_SPECIALIZE");
+  return v && *v && std::strcmp(v, "0") != 0;
+}
+
+std::string jit_source_of(const rebop_gillespie *g) {
+  std::vector<rebop::JitReaction> jr;
+  jr.reserve(g->rx.size());
+  for (auto &r : g->rx) {
+    rebop::JitReaction j;
+    j.k = r.k;
+    for (auto &e : r.entries)
+      for (uint32_t q = 0; q < e.second; ++q) j.factors.emplace_back(e.first, e.second - 1u - q);
+    j.diff = r.diff;
+    jr.push_back(std::move(j));
+  }
+  std::string src = rebop::jit_source(g->S, jr);
+  // REBOP_DUMP_SPECIALIZED=<file>: write the generated kernel source there
+  if (const char *path = std::getenv("REBOP_DUMP_SPECIALIZED")) {
+    if (FILE *f = std::fopen(path, "w")) {
+      std::fputs(src.c_str(), f);
+      std::fclose(f);
+    }
+  }
+  return src;
+}
+
+// Compiles (cached) and loads the network-specialised kernel on the current device.
+int prepare_jit(rebop_gillespie *g) {
+  int cur = -1;
+  HIP_TRY(hipGetDevice(&cur));
+  if (!g->jit_dirty && g->jit_dev == cur && g->jit.fn) return REBOP_OK;
+  g->jit_src = jit_source_of(g);
+  std::string err;
+  if (rebop::jit_get(g->jit_src, &g->jit, &err) != 0) {
+    g->jit_err = err;
+    return fail(REBOP_ERR_DEVICE, "specialised kernel: " + err);
+  }
+  g->jit_dirty = false;
+  g->jit_dev = cur;
+  return REBOP_OK;
+}
+
 // Enqueues one SSA launch (plus the counter reset) on `stream`.  When
 // `stats` is set, synchronises and fills it.
 int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t stream,
@@ -225,28 +300,44 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
   rc = upload_net(g);
   if (rc) return rc;
   const uint32_t S = g->S, R = (uint32_t)g->rx.size();
-  const int shape = pick_reg_shape(S, R);
-  if (shape < 0)
+  const int variant = kernel & 0xF;
+  if (variant == REBOP_KERNEL_LDS)
+    return fail(REBOP_ERR_UNSUPPORTED, "the LDS kernel is not built in this library");
+  bool spec = (kernel & REBOP_KERNEL_SPECIALIZED) != 0;
+  if (variant == REBOP_KERNEL_AUTO && !(kernel & REBOP_KERNEL_SPECIALIZED))
+    spec = jit_eligible(g) && !jit_disabled();
+  const bool persistent = variant != REBOP_KERNEL_PLAIN;
+  if (spec) {
+    rc = prepare_jit(g);
+    if (rc) {
+      if (kernel & REBOP_KERNEL_SPECIALIZED) return rc;
+      spec = false; // AUTO: the data-driven kernel runs instead
+    }
+  }
+  const int shape = spec ? -1 : pick_reg_shape(S, R);
+  if (!spec && shape < 0)
     return fail(REBOP_ERR_UNSUPPORTED, "network with " + std::to_string(S) + " species and " +
                                            std::to_string(R) +
                                            " reactions exceeds the register kernels");
-  if (kernel == REBOP_KERNEL_LDS)
-    return fail(REBOP_ERR_UNSUPPORTED, "the LDS kernel is not built in this library");
-  const bool persistent = kernel != REBOP_KERNEL_PLAIN;
   run.persistent = persistent ? 1 : 0;
   run.counters = (unsigned long long *)g->counters.p;
-  const size_t lds = sizeof(double) * (size_t)R * S;
+  const size_t lds = spec ? 0 : sizeof(double) * (size_t)R * S;
   const uint64_t need_blocks = (run.n_traj + kThreads - 1) / kThreads;
   uint64_t blocks = need_blocks;
   if (persistent) {
     int per_cu = 0;
-    HIP_TRY(rebop::reg_occupancy(shape, kThreads, lds, &per_cu));
+    if (spec)
+      HIP_TRY(rebop::jit_occupancy(g->jit, kThreads, &per_cu));
+    else
+      HIP_TRY(rebop::reg_occupancy(shape, kThreads, lds, &per_cu));
     if (per_cu < 1) per_cu = 1;
     const uint64_t resident = (uint64_t)per_cu * (uint64_t)g_dev.cus;
     if (blocks > resident) blocks = resident;
   }
   if (blocks == 0) blocks = 1;
   if (blocks > 0x7fffffffull) return fail(REBOP_ERR_VALUE, "too many trajectories for one launch");
+  const int kind = (persistent ? REBOP_KERNEL_PERSISTENT : REBOP_KERNEL_PLAIN) |
+                   (spec ? REBOP_KERNEL_SPECIALIZED : 0);
 
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (stats) {
@@ -255,8 +346,12 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
   }
   HIP_TRY(rebop::launch_reset_counters(run.counters, stream));
   if (stats) HIP_TRY(hipEventRecord(ev0, stream));
-  if (run.n_traj > 0)
-    HIP_TRY(rebop::launch_reg(shape, g->view, run, (int)blocks, kThreads, lds, stream));
+  if (run.n_traj > 0) {
+    if (spec)
+      HIP_TRY(rebop::jit_launch(g->jit, run, (int)blocks, kThreads, stream));
+    else
+      HIP_TRY(rebop::launch_reg(shape, g->view, run, (int)blocks, kThreads, lds, stream));
+  }
   if (stats) HIP_TRY(hipEventRecord(ev1, stream));
 
   unsigned long long ctr[REBOP_CTR_COUNT] = {0};
@@ -270,7 +365,7 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
     stats->events = ctr[REBOP_CTR_EVENTS];
     stats->draws = ctr[REBOP_CTR_DRAWS];
     stats->trajectories = ctr[REBOP_CTR_TRAJ];
-    stats->kernel = persistent ? REBOP_KERNEL_PERSISTENT : REBOP_KERNEL_PLAIN;
+    stats->kernel = (uint64_t)kind;
     stats->kernel_ms = ms;
     if (ctr[REBOP_CTR_FLAGS] & REBOP_FLAG_RANGE)
       return fail(REBOP_ERR_RANGE, "a count left the exactly simulated range |n| <= 2^52");
@@ -381,6 +476,7 @@ static int add_reaction_common(rebop_gillespie *g, double rate, bool rate_sparse
   r.diff.assign(diffs, diffs + nd);
   g->rx.push_back(std::move(r));
   g->dirty = true;
+  g->jit_dirty = true;
   return REBOP_OK;
 }
 
@@ -415,6 +511,25 @@ int rebop_gillespie_get_species(const rebop_gillespie *g, int64_t *species, uint
   if (n != g->S) return fail(REBOP_ERR_SHAPE, "length mismatch");
   if (n) std::memcpy(species, g->x0.data(), sizeof(int64_t) * n);
   return REBOP_OK;
+}
+
+int rebop_gillespie_prepare(const rebop_gillespie *gc, int kernel) {
+  if (int rc = check_handle(gc, "gillespie")) return rc;
+  rebop_gillespie *g = const_cast<rebop_gillespie *>(gc);
+  if (!(kernel & REBOP_KERNEL_SPECIALIZED)) return REBOP_OK;
+  if (!jit_eligible(g))
+    return fail(REBOP_ERR_UNSUPPORTED, "network too large for a specialised kernel");
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess) {
+    // no device: compile only, to check the generated kernel
+    std::string err;
+    size_t cs = 0;
+    if (rebop::jit_compile_only(jit_source_of(g), &cs, &err) != 0)
+      return fail(REBOP_ERR_DEVICE, "specialised kernel: " + err);
+    return REBOP_OK;
+  }
+  if (int rc = ensure_device()) return rc;
+  return prepare_jit(g);
 }
 
 int rebop_run_device(const rebop_gillespie *gc, double tmax, uint64_t nb_steps,

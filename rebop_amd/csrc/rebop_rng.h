@@ -60,6 +60,13 @@ typedef struct {
   uint32_t w[4];
 } rebop_u32x4;
 
+/* a ^ b ^ c: one v_bitop3_b32 (truth table 0x96) in gfx950 device code */
+#if defined(__HIP_DEVICE_COMPILE__) && defined(__gfx950__)
+#define REBOP_XOR3(a, b, c) __builtin_amdgcn_bitop3_b32((a), (b), (c), 0x96)
+#else
+#define REBOP_XOR3(a, b, c) ((a) ^ (b) ^ (c))
+#endif
+
 /* One Philox4x32-10 block: counter (c0..c3), key (k0, k1). */
 REBOP_HD rebop_u32x4 rebop_philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                          uint32_t k0, uint32_t k1) {
@@ -71,8 +78,8 @@ REBOP_HD rebop_u32x4 rebop_philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, 
     const uint64_t p1 = (uint64_t)REBOP_PHILOX_M1 * (uint64_t)c2;
     const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
     const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-    const uint32_t n0 = hi1 ^ c1 ^ k0;
-    const uint32_t n2 = hi0 ^ c3 ^ k1;
+    const uint32_t n0 = REBOP_XOR3(hi1, c1, k0);
+    const uint32_t n2 = REBOP_XOR3(hi0, c3, k1);
     c0 = n0;
     c1 = lo1;
     c2 = n2;

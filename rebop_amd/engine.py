@@ -165,6 +165,10 @@ class GillespieNet:
                 )
             )
 
+    def prepare(self, kernel: int = _native.REBOP_KERNEL_PERSISTENT | _native.REBOP_KERNEL_SPECIALIZED) -> None:
+        """Compile (and on a device, load) the kernel a run will use."""
+        check(self._lib.rebop_gillespie_prepare(self._h, int(kernel)))
+
     def get_species(self) -> np.ndarray:
         out = np.zeros(self.nb_species(), dtype=np.int64)
         check(self._lib.rebop_gillespie_get_species(self._h, _i64(out), len(out)))

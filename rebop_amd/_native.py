@@ -106,6 +106,13 @@ def load() -> ctypes.CDLL:
     if _lib is not None:
         return _lib
     path = Path(os.environ.get("REBOP_AMD_LIB", LIB_PATH))
+    # PyTorch-ROCm ships its own libamdhip64 and loads it by file name; the
+    # library binds libamdhip64.so.7 by SONAME.  Loading torch first makes
+    # both use torch's copy: one HIP runtime per process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not path.exists():
         raise RebopError(
             REBOP_ERR_DEVICE,

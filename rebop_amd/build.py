@@ -18,7 +18,7 @@ CSRC = HERE / "csrc"
 LIB = HERE / "librebop_gpu.so"
 ARCH = "gfx950"
 
-SOURCES = ["ssa_kernels.hip", "rebop_gpu.cpp", "jit.cpp"]
+SOURCES = ["ssa_kernels.hip", "ssa_lds.hip", "rebop_gpu.cpp", "jit.cpp"]
 # headers whose text is embedded in the library for run-time compilation
 EMBEDDED_HEADERS = ["rebop_rng.h", "ssa_layout.h", "ssa_lane.h"]
 DELIM = "REBOPSRC"

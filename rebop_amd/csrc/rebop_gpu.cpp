@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
@@ -27,6 +28,12 @@ hipError_t reg_occupancy(int c, int threads, size_t lds, int *blocks_per_cu);
 hipError_t launch_reset_counters(unsigned long long *counters, hipStream_t stream);
 hipError_t launch_reg(int c, const rebop_dev_net &net, const rebop_dev_run &run, int blocks,
                       int threads, size_t lds, hipStream_t stream);
+
+int lds_checkpoints();
+hipError_t lds_occupancy(int threads, size_t lds, int *blocks_per_cu);
+hipError_t launch_lds(const rebop_dev_job *jobs, const rebop_dev_item *items, uint32_t n_items,
+                      unsigned long long *counters, int blocks, int threads, size_t lds,
+                      hipStream_t stream);
 
 struct JitReaction {
   double k;
@@ -149,6 +156,9 @@ struct rebop_gillespie {
   rebop_dev_net view{};
   uint32_t F = 0;
   DevBuf counters;
+  // the network part of an LDS-kernel job (sparse jump, checkpoint blocks)
+  rebop_dev_job job{};
+  DevBuf djobs, ditems;
   // network-specialised kernel (run-time compiled), rebuilt when reactions change
   bool jit_dirty = true;
   std::string jit_src;
@@ -200,14 +210,46 @@ int upload_net(rebop_gillespie *g) {
     for (uint32_t s = 0; s < S; ++s) diff[(size_t)r * S + s] = (double)x.diff[s];
   }
   const uint32_t F = (uint32_t)fs.size();
+  // sparse jump (Jump::sparse, gillespie.rs:126-140) and whether the
+  // cumulative sum is guaranteed non-decreasing (all rate constants >= 0 and
+  // every consumed species appears once in the rate with order >= its use)
+  std::vector<uint32_t> jb(R + 1, 0), js;
+  std::vector<int32_t> jd;
+  bool linear = false;
+  for (uint32_t r = 0; r < R; ++r) {
+    const Reaction &x = g->rx[r];
+    if (x.k < 0.0) linear = true;
+    for (uint32_t sp = 0; sp < S; ++sp) {
+      if (x.diff[sp] == 0) continue;
+      js.push_back(sp);
+      jd.push_back((int32_t)x.diff[sp]);
+      if (x.diff[sp] < 0) {
+        uint32_t seen = 0, order = 0;
+        for (auto &e : x.entries)
+          if (e.first == sp) {
+            seen++;
+            order = e.second;
+          }
+        if (seen != 1 || (int64_t)order < -x.diff[sp]) linear = true;
+      }
+    }
+    jb[r + 1] = (uint32_t)js.size();
+  }
+  const uint32_t J = (uint32_t)js.size();
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t ok = 0;
   const size_t ofb = ok + al(sizeof(double) * R);
   const size_t ofs = ofb + al(sizeof(uint32_t) * (R + 1));
   const size_t ofc = ofs + al(sizeof(uint32_t) * F);
   const size_t odf = ofc + al(sizeof(double) * F);
-  const size_t total = odf + al(sizeof(double) * R * S);
+  const size_t ojb = odf + al(sizeof(double) * R * S);
+  const size_t ojs = ojb + al(sizeof(uint32_t) * (R + 1));
+  const size_t ojd = ojs + al(sizeof(uint32_t) * J);
+  const size_t total = ojd + al(sizeof(int32_t) * J);
   std::vector<unsigned char> host(total, 0);
+  std::memcpy(host.data() + ojb, jb.data(), sizeof(uint32_t) * (R + 1));
+  if (J) std::memcpy(host.data() + ojs, js.data(), sizeof(uint32_t) * J);
+  if (J) std::memcpy(host.data() + ojd, jd.data(), sizeof(int32_t) * J);
   if (R) std::memcpy(host.data() + ok, k.data(), sizeof(double) * R);
   std::memcpy(host.data() + ofb, fb.data(), sizeof(uint32_t) * (R + 1));
   if (F) std::memcpy(host.data() + ofs, fs.data(), sizeof(uint32_t) * F);
@@ -228,6 +270,21 @@ int upload_net(rebop_gillespie *g) {
   g->view.fac_c = (const double *)(base + ofc);
   g->view.diff = (const double *)(base + odf);
   g->F = F;
+  const uint32_t MB = (uint32_t)rebop::lds_checkpoints();
+  rebop_dev_job jt{};
+  jt.S = S;
+  jt.R = R;
+  jt.block = R ? (R + MB - 1) / MB : 1;
+  jt.nblocks = R ? (R + jt.block - 1) / jt.block : 0;
+  jt.flags = linear ? REBOP_JOB_LINEAR : 0u;
+  jt.k = g->view.k;
+  jt.fac_begin = g->view.fac_begin;
+  jt.fac_s = g->view.fac_s;
+  jt.fac_c = g->view.fac_c;
+  jt.jump_begin = (const uint32_t *)(base + ojb);
+  jt.jump_s = (const uint32_t *)(base + ojs);
+  jt.jump_d = (const int32_t *)(base + ojd);
+  g->job = jt;
   g->dirty = false;
   g->dev_id = cur;
   return REBOP_OK;
@@ -291,6 +348,78 @@ int prepare_jit(rebop_gillespie *g) {
   return REBOP_OK;
 }
 
+constexpr int kLdsThreads = 64;
+
+// Items of 64 trajectories per job, largest expected cost first.
+std::vector<rebop_dev_item> make_items(const std::vector<rebop_dev_job> &jobs,
+                                       const std::vector<double> &cost) {
+  std::vector<uint32_t> order(jobs.size());
+  for (uint32_t j = 0; j < order.size(); ++j) order[j] = j;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+  std::vector<rebop_dev_item> items;
+  for (uint32_t j : order)
+    for (uint64_t f = 0; f < jobs[j].n_traj; f += 64) {
+      rebop_dev_item it;
+      it.job = j;
+      it.count = (uint32_t)std::min<uint64_t>(64, jobs[j].n_traj - f);
+      it.first = f;
+      items.push_back(it);
+    }
+  return items;
+}
+
+// Uploads the job and item tables and enqueues the LDS kernel.
+int enqueue_lds(const std::vector<rebop_dev_job> &jobs, const std::vector<double> &cost,
+                DevBuf &djobs, DevBuf &ditems, unsigned long long *counters, hipStream_t stream) {
+  uint32_t smax = 1;
+  for (auto &j : jobs) smax = std::max(smax, j.S);
+  const size_t lds = (size_t)smax * kLdsThreads * sizeof(int32_t);
+  if (lds > 64 * 1024)
+    return fail(REBOP_ERR_UNSUPPORTED, "LDS kernel: " + std::to_string(smax) +
+                                           " species exceed the 64 KiB workgroup LDS budget");
+  const std::vector<rebop_dev_item> items = make_items(jobs, cost);
+  if (items.empty()) return REBOP_OK;
+  if (items.size() > 0xffffffffull) return fail(REBOP_ERR_VALUE, "too many work items");
+  HIP_TRY(djobs.ensure(sizeof(rebop_dev_job) * jobs.size()));
+  HIP_TRY(ditems.ensure(sizeof(rebop_dev_item) * items.size()));
+  HIP_TRY(hipMemcpy(djobs.p, jobs.data(), sizeof(rebop_dev_job) * jobs.size(),
+                    hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ditems.p, items.data(), sizeof(rebop_dev_item) * items.size(),
+                    hipMemcpyHostToDevice));
+  int per_cu = 0;
+  HIP_TRY(rebop::lds_occupancy(kLdsThreads, lds, &per_cu));
+  if (per_cu < 1) per_cu = 1;
+  const uint64_t waves_per_block = kLdsThreads / 64;
+  uint64_t blocks = (uint64_t)per_cu * (uint64_t)g_dev.cus;
+  const uint64_t need = (items.size() + waves_per_block - 1) / waves_per_block;
+  if (blocks > need) blocks = need;
+  HIP_TRY(rebop::launch_lds((const rebop_dev_job *)djobs.p, (const rebop_dev_item *)ditems.p,
+                            (uint32_t)items.size(), counters, (int)blocks, kLdsThreads, lds,
+                            stream));
+  return REBOP_OK;
+}
+
+// Reads the counters of a finished launch into `stats`.
+int finish_stats(rebop_stats *stats, unsigned long long *counters, hipEvent_t ev0, hipEvent_t ev1,
+                 int kind, hipStream_t stream) {
+  unsigned long long ctr[REBOP_CTR_COUNT] = {0};
+  HIP_TRY(hipMemcpyAsync(ctr, counters, sizeof ctr, hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  stats->events = ctr[REBOP_CTR_EVENTS];
+  stats->draws = ctr[REBOP_CTR_DRAWS];
+  stats->trajectories = ctr[REBOP_CTR_TRAJ];
+  stats->kernel = (uint64_t)kind;
+  stats->kernel_ms = ms;
+  if (ctr[REBOP_CTR_FLAGS] & REBOP_FLAG_RANGE)
+    return fail(REBOP_ERR_RANGE, "a count left the exactly simulated range");
+  return REBOP_OK;
+}
+
 // Enqueues one SSA launch (plus the counter reset) on `stream`.  When
 // `stats` is set, synchronises and fills it.
 int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t stream,
@@ -300,20 +429,60 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
   rc = upload_net(g);
   if (rc) return rc;
   const uint32_t S = g->S, R = (uint32_t)g->rx.size();
+  // kernel choice: LDS when asked; the specialised kernel when asked or (AUTO)
+  // when the network is small enough; else the data-driven register kernel
+  // when it fits, else the LDS kernel
   const int variant = kernel & 0xF;
-  if (variant == REBOP_KERNEL_LDS)
-    return fail(REBOP_ERR_UNSUPPORTED, "the LDS kernel is not built in this library");
-  bool spec = (kernel & REBOP_KERNEL_SPECIALIZED) != 0;
-  if (variant == REBOP_KERNEL_AUTO && !(kernel & REBOP_KERNEL_SPECIALIZED))
-    spec = jit_eligible(g) && !jit_disabled();
-  const bool persistent = variant != REBOP_KERNEL_PLAIN;
-  if (spec) {
+  const bool fits_regs = pick_reg_shape(S, R) >= 0;
+  bool spec = false, use_lds = variant == REBOP_KERNEL_LDS;
+  if (!use_lds && (kernel & REBOP_KERNEL_SPECIALIZED)) {
+    if (!jit_eligible(g))
+      return fail(REBOP_ERR_UNSUPPORTED, "network too large for a specialised kernel");
     rc = prepare_jit(g);
-    if (rc) {
-      if (kernel & REBOP_KERNEL_SPECIALIZED) return rc;
-      spec = false; // AUTO: the data-driven kernel runs instead
-    }
+    if (rc) return rc;
+    spec = true;
+  } else if (!use_lds && variant == REBOP_KERNEL_AUTO) {
+    spec = jit_eligible(g) && !jit_disabled() && prepare_jit(g) == REBOP_OK;
+    use_lds = !spec && !fits_regs;
+  } else if (!use_lds && !fits_regs) {
+    return fail(REBOP_ERR_UNSUPPORTED, "network with " + std::to_string(S) + " species and " +
+                                           std::to_string(R) +
+                                           " reactions exceeds the register kernels");
   }
+  if (use_lds) {
+    // LDS kernel: one job, items of 64 trajectories
+    rebop_dev_job job = g->job;
+    job.seed = run.seed;
+    job.traj_begin = run.traj_begin;
+    job.n_traj = run.n_traj;
+    job.tmax = run.tmax;
+    job.nb_steps = run.nb_steps;
+    job.mode = run.mode;
+    job.x0_per_traj = run.x0_per_traj;
+    job.x0 = run.x0;
+    job.out = run.out;
+    job.final_x = run.final_x;
+    job.st_x = run.st_x;
+    job.st_t = run.st_t;
+    job.st_draw = run.st_draw;
+    unsigned long long *ctrs = (unsigned long long *)g->counters.p;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (stats) {
+      HIP_TRY(hipEventCreate(&e0));
+      HIP_TRY(hipEventCreate(&e1));
+    }
+    HIP_TRY(rebop::launch_reset_counters(ctrs, stream));
+    if (stats) HIP_TRY(hipEventRecord(e0, stream));
+    rc = enqueue_lds(std::vector<rebop_dev_job>{job}, std::vector<double>{1.0}, g->djobs,
+                     g->ditems, ctrs, stream);
+    if (rc) return rc;
+    if (stats) {
+      HIP_TRY(hipEventRecord(e1, stream));
+      return finish_stats(stats, ctrs, e0, e1, REBOP_KERNEL_LDS, stream);
+    }
+    return REBOP_OK;
+  }
+  const bool persistent = variant != REBOP_KERNEL_PLAIN;
   const int shape = spec ? -1 : pick_reg_shape(S, R);
   if (!spec && shape < 0)
     return fail(REBOP_ERR_UNSUPPORTED, "network with " + std::to_string(S) + " species and " +
@@ -354,22 +523,7 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
   }
   if (stats) HIP_TRY(hipEventRecord(ev1, stream));
 
-  unsigned long long ctr[REBOP_CTR_COUNT] = {0};
-  if (stats) {
-    HIP_TRY(hipMemcpyAsync(ctr, run.counters, sizeof ctr, hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipStreamSynchronize(stream));
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
-    stats->events = ctr[REBOP_CTR_EVENTS];
-    stats->draws = ctr[REBOP_CTR_DRAWS];
-    stats->trajectories = ctr[REBOP_CTR_TRAJ];
-    stats->kernel = (uint64_t)kind;
-    stats->kernel_ms = ms;
-    if (ctr[REBOP_CTR_FLAGS] & REBOP_FLAG_RANGE)
-      return fail(REBOP_ERR_RANGE, "a count left the exactly simulated range |n| <= 2^52");
-  }
+  if (stats) return finish_stats(stats, run.counters, ev0, ev1, kind, stream);
   return REBOP_OK;
 }
 

@@ -24,10 +24,22 @@ struct GenericNet {
   const double *lds_diff; // [R][S]
 
   __device__ __forceinline__ uint32_t S() const { return net.S; }
-  __device__ __forceinline__ uint32_t R() const { return net.R; }
+  __device__ __forceinline__ void load_x0(double (&x)[MS], const int64_t *src) const {
+    reg_load_x0(x, src, net.S);
+  }
+  __device__ __forceinline__ void load_state(double (&x)[MS], const double *st, uint64_t stride) const {
+    reg_load_state(x, st, stride, net.S);
+  }
+  __device__ __forceinline__ void store_state(const double (&x)[MS], double *st, uint64_t stride) const {
+    reg_store_state(x, st, stride, net.S);
+  }
+  __device__ __forceinline__ void write_counts(const double (&x)[MS], int64_t *dst, uint64_t stride) const {
+    reg_write_counts(x, dst, stride, net.S);
+  }
+  __device__ __forceinline__ bool in_range(const double (&x)[MS]) const { return reg_in_range(x, net.S); }
 
   // make_cumrates (src/gillespie.rs:357-364) over Rate::rate (:71-90)
-  __device__ __forceinline__ double props(const double (&x)[MS], double (&cum)[MR]) const {
+  __device__ __forceinline__ double props(double (&x)[MS], double (&cum)[MR]) const {
     double acc = 0.0;
 #pragma unroll
     for (int r = 0; r < MR; ++r) {
@@ -44,7 +56,7 @@ struct GenericNet {
   }
 
   // choose_cumrate_sum (src/gillespie.rs:402-407)
-  __device__ __forceinline__ uint32_t choose(const double (&cum)[MR], double chosen) const {
+  __device__ __forceinline__ uint32_t choose(double (&)[MS], double (&cum)[MR], double chosen) const {
     uint32_t k = 0;
 #pragma unroll
     for (int r = 0; r < MR; ++r)
@@ -69,7 +81,8 @@ __global__ void __launch_bounds__(256) ssa_reg_kernel(rebop_dev_net net, rebop_d
   if (threadIdx.x < 4) blk_ctr[threadIdx.x] = 0ull;
   __syncthreads();
   const GenericNet<MS, MR> pol{net, lds_diff};
-  const LaneTotals acc = lane_loop(pol, run);
+  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const LaneTotals acc = lane_loop(pol, run, gtid, (uint64_t)gridDim.x * blockDim.x);
   flush_totals(acc, run, blk_ctr);
 }
 
@@ -90,7 +103,7 @@ struct RegShape {
   int ms, mr;
 };
 
-static const RegShape kRegShapes[] = {{4, 8}, {8, 16}, {16, 32}, {32, 64}};
+static const RegShape kRegShapes[] = {{4, 8}, {8, 16}, {16, 32}};
 
 int reg_shape_count() { return (int)(sizeof(kRegShapes) / sizeof(kRegShapes[0])); }
 int reg_shape_ms(int c) { return kRegShapes[c].ms; }
@@ -101,7 +114,6 @@ static const void *reg_kernel_ptr(int c) {
   case 0: return (const void *)&ssa_reg_kernel<4, 8>;
   case 1: return (const void *)&ssa_reg_kernel<8, 16>;
   case 2: return (const void *)&ssa_reg_kernel<16, 32>;
-  case 3: return (const void *)&ssa_reg_kernel<32, 64>;
   default: return nullptr;
   }
 }
@@ -122,7 +134,6 @@ hipError_t launch_reg(int c, const rebop_dev_net &net, const rebop_dev_run &run,
   case 0: hipLaunchKernelGGL((ssa_reg_kernel<4, 8>), dim3(blocks), dim3(threads), lds, stream, net, run); break;
   case 1: hipLaunchKernelGGL((ssa_reg_kernel<8, 16>), dim3(blocks), dim3(threads), lds, stream, net, run); break;
   case 2: hipLaunchKernelGGL((ssa_reg_kernel<16, 32>), dim3(blocks), dim3(threads), lds, stream, net, run); break;
-  case 3: hipLaunchKernelGGL((ssa_reg_kernel<32, 64>), dim3(blocks), dim3(threads), lds, stream, net, run); break;
   default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -16,19 +16,27 @@
 //                       are recomputed in the reference's order (make_cumrates
 //                       :357-364 over Rate::rate :71-90) at the next iteration
 //
-// The network enters through a policy class `Net` providing
-//   static constexpr int MS, MR;          register capacity (species, reactions)
-//   uint32_t S() const, R() const;
-//   double props(const double (&x)[MS], double (&cum)[MR]) const;   -> total
-//   uint32_t choose(const double (&cum)[MR], double chosen) const;
-//   void affect(double (&x)[MS], uint32_t k) const;
-// so that one loop serves both a data-driven network (any network within the
-// capacity) and a network compiled into the kernel's code.
+// The network enters through a policy object `net`.  The lane's register
+// state is two local arrays of lane_loop, double x[Net::MS] (counts, for
+// policies that keep them in VGPRs) and double cum[Net::MR] (what choose
+// needs), handed to every policy call (local arrays, not struct members, so
+// that the compiler keeps them in VGPRs even when x is indexed by a
+// wave-uniform runtime species index):
+//   uint32_t S() const;
+//   void load_x0(X &x, const int64_t *src);                counts from an int64 row
+//   void load_state(X &x, const double *st, uint64_t stride);  st[s * stride]
+//   void store_state(const X &x, double *st, uint64_t stride);
+//   void write_counts(const X &x, int64_t *dst, uint64_t stride);  dst[s * stride]
+//   bool in_range(const X &x);          counts inside the exactly simulated range
+//   double props(X &x, C &cum);         make_cumrates -> total
+//   uint32_t choose(X &x, C &cum, double chosen);
+//   void affect(X &x, uint32_t k);
+// so that one loop serves counts in VGPRs (reg_* helpers below) and in LDS,
+// for a data-driven network or one compiled into the kernel.
 //
-// Counts are exact FP64 integers in VGPRs (see ssa_layout.h for the range
-// guard).  A persistent launch refills a finished lane with the next
-// trajectory id from a device-wide counter: one atomic per wave per refill
-// (ballot over the finished lanes, rank by popcount).
+// A persistent launch refills a finished lane with the next trajectory id
+// from a device-wide counter: one atomic per wave per refill (ballot over the
+// finished lanes, rank by popcount).
 #ifndef REBOP_SSA_LANE_H
 #define REBOP_SSA_LANE_H
 
@@ -52,19 +60,57 @@ struct LaneTotals {
   uint32_t trajectories = 0, flags = 0;
 };
 
+// Helpers for counts held in VGPRs as exact FP64 integers (species >= S stay 0).
+template <int MS>
+__device__ __forceinline__ void reg_load_x0(double (&x)[MS], const int64_t *src, uint32_t S) {
+#pragma unroll
+  for (int s = 0; s < MS; ++s) x[s] = s < (int)S ? (double)src[s] : 0.0;
+}
+template <int MS>
+__device__ __forceinline__ void reg_load_state(double (&x)[MS], const double *st, uint64_t stride,
+                                               uint32_t S) {
+#pragma unroll
+  for (int s = 0; s < MS; ++s) x[s] = s < (int)S ? st[(uint64_t)s * stride] : 0.0;
+}
+template <int MS>
+__device__ __forceinline__ void reg_store_state(const double (&x)[MS], double *st, uint64_t stride,
+                                                uint32_t S) {
+#pragma unroll
+  for (int s = 0; s < MS; ++s)
+    if (s < (int)S) st[(uint64_t)s * stride] = x[s];
+}
+template <int MS>
+__device__ __forceinline__ void reg_write_counts(const double (&x)[MS], int64_t *dst,
+                                                 uint64_t stride, uint32_t S) {
+#pragma unroll
+  for (int s = 0; s < MS; ++s)
+    if (s < (int)S) dst[(uint64_t)s * stride] = (int64_t)x[s];
+}
+template <int MS>
+__device__ __forceinline__ bool reg_in_range(const double (&x)[MS], uint32_t S) {
+  bool ok = true;
+#pragma unroll
+  for (int s = 0; s < MS; ++s)
+    if (s < (int)S) ok = ok && (__builtin_fabs(x[s]) <= REBOP_EXACT_LIMIT);
+  return ok;
+}
+
+// Runs trajectories on this lane until none is left.  `first` is the lane's
+// first local trajectory index (>= run.n_traj for none); a persistent launch
+// then takes indices queue_offset + atomicAdd(queue) from the device queue.
 template <class Net>
-__device__ __forceinline__ LaneTotals lane_loop(const Net &net, const rebop_dev_run &run) {
-  constexpr int MS = Net::MS;
-  constexpr int MR = Net::MR;
-  const uint32_t S = net.S();
+__device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &run, uint64_t first,
+                                                uint64_t queue_offset) {
   const uint64_t n = run.n_traj;
   const uint32_t nb = run.mode == REBOP_MODE_TARGET ? 0u : (uint32_t)run.nb_steps;
-  const uint64_t Sn = (uint64_t)S * n;
-  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t total_threads = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t Sn = (uint64_t)net.S() * n;
 
-  double x[MS];
-  double cum[MR];
+  double x[Net::MS];
+  double cum[Net::MR];
+#pragma unroll
+  for (int s = 0; s < Net::MS; ++s) x[s] = 0.0;
+#pragma unroll
+  for (int r = 0; r < Net::MR; ++r) cum[r] = 0.0;
   double tot = 0.0, t = 0.0, T = 0.0;
   uint64_t draw = 0, draw0 = 0, mark = 0, jl = 0;
   int64_t *orec = nullptr;
@@ -72,18 +118,13 @@ __device__ __forceinline__ LaneTotals lane_loop(const Net &net, const rebop_dev_
   LaneTotals acc;
   bool done = true, started = false, need_props = false;
 
-#pragma unroll
-  for (int s = 0; s < MS; ++s) x[s] = 0.0;
-#pragma unroll
-  for (int r = 0; r < MR; ++r) cum[r] = 0.0;
-
   for (;;) {
     if (done) {
-      // next trajectory: the lane's own id first, then (persistent launches)
-      // ids from the device-wide queue
+      // next trajectory: the lane's own first index, then (persistent
+      // launches) indices from the device-wide queue
       uint64_t nj = n;
       if (!started) {
-        nj = gtid;
+        nj = first;
         started = true;
       } else if (run.persistent) {
         const uint64_t mask = __ballot(1);
@@ -93,25 +134,20 @@ __device__ __forceinline__ LaneTotals lane_loop(const Net &net, const rebop_dev_
         if ((int)lane == leader)
           base = atomicAdd(&run.counters[REBOP_CTR_QUEUE], (unsigned long long)__popcll(mask));
         base = __shfl(base, leader);
-        nj = total_threads + base + (uint64_t)__popcll(mask & ((1ull << lane) - 1ull));
+        nj = queue_offset + base + (uint64_t)__popcll(mask & ((1ull << lane) - 1ull));
       }
       if (nj >= n) break;
       jl = nj;
       if (run.st_x) {
-#pragma unroll
-        for (int s = 0; s < MS; ++s) x[s] = s < (int)S ? run.st_x[(uint64_t)s * n + jl] : 0.0;
+        net.load_state(x, run.st_x + jl, n);
         t = run.st_t[jl];
         draw = run.st_draw[jl];
       } else {
-        const int64_t *src = run.x0_per_traj ? run.x0 + jl * S : run.x0;
-#pragma unroll
-        for (int s = 0; s < MS; ++s) x[s] = s < (int)S ? (double)src[s] : 0.0;
+        net.load_x0(x, run.x0_per_traj ? run.x0 + jl * net.S() : run.x0);
         t = 0.0;
         draw = 0;
       }
-#pragma unroll
-      for (int s = 0; s < MS; ++s)
-        if (s < (int)S && !(__builtin_fabs(x[s]) <= REBOP_EXACT_LIMIT)) acc.flags |= REBOP_FLAG_RANGE;
+      if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
       draw0 = draw;
       mark = draw;
       ov = 0;
@@ -140,28 +176,18 @@ __device__ __forceinline__ LaneTotals lane_loop(const Net &net, const rebop_dev_
 
     if (rec) {
       if (orec) {
-#pragma unroll
-        for (int s = 0; s < MS; ++s)
-          if (s < (int)S) orec[(uint64_t)s * n] = (int64_t)x[s];
+        net.write_counts(x, orec, n);
         orec += Sn;
       }
-#pragma unroll
-      for (int s = 0; s < MS; ++s)
-        if (s < (int)S && !(__builtin_fabs(x[s]) <= REBOP_EXACT_LIMIT)) acc.flags |= REBOP_FLAG_RANGE;
+      if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
       if (draw - mark >= REBOP_DRAWS_PER_CHECK) acc.flags |= REBOP_FLAG_RANGE;
       mark = draw;
       t = T;
       i += 1;
       if (i > nb) {
-        if (run.final_x) {
-#pragma unroll
-          for (int s = 0; s < MS; ++s)
-            if (s < (int)S) run.final_x[(uint64_t)s * n + jl] = (int64_t)x[s];
-        }
+        if (run.final_x) net.write_counts(x, run.final_x + jl, n);
         if (run.st_x) {
-#pragma unroll
-          for (int s = 0; s < MS; ++s)
-            if (s < (int)S) run.st_x[(uint64_t)s * n + jl] = x[s];
+          net.store_state(x, run.st_x + jl, n);
           run.st_t[jl] = t;
           run.st_draw[jl] = draw;
         }
@@ -174,28 +200,38 @@ __device__ __forceinline__ LaneTotals lane_loop(const Net &net, const rebop_dev_
       }
     } else {
       t = tn;
-      const uint32_t k = net.choose(cum, tot * u);
-      net.affect(x, k);
+      net.affect(x, net.choose(x, cum, tot * u));
       need_props = true;
     }
   }
   return acc;
 }
 
-// Sums the lane totals of a workgroup in LDS, then once into the launch counters.
-__device__ __forceinline__ void flush_totals(const LaneTotals &acc, const rebop_dev_run &run,
-                                             unsigned long long *blk_ctr) {
+// Sums lane totals over the workgroup in LDS (blk_ctr zeroed before use).
+__device__ __forceinline__ void add_totals(const LaneTotals &acc, unsigned long long *blk_ctr) {
   atomicAdd(&blk_ctr[0], (unsigned long long)acc.events);
   atomicAdd(&blk_ctr[1], (unsigned long long)acc.draws);
   atomicAdd(&blk_ctr[2], (unsigned long long)acc.trajectories);
   if (acc.flags) atomicOr(&blk_ctr[3], (unsigned long long)acc.flags);
-  __syncthreads();
+}
+
+// After a workgroup barrier: one global update per workgroup.
+__device__ __forceinline__ void flush_block_totals(unsigned long long *counters,
+                                                   const unsigned long long *blk_ctr) {
   if (threadIdx.x == 0) {
-    atomicAdd(&run.counters[REBOP_CTR_EVENTS], blk_ctr[0]);
-    atomicAdd(&run.counters[REBOP_CTR_DRAWS], blk_ctr[1]);
-    atomicAdd(&run.counters[REBOP_CTR_TRAJ], blk_ctr[2]);
-    if (blk_ctr[3]) atomicOr(&run.counters[REBOP_CTR_FLAGS], blk_ctr[3]);
+    atomicAdd(&counters[REBOP_CTR_EVENTS], blk_ctr[0]);
+    atomicAdd(&counters[REBOP_CTR_DRAWS], blk_ctr[1]);
+    atomicAdd(&counters[REBOP_CTR_TRAJ], blk_ctr[2]);
+    if (blk_ctr[3]) atomicOr(&counters[REBOP_CTR_FLAGS], blk_ctr[3]);
   }
+}
+
+// The whole-launch epilogue of a single-network kernel.
+__device__ __forceinline__ void flush_totals(const LaneTotals &acc, const rebop_dev_run &run,
+                                             unsigned long long *blk_ctr) {
+  add_totals(acc, blk_ctr);
+  __syncthreads();
+  flush_block_totals(run.counters, blk_ctr);
 }
 
 } // namespace rebop

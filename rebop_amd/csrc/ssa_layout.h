@@ -75,4 +75,45 @@ typedef struct {
   unsigned long long *counters; /* [REBOP_CTR_COUNT] */
 } rebop_dev_run;
 
+/*
+ * One network of an LDS-kernel launch (a "job"), read at wave-uniform
+ * addresses.  Counts are int32 in LDS; reactions are grouped in blocks of
+ * `block` for the checkpointed cumulative sum.  The jump is sparse.
+ */
+#define REBOP_JOB_LINEAR 1u /* cumulative sum may be non-monotone: full linear search */
+#define REBOP_LDS_LIMIT 1073741824 /* |count| <= 2^30 in the int32 LDS kernel */
+
+typedef struct {
+  uint32_t S, R, block, nblocks;
+  uint32_t flags, pad;
+  const double *k;
+  const uint32_t *fac_begin;
+  const uint32_t *fac_s;
+  const double *fac_c;
+  const uint32_t *jump_begin; /* [R+1] */
+  const uint32_t *jump_s;
+  const int32_t *jump_d;
+  /* the run of this job (fields as rebop_dev_run) */
+  uint64_t seed;
+  uint64_t traj_begin;
+  uint64_t n_traj;
+  double tmax;
+  uint64_t nb_steps;
+  int32_t mode;
+  int32_t x0_per_traj;
+  const int64_t *x0;
+  int64_t *out;
+  int64_t *final_x;
+  double *st_x;
+  double *st_t;
+  uint64_t *st_draw;
+} rebop_dev_job;
+
+/* a wave's unit of work: trajectories first .. first+count-1 of one job */
+typedef struct {
+  uint32_t job;
+  uint32_t count;
+  uint64_t first;
+} rebop_dev_item;
+
 #endif

@@ -1,0 +1,244 @@
+// ssa_lds.hip -- LDS-resident SSA kernel for large and heterogeneous networks.
+//
+// Counts live in LDS as int32, one column per lane (x[s] at lds[s * BL +
+// lane]), so every access -- uniform or lane-varying species index -- maps
+// lane l to bank l mod 32: conflict-free.  The network (a "job") is read at
+// wave-uniform addresses through the scalar cache, except in the per-lane
+// block scan below.
+//
+// make_cumrates (src/gillespie.rs:357-364) runs as one uniform pass that
+// keeps the running sum at the end of every block of B reactions (cum[b],
+// B = ceil(R / MB)).  choose_cumrate_sum (:402-407) counts the reactions with
+// cum_r < chosen; while every propensity is >= 0 the cumulative sum is
+// non-decreasing and that count is the first r with cum_r >= chosen, so a
+// lane finds its block from the checkpoints and re-adds only that block's
+// propensities, starting from the checkpoint before it: the same sequence of
+// additions as the pass, hence the same values.  A lane whose counts can be
+// negative (negative initial counts, a network whose jump consumes more than
+// its rate's order, a negative rate constant, or a count that went negative)
+// uses the full linear count instead, which is the reference's definition.
+//
+// Work items are (job, up to 64 trajectories); a wave takes items from a
+// device-wide queue, so one launch can run thousands of different networks
+// (heterogeneous batch) with the items ordered largest-first by the host.
+#include <hip/hip_runtime.h>
+
+#include "rebop_rng.h"
+#include "ssa_lane.h"
+#include "ssa_layout.h"
+
+#pragma clang fp contract(off)
+
+namespace rebop {
+
+template <int MB>
+struct LdsNet {
+  static constexpr int MS = 1;  // counts are in LDS, not in x[]
+  static constexpr int MR = MB; // cum[] holds the block checkpoints
+  // job fields (wave-uniform)
+  uint32_t nS, R, B, nblk;
+  bool job_linear;
+  const double *k;
+  const uint32_t *fb;
+  const uint32_t *fs;
+  const double *fc;
+  const uint32_t *jb;
+  const uint32_t *js;
+  const int32_t *jd;
+  // this lane's counts
+  int32_t *xl;
+  uint32_t BL;
+  // per-lane flags
+  bool linear;
+  bool bad;
+
+  __device__ __forceinline__ int32_t &cnt(uint32_t s) const { return xl[(uint64_t)s * BL]; }
+  __device__ __forceinline__ double fcnt(uint32_t s) const { return (double)xl[(uint64_t)s * BL]; }
+
+  __device__ __forceinline__ uint32_t S() const { return nS; }
+
+  __device__ __forceinline__ void set_count(uint32_t s, double v) {
+    if (!(__builtin_fabs(v) <= (double)REBOP_LDS_LIMIT)) {
+      bad = true;
+      v = 0.0;
+    }
+    cnt(s) = (int32_t)v;
+    if (v < 0.0) linear = true;
+  }
+  __device__ __forceinline__ void load_x0(double (&)[MS], const int64_t *src) {
+    linear = job_linear;
+    bad = false;
+    for (uint32_t s = 0; s < nS; ++s) set_count(s, (double)src[s]);
+  }
+  __device__ __forceinline__ void load_state(double (&)[MS], const double *st, uint64_t stride) {
+    linear = job_linear;
+    bad = false;
+    for (uint32_t s = 0; s < nS; ++s) set_count(s, st[(uint64_t)s * stride]);
+  }
+  __device__ __forceinline__ void store_state(const double (&)[MS], double *st, uint64_t stride) const {
+    for (uint32_t s = 0; s < nS; ++s) st[(uint64_t)s * stride] = fcnt(s);
+  }
+  __device__ __forceinline__ void write_counts(const double (&)[MS], int64_t *dst, uint64_t stride) const {
+    for (uint32_t s = 0; s < nS; ++s) dst[(uint64_t)s * stride] = (int64_t)cnt(s);
+  }
+  __device__ __forceinline__ bool in_range(const double (&)[MS]) const { return !bad; }
+
+  // Rate::rate of reaction r (src/gillespie.rs:71-90), factors in stored order
+  __device__ __forceinline__ double rate(uint32_t r) const {
+    double a = k[r];
+    const uint32_t fe = fb[r + 1];
+    for (uint32_t f = fb[r]; f < fe; ++f) a = a * (fcnt(fs[f]) - fc[f]);
+    return a;
+  }
+
+  // make_cumrates with a checkpoint at the end of every block
+  __device__ __forceinline__ double props(double (&)[MS], double (&cum)[MR]) const {
+    double acc = 0.0;
+    uint32_t r = 0;
+#pragma unroll
+    for (int b = 0; b < MB; ++b) { // static: cum[] stays in VGPRs
+      if ((uint32_t)b < nblk) {
+        const uint32_t re = (r + B < R) ? r + B : R;
+        for (; r < re; ++r) acc = acc + rate(r);
+        cum[b] = acc;
+      }
+    }
+    return acc;
+  }
+
+  __device__ __forceinline__ uint32_t choose(double (&)[MS], double (&cum)[MR], double chosen) const {
+    if (linear) {
+      // choose_cumrate_sum as defined: count every r with cum_r < chosen
+      double acc = 0.0;
+      uint32_t kk = 0;
+      for (uint32_t r = 0; r < R; ++r) {
+        acc = acc + rate(r);
+        kk += (acc < chosen) ? 1u : 0u;
+      }
+      return kk;
+    }
+    // non-decreasing sums: the block holding the first cum_r >= chosen
+    uint32_t b = 0;
+    double start = 0.0;
+#pragma unroll
+    for (int q = 0; q < MB; ++q) {
+      if ((uint32_t)q < nblk && cum[q] < chosen) {
+        b = (uint32_t)q + 1u;
+        start = cum[q];
+      }
+    }
+    if (b >= nblk) b = nblk - 1u; // unreachable: cum[nblk-1] = total > chosen
+    uint32_t r = b * B;
+    const uint32_t re = (r + B < R) ? r + B : R;
+    double acc = start;
+    for (; r < re; ++r) {
+      acc = acc + rate(r);
+      if (!(acc < chosen)) break;
+    }
+    return r < re ? r : re - 1u;
+  }
+
+  // Jump::affect (src/gillespie.rs:142-152), sparse form
+  __device__ __forceinline__ void affect(double (&)[MS], uint32_t kk) {
+    const uint32_t e1 = jb[kk + 1];
+    for (uint32_t e = jb[kk]; e < e1; ++e) {
+      const uint32_t s = js[e];
+      const int32_t v = (int32_t)((uint32_t)cnt(s) + (uint32_t)jd[e]);
+      cnt(s) = v;
+      if (v < 0) linear = true;
+      if (v > REBOP_LDS_LIMIT || v < -REBOP_LDS_LIMIT) bad = true;
+    }
+  }
+};
+
+template <int MB>
+__global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__restrict__ jobs,
+                                                      const rebop_dev_item *__restrict__ items,
+                                                      uint32_t n_items,
+                                                      unsigned long long *counters) {
+  extern __shared__ int32_t lds_x[]; // [S_max][blockDim.x]
+  __shared__ unsigned long long blk_ctr[4];
+  if (threadIdx.x < 4) blk_ctr[threadIdx.x] = 0ull;
+  __syncthreads();
+
+  const uint32_t lane = __lane_id();
+  LaneTotals total;
+  for (;;) {
+    // one item per wave; readfirstlane makes the index (and so every job
+    // field and network read below) wave-uniform: scalar loads into SGPRs
+    unsigned long long got = 0ull;
+    if (lane == 0) got = atomicAdd(&counters[REBOP_CTR_QUEUE], 1ull);
+    const uint32_t it = __builtin_amdgcn_readfirstlane(
+        (uint32_t)(got < (unsigned long long)n_items ? got : (unsigned long long)n_items));
+    if (it >= n_items) break;
+    const rebop_dev_item item = items[it];
+    const rebop_dev_job &job = jobs[item.job];
+
+    LdsNet<MB> net;
+    net.nS = job.S;
+    net.R = job.R;
+    net.B = job.block;
+    net.nblk = job.nblocks;
+    net.job_linear = (job.flags & REBOP_JOB_LINEAR) != 0u;
+    net.k = job.k;
+    net.fb = job.fac_begin;
+    net.fs = job.fac_s;
+    net.fc = job.fac_c;
+    net.jb = job.jump_begin;
+    net.js = job.jump_s;
+    net.jd = job.jump_d;
+    net.xl = lds_x + threadIdx.x;
+    net.BL = blockDim.x;
+    net.linear = net.job_linear;
+    net.bad = false;
+
+    rebop_dev_run run{};
+    run.seed = job.seed;
+    run.traj_begin = job.traj_begin;
+    run.n_traj = job.n_traj;
+    run.tmax = job.tmax;
+    run.nb_steps = job.nb_steps;
+    run.mode = job.mode;
+    run.persistent = 0;
+    run.x0 = job.x0;
+    run.x0_per_traj = job.x0_per_traj;
+    run.out = job.out;
+    run.final_x = job.final_x;
+    run.st_x = job.st_x;
+    run.st_t = job.st_t;
+    run.st_draw = job.st_draw;
+    run.counters = counters;
+
+    const uint64_t first = lane < item.count ? item.first + lane : job.n_traj;
+    const LaneTotals acc = lane_loop(net, run, first, 0ull);
+    total.events += acc.events;
+    total.draws += acc.draws;
+    total.trajectories += acc.trajectories;
+    total.flags |= acc.flags;
+  }
+  add_totals(total, blk_ctr);
+  __syncthreads();
+  flush_block_totals(counters, blk_ctr);
+}
+
+// ---------------------------------------------------------------------------
+// launchers (C++ linkage, used by rebop_gpu.cpp)
+
+constexpr int kLdsCheckpoints = 32;
+
+int lds_checkpoints() { return kLdsCheckpoints; }
+
+hipError_t lds_occupancy(int threads, size_t lds, int *blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      blocks_per_cu, (const void *)&ssa_lds_kernel<kLdsCheckpoints>, threads, lds);
+}
+
+hipError_t launch_lds(const rebop_dev_job *jobs, const rebop_dev_item *items, uint32_t n_items,
+                      unsigned long long *counters, int blocks, int threads, size_t lds,
+                      hipStream_t stream) {
+  hipLaunchKernelGGL((ssa_lds_kernel<kLdsCheckpoints>), dim3(blocks), dim3(threads), lds, stream,
+                     jobs, items, n_items, counters);
+  return hipGetLastError();
+}
+
+} // namespace rebop

@@ -47,7 +47,7 @@ def _device_run(g, tmax, nb, n, kernel, traj_begin=0, x0=None):
         ("sir", sir(), 250.0, 250, 2048),
         ("dimers", dimers(), 1.0, 100, 1024),
         ("mm", michaelis_menten(), 50.0, 32, 2048),
-        ("syn16x40", synthetic(16, 40, 3, init_mean=20.0), 0.5, 20, 512),
+        ("syn16x40", synthetic(16, 40, 3, init_mean=20.0), 5.0, 20, 512),
     ],
 )
 def test_variants_bit_exact(name, spec, tmax, nb, n, kernel, nthreads):

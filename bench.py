@@ -23,6 +23,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -58,30 +60,158 @@ def parse_traffic(path: Path):
         return None
 
 
-def cpu_baseline(spec, seed: int) -> dict:
+def host_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(jobs, note: str) -> dict:
+    """Times the CPU oracle over `jobs` = [(spec, seed, tmax, nb_steps, n_traj, traj_begin)]."""
     from oracle import pyoracle
 
-    try:
-        threads = len(os.sched_getaffinity(0))
-    except AttributeError:
-        threads = os.cpu_count() or 1
-    threads = max(1, min(16, threads))
-    net = pyoracle.OracleNet(spec[0])
-    for k, o, d in spec[1]:
-        net.add_lma(k, o, d)
+    threads = host_threads()
+    events = 0
     t0 = time.perf_counter()
-    _, _, _, ev, _ = net.run_grid(seed, spec[2], TMAX, NB_STEPS, n_traj=CPU_SAMPLE,
-                                  nthreads=threads, want_out=False)
+    for spec, seed, tmax, nb, n, tb in jobs:
+        net = pyoracle.OracleNet(spec[0], sparse=True)
+        for k, o, d in spec[1]:
+            net.add_lma(k, o, d)
+        _, _, _, ev, _ = net.run_grid(seed, spec[2], tmax, nb, n_traj=n, traj_begin=tb,
+                                      nthreads=threads, want_out=False)
+        events += ev
     dt = time.perf_counter() - t0
     return {
-        "value": ev / dt,
+        "value": events / dt,
         "unit": "events/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"first {CPU_SAMPLE} trajectories of the same workload ({ev} events, "
-                  f"{dt:.2f} s wall on {threads} threads); C restatement of advance_until "
-                  "with the same Philox stream (the Rust reference cannot be built here)",
+        "sample": f"{note} ({events} events, {dt:.2f} s wall on {threads} threads); C "
+                  "restatement of advance_until with the same Philox stream, sparse rate form "
+                  "(bit-identical to dense, faster); the Rust reference cannot be built here",
     }
+
+
+class Workload:
+    """One benchmark configuration: set up device buffers, run one step."""
+
+    name = ""
+    description = ""
+
+    def setup(self):
+        raise NotImplementedError
+
+    def step(self, seed: int):
+        """Runs one step; returns a Stats."""
+        raise NotImplementedError
+
+    def ops_per_event(self) -> float:
+        raise NotImplementedError
+
+    def cpu_jobs(self):
+        raise NotImplementedError
+
+
+class SingleNetwork(Workload):
+    def __init__(self, name, spec, tmax, nb, n, kernel, description, cpu_n):
+        self.name, self.spec, self.tmax, self.nb, self.n = name, spec, tmax, nb, n
+        self.kernel, self.description, self.cpu_n = kernel, description, cpu_n
+
+    def setup(self):
+        import torch
+
+        from rebop_amd import networks
+
+        self.out = torch.empty((self.nb + 1, self.spec[0], self.n), dtype=torch.int64, device="cuda")
+        self.stream = torch.cuda.current_stream().cuda_stream
+        self.g = networks.build(self.spec, 0)
+
+    def step(self, seed):
+        self.g.seed(seed)
+        return self.g.run_device(self.tmax, self.nb, self.n, self.out.data_ptr(), kernel=self.kernel,
+                                 stream=self.stream, with_stats=True)
+
+    def ops_per_event(self):
+        return algorithmic_ops_per_event(self.spec[1])
+
+    def cpu_jobs(self):
+        return [(self.spec, 42, self.tmax, self.nb, self.cpu_n, 0)], \
+            f"first {self.cpu_n} trajectories of the same workload"
+
+
+class Heterogeneous(Workload):
+    name = "het"
+
+    def __init__(self, n_networks, n_traj, nb, seed=2026):
+        self.n_networks, self.n_traj, self.nb, self.seed = n_networks, n_traj, nb, seed
+        self.description = (f"heterogeneous batch: {n_networks} random mass-action networks "
+                            f"(species U{{3..64}}, reactions U{{4..256}}, init Poisson(50)), "
+                            f"tmax per network for 1e2..1e5 mean events, {n_traj} trajectories "
+                            f"each, nb_steps={nb}, one fused launch")
+
+    def setup(self):
+        import torch
+
+        from rebop_amd import workloads
+
+        self.specs = workloads.heterogeneous_specs(self.n_networks, self.seed)
+        self.nets = workloads.build_nets(self.specs, seed=0)
+        self.targets = workloads.target_events(self.n_networks)
+        self.tmax = workloads.calibrate_tmax(self.nets, self.specs, self.targets)
+        self.outs = [torch.empty((self.nb + 1, sp[0], self.n_traj), dtype=torch.int64, device="cuda")
+                     for sp in self.specs]
+        self.stream = torch.cuda.current_stream().cuda_stream
+        R = np.array([len(sp[1]) for sp in self.specs], dtype=np.float64)
+        self.cost = self.targets * (R + 16.0)
+
+    def step(self, seed):
+        from rebop_amd.engine import run_jobs
+
+        for g in self.nets:
+            g.seed(seed)
+        jobs = [dict(net=g, tmax=float(t), nb_steps=self.nb, n_traj=self.n_traj,
+                     out_ptr=o.data_ptr(), cost_hint=float(c))
+                for g, t, o, c in zip(self.nets, self.tmax, self.outs, self.cost)]
+        st, ev = run_jobs(jobs, stream=self.stream, per_job_events=True)
+        self.last_events = ev
+        return st
+
+    def ops_per_event(self):
+        # events-weighted mean of each network's F
+        F = np.array([algorithmic_ops_per_event(sp[1]) for sp in self.specs], dtype=np.float64)
+        ev = getattr(self, "last_events", None)
+        if ev is None or ev.sum() == 0:
+            return float(F.mean())
+        return float((F * ev).sum() / ev.sum())
+
+    def cpu_jobs(self):
+        # every 64th network, 16 trajectories each
+        idx = list(range(0, self.n_networks, 64))
+        jobs = [(self.specs[i], 42, float(self.tmax[i]), self.nb, 16, 0) for i in idx]
+        return jobs, f"{len(idx)} networks (every 64th) x 16 trajectories of the same batch"
+
+
+def make_workload(name: str, kernel: int) -> Workload:
+    from rebop_amd import networks, workloads
+
+    if name == "dimer":
+        return SingleNetwork("dimer", networks.dimers(), TMAX, NB_STEPS, N_TRAJ, kernel,
+                             "dimer gene-expression network (4 species, 5 reactions), 10^6 "
+                             "trajectories, tmax=1.0, nb_steps=100, int64 counts out [101][4][10^6]",
+                             CPU_SAMPLE)
+    if name == "sir":
+        return SingleNetwork("sir", networks.sir(), 250.0, 250, 10_000, kernel,
+                             "SIR (3 species, 2 reactions), 10^4 trajectories, tmax=250, "
+                             "nb_steps=250", 10_000)
+    if name == "syn":
+        return SingleNetwork("syn", workloads.synthetic_large(), 10.0, 200, 2**18, kernel,
+                             "synthetic network, 128 species x 512 reactions (seed 7), 2^18 "
+                             "trajectories, tmax=10, nb_steps=200", 64)
+    if name == "het":
+        return Heterogeneous(4096, 256, 50)
+    raise SystemExit(f"unknown workload {name}")
 
 
 def main() -> int:
@@ -91,6 +221,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--kernel", type=int, default=0, help="REBOP_KERNEL_* (0 = auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="dimer", choices=["dimer", "sir", "syn", "het"],
+                    help="dimer (the headline configuration), sir, syn (128x512), het (4096 networks)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one device copy of 1 GiB (read 1 GiB, write 1 GiB) and exit; "
                          "scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
@@ -113,31 +245,21 @@ def main() -> int:
 
     import torch
 
-    from rebop_amd import _native, networks
+    from rebop_amd import _native
 
-    spec = networks.dimers()
-    S, rx, x0 = spec
-    F = algorithmic_ops_per_event(rx)
+    wl = make_workload(args.workload, args.kernel)
     dev_name, cus = _native.device_info()
-    out = torch.empty((NB_STEPS + 1, S, N_TRAJ), dtype=torch.int64, device="cuda")
-    stream = torch.cuda.current_stream().cuda_stream
-
-    g = networks.build(spec, 0)
-
-    def step(seed: int):
-        g.seed(seed)
-        return g.run_device(TMAX, NB_STEPS, N_TRAJ, out.data_ptr(), kernel=args.kernel,
-                            stream=stream, with_stats=True)
+    wl.setup()
 
     for w in range(args.warmup):
-        step(1000 + w)
+        wl.step(1000 + w)
     torch.cuda.synchronize()
     events = 0
     kernel_ms = []
     kinds = set()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        st = step(42 + k)
+        st = wl.step(42 + k)
         events += st.events
         kernel_ms.append(st.kernel_ms)
         kinds.add(int(st.kernel))
@@ -147,9 +269,11 @@ def main() -> int:
     value = events / wall
     avg_kernel_s = sum(kernel_ms) / len(kernel_ms) / 1e3
     ev_per_launch = events / args.steps
+    F = wl.ops_per_event()
     achieved = ev_per_launch * F / avg_kernel_s / 1e12
     peak = PEAK_FP64_OPS / 1e12
-    traffic = parse_traffic(ROOT / "profiles" / "ssa_dimer_1e6_traffic.txt")
+    traffic = parse_traffic(ROOT / "profiles" / f"ssa_{wl.name}_traffic.txt") if wl.name != "dimer" \
+        else parse_traffic(ROOT / "profiles" / "ssa_dimer_1e6_traffic.txt")
     line = {
         "metric": "SSA reaction events/sec on 1 MI355X",
         "value": value,
@@ -164,11 +288,7 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "dimer gene-expression network (4 species, 5 reactions), 10^6 "
-                        "trajectories, tmax=1.0, nb_steps=100, int64 counts out [101][4][10^6]",
-            "n_trajectories": N_TRAJ,
-            "nb_steps": NB_STEPS,
-            "tmax": TMAX,
+            "workload": wl.description,
             "events_per_step": ev_per_launch,
             "kernel": sorted(kinds),
             "device": dev_name,
@@ -186,7 +306,8 @@ def main() -> int:
         },
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(spec, 42)
+        jobs, note = wl.cpu_jobs()
+        line["cpu_baseline"] = cpu_baseline(jobs, note)
         line["speedup_vs_cpu_baseline"] = value / line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)
     return 0

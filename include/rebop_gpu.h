@@ -56,7 +56,7 @@ enum rebop_kernel_kind {
   REBOP_KERNEL_AUTO = 0,
   REBOP_KERNEL_PLAIN = 1,      /* one trajectory per lane, lanes retire when done */
   REBOP_KERNEL_PERSISTENT = 2, /* lanes refill from an atomic trajectory queue */
-  REBOP_KERNEL_LDS = 3,        /* counts in LDS, for networks beyond the register kernels */
+  REBOP_KERNEL_LDS = 3,        /* counts in LDS (int32), any network up to 256 species */
   /* flag: the network compiled into the kernel (run-time compilation,
    * cached); combine with PLAIN or PERSISTENT */
   REBOP_KERNEL_SPECIALIZED = 16
@@ -118,8 +118,8 @@ int rebop_gillespie_get_species(const rebop_gillespie *g, int64_t *species, uint
  *   out     [nb_steps+1][nb_species][n_traj] int64, trajectory fastest (may be NULL).
  *   times   [nb_steps+1] sample times tmax*i/nb_steps (may be NULL).
  *   stats   counters of the run (may be NULL).
- * nb_steps must be > 0 (the nb_steps == 0 all-events mode is
- * rebop_run_all_events).  Host buffers; blocks until the results are copied.
+ * nb_steps must be > 0 (nb_steps == 0, every event, is rebop_run_all_events).
+ * Host buffers; blocks until the results are copied.
  */
 int rebop_run(const rebop_gillespie *g, double tmax, uint64_t nb_steps, uint64_t traj_begin,
               uint64_t n_traj, const int64_t *x0, int64_t *out, double *times,
@@ -138,6 +138,47 @@ int rebop_run(const rebop_gillespie *g, double tmax, uint64_t nb_steps, uint64_t
 int rebop_run_device(const rebop_gillespie *g, double tmax, uint64_t nb_steps,
                      uint64_t traj_begin, uint64_t n_traj, const int64_t *d_x0, int64_t *d_out,
                      int64_t *d_final, int kernel, void *stream, rebop_stats *stats);
+
+/*
+ * The nb_steps == 0 mode of rebop's Python `Gillespie.run`
+ * (src/pyo3_gillespie.rs:209-223 over Gillespie::_advance_one_reaction,
+ * src/gillespie.rs:275-297) for n_traj trajectories: the state before the
+ * first event and after every event while t < tmax (the last event lies past
+ * tmax; an absorbing state records t = +inf once).  Two calls: with times and
+ * states NULL it fills n_records[j], trajectory j's record count (initial
+ * state included); then, given those counts, it writes trajectory j's records
+ * at row offset sum_{i<j} n_records[i] of times[] and states[][nb_species].
+ * tmax must not be +inf.  Host buffers.
+ */
+int rebop_run_all_events(const rebop_gillespie *g, double tmax, uint64_t traj_begin,
+                         uint64_t n_traj, uint64_t *n_records, double *times, int64_t *states,
+                         rebop_stats *stats);
+
+/* One network's run inside rebop_run_jobs_device. */
+typedef struct rebop_job {
+  const rebop_gillespie *net; /* network, initial species and seed */
+  double tmax;
+  uint64_t nb_steps;          /* > 0 */
+  uint64_t traj_begin;        /* trajectory j uses the stream (seed, traj_begin + j) */
+  uint64_t n_traj;
+  const int64_t *d_x0;        /* device [n_traj][S], or NULL for the handle's species */
+  int64_t *d_out;             /* device [nb_steps+1][S][n_traj], or NULL */
+  int64_t *d_final;           /* device [S][n_traj], or NULL */
+  double cost_hint;           /* expected relative work per trajectory (orders the
+                                 work queue largest-first; 0 if unknown) */
+} rebop_job;
+
+/*
+ * Many independent grid runs, one per job -- each is rebop_run_device's
+ * src/pyo3_gillespie.rs:197-208 loop for its own network -- fused into one
+ * launch of the LDS kernel: waves take (job, 64 trajectories) items from one
+ * queue.  Results are identical to running the jobs one by one.  Enqueued on
+ * `stream`; the call waits for completion.  events_per_job (nullable host
+ * array [n_jobs]) receives each job's applied-reaction count.  No counterpart
+ * in the reference beyond calling `Gillespie.run` once per network.
+ */
+int rebop_run_jobs_device(const rebop_job *jobs, uint32_t n_jobs, void *stream,
+                          rebop_stats *stats, uint64_t *events_per_job);
 
 /* Batched resumable state: N trajectories of one network, each with its own
  * (species, t, rng) -- the fields of gillespie::Gillespie (src/gillespie.rs:157-163)

@@ -61,6 +61,11 @@ SIGNATURES: dict[str, tuple] = {
         [c_void_p, c_double, c_uint64, c_uint64, c_uint64, c_void_p, c_void_p, c_void_p, c_int,
          c_void_p, c_void_p],
     ),
+    "rebop_run_all_events": (
+        c_int,
+        [c_void_p, c_double, c_uint64, c_uint64, POINTER(c_uint64), POINTER(c_double), POINTER(c_int64), c_void_p],
+    ),
+    "rebop_run_jobs_device": (c_int, [c_void_p, c_uint32, c_void_p, c_void_p, POINTER(c_uint64)]),
     "rebop_batch_new": (c_int, [c_void_p, c_uint64, c_uint64, POINTER(c_int64), POINTER(c_void_p)]),
     "rebop_batch_free": (None, [c_void_p]),
     "rebop_batch_advance_until": (c_int, [c_void_p, c_double, c_void_p]),
@@ -82,6 +87,22 @@ class Stats(ctypes.Structure):
         ("trajectories", c_uint64),
         ("kernel", c_uint64),
         ("kernel_ms", c_double),
+    ]
+
+
+class Job(ctypes.Structure):
+    """``rebop_job`` of include/rebop_gpu.h."""
+
+    _fields_ = [
+        ("net", c_void_p),
+        ("tmax", c_double),
+        ("nb_steps", c_uint64),
+        ("traj_begin", c_uint64),
+        ("n_traj", c_uint64),
+        ("d_x0", c_void_p),
+        ("d_out", c_void_p),
+        ("d_final", c_void_p),
+        ("cost_hint", c_double),
     ]
 
 

@@ -32,8 +32,8 @@ hipError_t launch_reg(int c, const rebop_dev_net &net, const rebop_dev_run &run,
 int lds_checkpoints();
 hipError_t lds_occupancy(int threads, size_t lds, int *blocks_per_cu);
 hipError_t launch_lds(const rebop_dev_job *jobs, const rebop_dev_item *items, uint32_t n_items,
-                      unsigned long long *counters, int blocks, int threads, size_t lds,
-                      hipStream_t stream);
+                      unsigned long long *counters, unsigned long long *job_events, int blocks,
+                      int threads, size_t lds, hipStream_t stream);
 
 struct JitReaction {
   double k;
@@ -371,7 +371,8 @@ std::vector<rebop_dev_item> make_items(const std::vector<rebop_dev_job> &jobs,
 
 // Uploads the job and item tables and enqueues the LDS kernel.
 int enqueue_lds(const std::vector<rebop_dev_job> &jobs, const std::vector<double> &cost,
-                DevBuf &djobs, DevBuf &ditems, unsigned long long *counters, hipStream_t stream) {
+                DevBuf &djobs, DevBuf &ditems, unsigned long long *counters,
+                unsigned long long *job_events, hipStream_t stream) {
   uint32_t smax = 1;
   for (auto &j : jobs) smax = std::max(smax, j.S);
   const size_t lds = (size_t)smax * kLdsThreads * sizeof(int32_t);
@@ -395,8 +396,8 @@ int enqueue_lds(const std::vector<rebop_dev_job> &jobs, const std::vector<double
   const uint64_t need = (items.size() + waves_per_block - 1) / waves_per_block;
   if (blocks > need) blocks = need;
   HIP_TRY(rebop::launch_lds((const rebop_dev_job *)djobs.p, (const rebop_dev_item *)ditems.p,
-                            (uint32_t)items.size(), counters, (int)blocks, kLdsThreads, lds,
-                            stream));
+                            (uint32_t)items.size(), counters, job_events, (int)blocks,
+                            kLdsThreads, lds, stream));
   return REBOP_OK;
 }
 
@@ -465,6 +466,10 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
     job.st_x = run.st_x;
     job.st_t = run.st_t;
     job.st_draw = run.st_draw;
+    job.n_records = run.n_records;
+    job.rec_offsets = run.rec_offsets;
+    job.rec_times = run.rec_times;
+    job.rec_states = run.rec_states;
     unsigned long long *ctrs = (unsigned long long *)g->counters.p;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (stats) {
@@ -474,7 +479,7 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
     HIP_TRY(rebop::launch_reset_counters(ctrs, stream));
     if (stats) HIP_TRY(hipEventRecord(e0, stream));
     rc = enqueue_lds(std::vector<rebop_dev_job>{job}, std::vector<double>{1.0}, g->djobs,
-                     g->ditems, ctrs, stream);
+                     g->ditems, ctrs, nullptr, stream);
     if (rc) return rc;
     if (stats) {
       HIP_TRY(hipEventRecord(e1, stream));
@@ -743,6 +748,136 @@ int rebop_run(const rebop_gillespie *gc, double tmax, uint64_t nb_steps, uint64_
                         REBOP_KERNEL_AUTO, nullptr, &local);
   if (rc) return rc;
   if (out && out_bytes) HIP_TRY(hipMemcpy(out, dout.p, out_bytes, hipMemcpyDeviceToHost));
+  if (stats) *stats = local;
+  return REBOP_OK;
+}
+
+int rebop_run_jobs_device(const rebop_job *jobs, uint32_t n_jobs, void *stream_,
+                          rebop_stats *stats, uint64_t *events_per_job) {
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  if (n_jobs && !jobs) return fail(REBOP_ERR_VALUE, "jobs is NULL");
+  int rc = ensure_device();
+  if (rc) return rc;
+  hipStream_t stream = (hipStream_t)stream_;
+  std::vector<rebop_dev_job> dj;
+  std::vector<double> cost;
+  dj.reserve(n_jobs);
+  for (uint32_t j = 0; j < n_jobs; ++j) {
+    const rebop_job &J = jobs[j];
+    if (int e = check_handle(J.net, "job net")) return e;
+    if (J.nb_steps == 0) return fail(REBOP_ERR_VALUE, "job " + std::to_string(j) + ": nb_steps must be > 0");
+    rebop_gillespie *g = const_cast<rebop_gillespie *>(J.net);
+    if (!J.d_x0) {
+      if (int e = check_x0(g->x0.data(), g->x0.size())) return e;
+    }
+    rc = upload_net(g);
+    if (rc) return rc;
+    rebop_dev_job d = g->job;
+    d.seed = g->seed;
+    d.traj_begin = J.traj_begin;
+    d.n_traj = J.n_traj;
+    d.tmax = J.tmax;
+    d.nb_steps = J.nb_steps;
+    d.mode = REBOP_MODE_GRID;
+    d.x0_per_traj = J.d_x0 ? 1 : 0;
+    d.x0 = J.d_x0 ? J.d_x0 : (const int64_t *)g->dx0.p;
+    d.out = J.d_out;
+    d.final_x = J.d_final;
+    dj.push_back(d);
+    cost.push_back(J.cost_hint);
+  }
+  DevBuf counters, djobs, ditems, dev_ev;
+  HIP_TRY(counters.ensure(sizeof(unsigned long long) * REBOP_CTR_COUNT));
+  unsigned long long *ctrs = (unsigned long long *)counters.p;
+  unsigned long long *jev = nullptr;
+  if (events_per_job && n_jobs) {
+    HIP_TRY(dev_ev.ensure(sizeof(unsigned long long) * n_jobs));
+    HIP_TRY(hipMemsetAsync(dev_ev.p, 0, sizeof(unsigned long long) * n_jobs, stream));
+    jev = (unsigned long long *)dev_ev.p;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(rebop::launch_reset_counters(ctrs, stream));
+  HIP_TRY(hipEventRecord(e0, stream));
+  rc = enqueue_lds(dj, cost, djobs, ditems, ctrs, jev, stream);
+  if (rc) {
+    (void)hipStreamSynchronize(stream);
+    return rc;
+  }
+  HIP_TRY(hipEventRecord(e1, stream));
+  rebop_stats local{};
+  rc = finish_stats(&local, ctrs, e0, e1, REBOP_KERNEL_LDS, stream);
+  if (stats) *stats = local;
+  if (jev)
+    HIP_TRY(hipMemcpy(events_per_job, jev, sizeof(unsigned long long) * n_jobs,
+                      hipMemcpyDeviceToHost));
+  return rc;
+}
+
+int rebop_run_all_events(const rebop_gillespie *gc, double tmax, uint64_t traj_begin,
+                         uint64_t n_traj, uint64_t *n_records, double *times, int64_t *states,
+                         rebop_stats *stats) {
+  if (int rc = check_handle(gc, "gillespie")) return rc;
+  if (!n_records && n_traj) return fail(REBOP_ERR_VALUE, "n_records is NULL");
+  if (std::isinf(tmax) && tmax > 0)
+    return fail(REBOP_ERR_VALUE, "tmax must be finite: every event up to +inf never ends");
+  rebop_gillespie *g = const_cast<rebop_gillespie *>(gc);
+  if (int rc = check_x0(g->x0.data(), g->x0.size())) return rc;
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  int rc = ensure_device();
+  if (rc) return rc;
+  rc = upload_net(g);
+  if (rc) return rc;
+  const uint32_t S = g->S;
+  DevBuf dcount, doff, dt, ds;
+  HIP_TRY(dcount.ensure(sizeof(uint64_t) * (n_traj ? n_traj : 1)));
+  rebop_dev_run run{};
+  run.seed = g->seed;
+  run.traj_begin = traj_begin;
+  run.n_traj = n_traj;
+  run.tmax = tmax;
+  run.nb_steps = 0;
+  run.mode = REBOP_MODE_EVENTS;
+  run.x0 = (const int64_t *)g->dx0.p;
+  run.x0_per_traj = 0;
+  run.n_records = (uint64_t *)dcount.p;
+  const bool want = times != nullptr || states != nullptr;
+  rebop_stats local{};
+  if (!want) {
+    // pass 1: count the records of every trajectory
+    rc = launch_run(g, run, REBOP_KERNEL_AUTO, nullptr, &local);
+    if (rc) return rc;
+    if (n_traj)
+      HIP_TRY(hipMemcpy(n_records, dcount.p, sizeof(uint64_t) * n_traj, hipMemcpyDeviceToHost));
+    if (stats) *stats = local;
+    return REBOP_OK;
+  }
+  // pass 2: the caller's counts give the offsets; the kernel checks them
+  std::vector<uint64_t> off(n_traj ? n_traj : 1, 0);
+  uint64_t total = 0;
+  for (uint64_t j = 0; j < n_traj; ++j) {
+    off[j] = total;
+    total += n_records[j];
+  }
+  if (n_traj) {
+    HIP_TRY(doff.ensure(sizeof(uint64_t) * n_traj));
+    HIP_TRY(hipMemcpy(doff.p, off.data(), sizeof(uint64_t) * n_traj, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dcount.p, n_records, sizeof(uint64_t) * n_traj, hipMemcpyHostToDevice));
+  }
+  HIP_TRY(dt.ensure(sizeof(double) * (total ? total : 1)));
+  HIP_TRY(ds.ensure(sizeof(int64_t) * (total * S ? total * S : 1)));
+  run.rec_offsets = (const uint64_t *)doff.p;
+  run.rec_times = (double *)dt.p;
+  run.rec_states = (int64_t *)ds.p;
+  rc = launch_run(g, run, REBOP_KERNEL_AUTO, nullptr, &local);
+  if (rc == REBOP_ERR_RANGE)
+    return fail(REBOP_ERR_VALUE, "n_records does not match this network and seed (or a count "
+                                 "left the exactly simulated range)");
+  if (rc) return rc;
+  if (times && total) HIP_TRY(hipMemcpy(times, dt.p, sizeof(double) * total, hipMemcpyDeviceToHost));
+  if (states && total * S)
+    HIP_TRY(hipMemcpy(states, ds.p, sizeof(int64_t) * total * S, hipMemcpyDeviceToHost));
   if (stats) *stats = local;
   return REBOP_OK;
 }

@@ -82,7 +82,9 @@ __global__ void __launch_bounds__(256) ssa_reg_kernel(rebop_dev_net net, rebop_d
   __syncthreads();
   const GenericNet<MS, MR> pol{net, lds_diff};
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const LaneTotals acc = lane_loop(pol, run, gtid, (uint64_t)gridDim.x * blockDim.x);
+  const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
+  const LaneTotals acc = run.mode == REBOP_MODE_EVENTS ? lane_events(pol, run, gtid, nthreads)
+                                                       : lane_loop(pol, run, gtid, nthreads);
   flush_totals(acc, run, blk_ctr);
 }
 

@@ -95,6 +95,26 @@ __device__ __forceinline__ bool reg_in_range(const double (&x)[MS], uint32_t S) 
   return ok;
 }
 
+// The next local trajectory index of a lane: its own `first` the first time,
+// then (persistent launches) queue_offset + an index from the device queue,
+// one atomic per wave (ballot over the lanes asking, rank by popcount).
+__device__ __forceinline__ uint64_t next_trajectory(const rebop_dev_run &run, bool &started,
+                                                    uint64_t first, uint64_t queue_offset) {
+  if (!started) {
+    started = true;
+    return first;
+  }
+  if (!run.persistent) return run.n_traj;
+  const uint64_t mask = __ballot(1);
+  const uint32_t lane = __lane_id();
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  unsigned long long base = 0ull;
+  if ((int)lane == leader)
+    base = atomicAdd(&run.counters[REBOP_CTR_QUEUE], (unsigned long long)__popcll(mask));
+  base = __shfl(base, leader);
+  return queue_offset + base + (uint64_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
 // Runs trajectories on this lane until none is left.  `first` is the lane's
 // first local trajectory index (>= run.n_traj for none); a persistent launch
 // then takes indices queue_offset + atomicAdd(queue) from the device queue.
@@ -122,20 +142,7 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
     if (done) {
       // next trajectory: the lane's own first index, then (persistent
       // launches) indices from the device-wide queue
-      uint64_t nj = n;
-      if (!started) {
-        nj = first;
-        started = true;
-      } else if (run.persistent) {
-        const uint64_t mask = __ballot(1);
-        const uint32_t lane = __lane_id();
-        const int leader = __ffsll((unsigned long long)mask) - 1;
-        unsigned long long base = 0ull;
-        if ((int)lane == leader)
-          base = atomicAdd(&run.counters[REBOP_CTR_QUEUE], (unsigned long long)__popcll(mask));
-        base = __shfl(base, leader);
-        nj = queue_offset + base + (uint64_t)__popcll(mask & ((1ull << lane) - 1ull));
-      }
+      const uint64_t nj = next_trajectory(run, started, first, queue_offset);
       if (nj >= n) break;
       jl = nj;
       if (run.st_x) {
@@ -203,6 +210,65 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
       net.affect(x, net.choose(x, cum, tot * u));
       need_props = true;
     }
+  }
+  return acc;
+}
+
+// The nb_steps == 0 loop of Gillespie.run (src/pyo3_gillespie.rs:209-223):
+// record (t, counts) before the first event and after every
+// _advance_one_reaction (src/gillespie.rs:275-297) while t < tmax.  That step
+// has no tmax check (the last event lies past tmax) and sets t = +inf in an
+// absorbing state.  Without rec_offsets the lane only counts its records
+// into n_records; with them it writes row r of trajectory j at
+// rec_offsets[j] + r (the stream is deterministic, so both passes agree).
+template <class Net>
+__device__ __forceinline__ LaneTotals lane_events(Net &net, const rebop_dev_run &run,
+                                                  uint64_t first, uint64_t queue_offset) {
+  const uint64_t n = run.n_traj;
+  const uint32_t S = net.S();
+  double x[Net::MS];
+  double cum[Net::MR];
+#pragma unroll
+  for (int s = 0; s < Net::MS; ++s) x[s] = 0.0;
+#pragma unroll
+  for (int r = 0; r < Net::MR; ++r) cum[r] = 0.0;
+  LaneTotals acc;
+  bool started = false;
+  for (;;) {
+    const uint64_t jl = next_trajectory(run, started, first, queue_offset);
+    if (jl >= n) break;
+    net.load_x0(x, run.x0_per_traj ? run.x0 + jl * S : run.x0);
+    if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
+    const bool write = run.rec_offsets != nullptr;
+    const uint64_t base = write ? run.rec_offsets[jl] : 0ull;
+    double t = 0.0;
+    uint64_t draw = 0, rec = 0;
+    auto emit = [&]() {
+      if (write) {
+        run.rec_times[base + rec] = t;
+        net.write_counts(x, run.rec_states + (base + rec) * S, 1);
+      }
+      rec += 1;
+    };
+    emit();
+    while (t < run.tmax) {
+      const double tot = net.props(x, cum);
+      if (!(0.0 < tot)) {
+        t = __builtin_inf();
+      } else {
+        const rebop_u32x4 blk = rebop_draw_block(run.seed, run.traj_begin + jl, draw);
+        draw += 1;
+        t = t + rebop_exp1_of(blk) / tot;
+        net.affect(x, net.choose(x, cum, tot * rebop_unif_of(blk)));
+        if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
+      }
+      emit();
+    }
+    if (!write) run.n_records[jl] = rec;
+    else if (run.n_records[jl] != rec) acc.flags |= REBOP_FLAG_RANGE;
+    acc.events += draw;
+    acc.draws += draw;
+    acc.trajectories += 1;
   }
   return acc;
 }

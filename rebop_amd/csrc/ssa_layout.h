@@ -55,6 +55,7 @@ typedef struct {
 /* modes of one launch */
 #define REBOP_MODE_GRID 0   /* sample at tmax*i/nb_steps, i = 0..nb_steps */
 #define REBOP_MODE_TARGET 1 /* one advance_until(tmax) from the stored state */
+#define REBOP_MODE_EVENTS 2 /* every event while t < tmax (nb_steps == 0) */
 
 typedef struct {
   uint64_t seed;
@@ -73,6 +74,12 @@ typedef struct {
   double *st_t;        /* [n] */
   uint64_t *st_draw;   /* [n] */
   unsigned long long *counters; /* [REBOP_CTR_COUNT] */
+  /* REBOP_MODE_EVENTS: records of trajectory j go to rows rec_offsets[j] ..
+   * (row = time + counts); with rec_offsets == NULL only n_records[j] is set */
+  uint64_t *n_records;         /* [n] */
+  const uint64_t *rec_offsets; /* [n] or NULL */
+  double *rec_times;           /* [total] */
+  int64_t *rec_states;         /* [total][S] */
 } rebop_dev_run;
 
 /*
@@ -107,6 +114,10 @@ typedef struct {
   double *st_x;
   double *st_t;
   uint64_t *st_draw;
+  uint64_t *n_records;
+  const uint64_t *rec_offsets;
+  double *rec_times;
+  int64_t *rec_states;
 } rebop_dev_job;
 
 /* a wave's unit of work: trajectories first .. first+count-1 of one job */

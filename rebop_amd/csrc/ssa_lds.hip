@@ -155,7 +155,8 @@ template <int MB>
 __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__restrict__ jobs,
                                                       const rebop_dev_item *__restrict__ items,
                                                       uint32_t n_items,
-                                                      unsigned long long *counters) {
+                                                      unsigned long long *counters,
+                                                      unsigned long long *job_events) {
   extern __shared__ int32_t lds_x[]; // [S_max][blockDim.x]
   __shared__ unsigned long long blk_ctr[4];
   if (threadIdx.x < 4) blk_ctr[threadIdx.x] = 0ull;
@@ -208,9 +209,21 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
     run.st_t = job.st_t;
     run.st_draw = job.st_draw;
     run.counters = counters;
+    run.n_records = job.n_records;
+    run.rec_offsets = job.rec_offsets;
+    run.rec_times = job.rec_times;
+    run.rec_states = job.rec_states;
 
     const uint64_t first = lane < item.count ? item.first + lane : job.n_traj;
-    const LaneTotals acc = lane_loop(net, run, first, 0ull);
+    const LaneTotals acc = run.mode == REBOP_MODE_EVENTS ? lane_events(net, run, first, 0ull)
+                                                         : lane_loop(net, run, first, 0ull);
+    if (job_events) {
+      // events of this item, summed over the wave, one atomic
+      unsigned long long e = acc.events;
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) e += __shfl_xor(e, off);
+      if (lane == 0) atomicAdd(&job_events[item.job], e);
+    }
     total.events += acc.events;
     total.draws += acc.draws;
     total.trajectories += acc.trajectories;
@@ -234,10 +247,10 @@ hipError_t lds_occupancy(int threads, size_t lds, int *blocks_per_cu) {
 }
 
 hipError_t launch_lds(const rebop_dev_job *jobs, const rebop_dev_item *items, uint32_t n_items,
-                      unsigned long long *counters, int blocks, int threads, size_t lds,
-                      hipStream_t stream) {
+                      unsigned long long *counters, unsigned long long *job_events, int blocks,
+                      int threads, size_t lds, hipStream_t stream) {
   hipLaunchKernelGGL((ssa_lds_kernel<kLdsCheckpoints>), dim3(blocks), dim3(threads), lds, stream,
-                     jobs, items, n_items, counters);
+                     jobs, items, n_items, counters, job_events);
   return hipGetLastError();
 }
 

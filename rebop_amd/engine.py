@@ -239,6 +239,29 @@ class GillespieNet:
         )
         return st if with_stats else None
 
+    def run_all_events(self, tmax: float, n_traj: int = 1, traj_begin: int = 0):
+        """The nb_steps == 0 loop (src/pyo3_gillespie.rs:209-223).
+
+        Returns ``(times, states)`` with ``states[record, species]`` for one
+        trajectory, or lists of them for ``n_traj > 1``.
+        """
+        S = self.nb_species()
+        n = int(n_traj)
+        counts = np.zeros(max(n, 1), dtype=np.uint64)
+        cp = counts.ctypes.data_as(POINTER(ctypes.c_uint64))
+        check(self._lib.rebop_run_all_events(self._h, float(tmax), int(traj_begin), n, cp, None, None, None))
+        total = int(counts[:n].sum())
+        times = np.zeros(max(total, 1), dtype=np.float64)
+        states = np.zeros((max(total, 1), S), dtype=np.int64)
+        check(self._lib.rebop_run_all_events(self._h, float(tmax), int(traj_begin), n, cp,
+                                             _f64(times), _i64(states), None))
+        offs = np.concatenate([[0], np.cumsum(counts[:n].astype(np.int64))])
+        ts = [times[offs[j]:offs[j + 1]] for j in range(n)]
+        xs = [states[offs[j]:offs[j + 1]] for j in range(n)]
+        if n == 1:
+            return ts[0], xs[0]
+        return ts, xs
+
     def batch(self, n_traj: int, traj_begin: int = 0, x0: np.ndarray | None = None) -> "Batch":
         return Batch(self, n_traj, traj_begin, x0)
 
@@ -288,3 +311,34 @@ class Batch:
     def set_time(self, t) -> None:
         a = np.ascontiguousarray(np.broadcast_to(np.asarray(t, dtype=np.float64), (self.n,)))
         check(self._lib.rebop_batch_set_time(self._h, _f64(a)))
+
+
+def run_jobs(jobs, stream: int | None = None, per_job_events: bool = False):
+    """``rebop_run_jobs_device``: many networks' grid runs in one launch.
+
+    ``jobs`` is a sequence of dicts with keys ``net`` (GillespieNet), ``tmax``,
+    ``nb_steps``, ``n_traj`` and optionally ``traj_begin``, ``out_ptr``,
+    ``x0_ptr``, ``final_ptr`` (device pointers) and ``cost_hint``.  Returns
+    the ``Stats``, plus each job's event count if ``per_job_events``.
+    """
+    lib = load()
+    arr = (_native.Job * len(jobs))()
+    keep = []
+    for a, j in zip(arr, jobs):
+        net = j["net"]
+        keep.append(net)
+        a.net = net.handle.value
+        a.tmax = float(j["tmax"])
+        a.nb_steps = int(j["nb_steps"])
+        a.traj_begin = int(j.get("traj_begin", 0))
+        a.n_traj = int(j["n_traj"])
+        a.d_x0 = j.get("x0_ptr") or None
+        a.d_out = j.get("out_ptr") or None
+        a.d_final = j.get("final_ptr") or None
+        a.cost_hint = float(j.get("cost_hint", 0.0))
+    st = Stats()
+    ev = np.zeros(len(jobs), dtype=np.uint64)
+    check(lib.rebop_run_jobs_device(arr, len(jobs), c_void_p(stream) if stream else None,
+                                    ctypes.byref(st), ev.ctypes.data_as(POINTER(ctypes.c_uint64))
+                                    if per_job_events else None))
+    return (st, ev) if per_job_events else st

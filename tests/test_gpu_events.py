@@ -1,0 +1,75 @@
+"""The nb_steps == 0 mode (every event; src/pyo3_gillespie.rs:209-223) on the
+device, bit for bit against the oracle, and the reference's
+test_all_reactions invariants through the Python API."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from rebop_amd import Gillespie, GillespieNet, Rate, _native
+from tests.networks import build_device, build_oracle, dimers, sir, synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize(
+    "name,spec,tmax,n",
+    [
+        ("sir", sir(), 250.0, 300),
+        ("dimers", dimers(), 0.05, 40),
+        ("syn40x80", synthetic(40, 80, 5, init_mean=25.0), 0.01, 30),  # LDS kernel
+    ],
+)
+def test_all_events_bit_exact(name, spec, tmax, n):
+    g = build_device(spec, 17)
+    ts, xs = g.run_all_events(tmax, n_traj=n, traj_begin=4)
+    o = build_oracle(spec)
+    for j in range(n):
+        ot, ox = o.run_all_events(17, 4 + j, spec[2], tmax)
+        assert np.array_equal(ts[j].view(np.uint64), ot.view(np.uint64)), (name, j)
+        assert np.array_equal(xs[j], ox), (name, j)
+
+
+def test_all_events_absorbing_records_inf():
+    g = GillespieNet([3], seed=2)
+    g.add_reaction(Rate.lma(1.0, [1]), [-1])
+    t, x = g.run_all_events(1e9)
+    assert t[0] == 0 and np.isinf(t[-1]) and x[-1, 0] == 0 and len(t) == 5
+
+
+def test_all_events_refuses_infinite_tmax():
+    g = GillespieNet([3], seed=2)
+    g.add_reaction(Rate.lma(1.0, [0]), [1])
+    with pytest.raises(_native.RebopValueError):
+        g.run_all_events(float("inf"))
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_python_all_reactions(seed):
+    """tests/test_rebop.py:39-52 through the Python API."""
+    s = Gillespie()
+    s.add_reaction(1e-4, ["S", "I"], ["I", "I"])
+    s.add_reaction(0.01, ["I"], ["R"])
+    ds = s.run({"S": 999, "I": 1}, tmax=250, nb_steps=0, rng=seed)
+    time = np.asarray(ds.time)
+    assert time[0] == 0 and time[-1] > 250
+    assert (np.diff(time) > 0).all()
+    S, I, R = (np.asarray(ds[v]) for v in "SIR")
+    if np.isinf(time[-1]):
+        S, I, R = S[:-1], I[:-1], R[:-1]
+    assert set(np.diff(S)) <= {-1, 0}
+    assert set(np.diff(I)) <= {-1, 1}
+    assert set(np.diff(R)) <= {0, 1}
+
+
+def test_python_var_names_all_events():
+    """tests/test_rebop.py:68-89 with nb_steps = 0."""
+    s = Gillespie()
+    s.add_reaction(1e-4, ["S", "I"], ["I", "I"])
+    s.add_reaction(0.01, ["I"], ["R"])
+    full = s.run({"S": 999, "I": 1}, tmax=250, nb_steps=0, rng=0)
+    sub = s.run({"S": 999, "I": 1}, tmax=250, nb_steps=0, rng=0, var_names=["S", "I"])
+    assert "R" not in sub
+    assert np.array_equal(np.asarray(sub["S"]), np.asarray(full["S"]))
+    assert np.array_equal(np.asarray(sub.time), np.asarray(full.time))

@@ -1,0 +1,139 @@
+"""GPU parity of the LDS-resident kernel (large networks, heterogeneous fused
+batches) against the CPU oracle, bit for bit."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from rebop_amd import GillespieNet, Rate, _native, workloads
+from rebop_amd.engine import run_jobs
+from tests.networks import build_device, build_oracle, dimers, michaelis_menten, sir, synthetic
+
+pytestmark = pytest.mark.gpu
+LDS = _native.REBOP_KERNEL_LDS
+
+
+def _dev(g, tmax, nb, n, kernel, x0=None, traj_begin=0):
+    import torch
+
+    S = g.nb_species()
+    out = torch.zeros((nb + 1, S, n), dtype=torch.int64, device="cuda")
+    x0_ptr = None
+    if x0 is not None:
+        xt = torch.as_tensor(np.ascontiguousarray(x0, dtype=np.int64), device="cuda")
+        x0_ptr = xt.data_ptr()
+    st = g.run_device(tmax, nb, n, out.data_ptr(), kernel=kernel, x0_ptr=x0_ptr, traj_begin=traj_begin)
+    torch.cuda.synchronize()
+    return out.cpu().numpy(), st
+
+
+@pytest.mark.parametrize(
+    "name,spec,tmax,nb,n",
+    [
+        ("sir", sir(), 250.0, 250, 700),
+        ("dimers", dimers(), 1.0, 100, 300),
+        ("mm", michaelis_menten(), 50.0, 32, 1000),
+        ("syn16x40", synthetic(16, 40, 3, init_mean=20.0), 5.0, 20, 300),
+        ("syn64x256", synthetic(64, 256, 11, init_mean=30.0), 0.05, 10, 200),
+    ],
+)
+def test_lds_kernel_bit_exact(name, spec, tmax, nb, n, nthreads):
+    g = build_device(spec, 77)
+    out, st = _dev(g, tmax, nb, n, LDS, traj_begin=5)
+    assert st.kernel == LDS
+    o = build_oracle(spec, sparse=True)  # same results as dense, faster
+    _, oout, _, oev, odr = o.run_grid(77, spec[2], tmax, nb, n_traj=n, traj_begin=5, nthreads=nthreads)
+    assert np.array_equal(out, oout), np.argwhere(out != oout)[:5].tolist()
+    assert st.events == oev and st.draws == odr
+
+
+def test_synthetic_128x512_auto_uses_lds(nthreads):
+    """The 128-species, 512-reaction network: AUTO selects the LDS kernel; a
+    short horizon is compared with the oracle bit for bit."""
+    spec = workloads.synthetic_large()
+    g = build_device(spec, 7)
+    out, st = _dev(g, 0.05, 20, 64, _native.REBOP_KERNEL_AUTO)
+    assert st.kernel == LDS
+    o = build_oracle(spec, sparse=True)
+    _, oout, _, oev, _ = o.run_grid(7, spec[2], 0.05, 20, n_traj=64, nthreads=nthreads)
+    assert np.array_equal(out, oout)
+    assert st.events == oev and st.events > 64 * 1000
+
+
+def test_lds_linear_search_paths(nthreads):
+    """Non-monotone cumulative sums: a negative rate constant, and a jump that
+    consumes more than the rate's order (counts can go negative); both use the
+    full linear count, which must match the oracle exactly."""
+    x0 = [1, 3, 40]
+    rx = [
+        (5.0, [1, 0, 0], [-2, 1, 0]),   # consumes 2 A at order 1: A goes negative
+        (-0.01, [0, 1, 0], [0, 0, 1]),  # negative constant: negative propensity
+        (0.001, [0, 0, 1], [1, 0, -1]),
+        (0.3, [0, 1, 1], [1, -1, 0]),
+    ]
+    spec = (3, rx, x0)
+    g = build_device(spec, 3)
+    for kernel in (LDS, _native.REBOP_KERNEL_PERSISTENT, _native.REBOP_KERNEL_PERSISTENT | _native.REBOP_KERNEL_SPECIALIZED):
+        out, st = _dev(g, 4.0, 16, 400, kernel)
+        o = build_oracle(spec)
+        _, oout, _, oev, _ = o.run_grid(3, x0, 4.0, 16, n_traj=400, nthreads=nthreads)
+        assert np.array_equal(out, oout), kernel
+        assert st.events == oev
+    assert (oout[:, 0, :] < 0).any()  # the negative-count path was exercised
+
+
+def test_lds_batch_advance_until(nthreads):
+    """The resumable batch state on a network beyond the register kernels."""
+    spec = synthetic(40, 80, 5, init_mean=25.0)
+    g = build_device(spec, 13)
+    o = build_oracle(spec, sparse=True)
+    b = g.batch(96, traj_begin=3)
+    for t in (0.01, 0.02, 0.005):
+        b.advance_until(t)
+    x = b.get_species()
+    for j in range(0, 96, 11):
+        tr = o.state(spec[2], 13, traj=3 + j)
+        for t in (0.01, 0.02, 0.005):
+            tr.advance_until(t)
+        assert x[j].tolist() == tr.x.tolist(), j
+
+
+def test_lds_per_trajectory_x0_and_range_guard(nthreads):
+    spec = synthetic(24, 48, 9, init_mean=10.0)
+    rng = np.random.default_rng(1)
+    x0 = rng.poisson(10, size=(200, 24))
+    g = build_device(spec, 4)
+    out, st = _dev(g, 0.5, 8, 200, LDS, x0=x0)
+    o = build_oracle(spec, sparse=True)
+    _, oout, _, _, _ = o.run_grid(4, x0, 0.5, 8, n_traj=200, x0_per_traj=True, nthreads=nthreads)
+    assert np.array_equal(out, oout)
+    # counts beyond the int32 LDS range are refused
+    big = GillespieNet([2**31, 0] + [0] * 38, seed=1)
+    big.add_reaction(Rate.lma(1.0, [1] + [0] * 39), [-1, 1] + [0] * 38)
+    with pytest.raises(_native.RebopValueError):
+        big.run(1.0, 2, n_traj=4)
+
+
+def test_fused_heterogeneous_jobs_bit_exact(nthreads):
+    """A fused launch over many different networks equals each network's own
+    oracle run; per-job event counts are exact."""
+    import torch
+
+    specs = workloads.heterogeneous_specs(24, seed=5, s_range=(3, 64), r_range=(4, 256))
+    nets = workloads.build_nets(specs, seed=31)
+    targets = workloads.target_events(len(specs), lo=50, hi=400)
+    tmax = [t / max(workloads.initial_total_propensity(sp), 1e-9) for sp, t in zip(specs, targets)]
+    n = 96
+    outs = [torch.zeros((6, sp[0], n), dtype=torch.int64, device="cuda") for sp in specs]
+    jobs = [dict(net=g, tmax=float(t), nb_steps=5, n_traj=n, traj_begin=10 * i, out_ptr=o.data_ptr(),
+                 cost_hint=float(i)) for i, (g, t, o) in enumerate(zip(nets, tmax, outs))]
+    st, ev = run_jobs(jobs, per_job_events=True)
+    torch.cuda.synchronize()
+    assert st.trajectories == n * len(specs)
+    for i, (sp, t) in enumerate(zip(specs, tmax)):
+        o = build_oracle(sp, sparse=True)
+        _, oout, _, oev, _ = o.run_grid(31, sp[2], float(t), 5, n_traj=n, traj_begin=10 * i, nthreads=nthreads)
+        assert np.array_equal(outs[i].cpu().numpy(), oout), i
+        assert int(ev[i]) == oev, i
+    assert int(ev.sum()) == st.events

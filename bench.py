@@ -214,6 +214,70 @@ def make_workload(name: str, kernel: int) -> Workload:
     raise SystemExit(f"unknown workload {name}")
 
 
+PEAK_BF16_DENSE = 2.5e15
+
+
+def surrogate_main(args) -> int:
+    """Surrogate workload: 10^5 log-uniform Michaelis-Menten parameter points x
+    64 SSA trajectories at 32 times in [0, 50] (one sweep launch), training of
+    the 5-layer width-512 BF16 MLP with an 8-component negative-binomial
+    mixture head, W1 validation against 10^4-trajectory SSA histograms, and
+    10^6 queries timed on the device.  ``value`` is surrogate queries/s; the
+    GPU SSA's queries/s at the same statistical quality (10^4 trajectories
+    per point, shared by its 32 times) is reported beside it."""
+    import torch
+
+    from rebop_amd import _native, surrogate
+
+    dev_name, cus = _native.device_info()
+    res = surrogate.benchmark(n_train=args.surrogate_points, steps=args.surrogate_train_steps,
+                              log=lambda m: print(m, file=sys.stderr, flush=True))
+    torch.cuda.synchronize()
+    flops_per_query = 2.0 * sum(a * b for a, b in zip(surrogate.MLP_DIMS[:-1], surrogate.MLP_DIMS[1:]))
+    achieved = res["gemm_tflops"]
+    line = {
+        "metric": "surrogate distribution queries/s on 1 MI355X (vs GPU SSA at 10^4 trajectories)",
+        "value": res["surrogate_qps"],
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": res["surrogate_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 (fp32 accumulate)",
+        "data": "synthetic",
+        "config": {
+            "workload": "Michaelis-Menten surrogate: %d training points x 64 trajectories x 32 "
+                        "times, 5-layer width-512 MLP, 8-component NB mixture per species, "
+                        "10^6 queries" % args.surrogate_points,
+            "device": dev_name,
+            "compute_units": cus,
+            "ssa_queries_per_s": res["ssa_qps"],
+            "speedup_vs_gpu_ssa": res["speedup"],
+            "w1_pass_fraction": res["w1_pass_fraction"],
+            "point_pass_fraction": res["point_pass_fraction"],
+            "trusted_cells": res["trusted_cells"],
+            "rel_w1_median": res["rel_w1_median"],
+            "train_s": res["train_s"],
+            "nll_first": res["nll_first"],
+            "nll_last": res["nll_last"],
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": PEAK_BF16_DENSE / 1e12,
+            "unit": "TFLOP/s",
+            "frac": achieved * 1e12 / PEAK_BF16_DENSE,
+            "traffic": None,
+            "flops_per_query": flops_per_query,
+        },
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,8 +285,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--kernel", type=int, default=0, help="REBOP_KERNEL_* (0 = auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="dimer", choices=["dimer", "sir", "syn", "het"],
-                    help="dimer (the headline configuration), sir, syn (128x512), het (4096 networks)")
+    ap.add_argument("--workload", default="dimer", choices=["dimer", "sir", "syn", "het", "surrogate"],
+                    help="dimer (the headline configuration), sir, syn (128x512), het (4096 "
+                         "networks), surrogate (Michaelis-Menten sweep -> MLP -> W1 gate)")
+    ap.add_argument("--surrogate-points", type=int, default=100_000)
+    ap.add_argument("--surrogate-train-steps", type=int, default=1500)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one device copy of 1 GiB (read 1 GiB, write 1 GiB) and exit; "
                          "scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
@@ -242,6 +309,9 @@ def main() -> int:
               f"communication, and this build runs on a single MI355X (--gpus {args.gpus} "
               "is not supported)", file=sys.stderr)
         return 2
+
+    if args.workload == "surrogate":
+        return surrogate_main(args)
 
     import torch
 

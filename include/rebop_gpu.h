@@ -104,6 +104,24 @@ int rebop_gillespie_add_reaction_lma_sparse(rebop_gillespie *g, double rate,
  * counterpart in the reference (which has no device code to prepare). */
 int rebop_gillespie_prepare(const rebop_gillespie *g, int kernel);
 
+/* The autotuner's choice: which kernel and launch shape AUTO runs. */
+typedef struct rebop_tune_result {
+  int32_t kernel;   /* REBOP_KERNEL_* */
+  int32_t threads;  /* workgroup size: 64, 128 or 256 */
+  double occupancy; /* fraction of the resident workgroups a persistent launch uses */
+  double ms;        /* pilot kernel time of that choice */
+  uint32_t n_tried; /* variants timed */
+} rebop_tune_result;
+
+/* Times every applicable kernel variant (specialised / data-driven register,
+ * persistent / plain, workgroup size 64/128/256, full or half resident grid,
+ * LDS) on a pilot of n_pilot trajectories of this network (no output), and
+ * makes REBOP_KERNEL_AUTO runs of this handle use the fastest.  All variants
+ * give identical results, so the choice only affects speed.  No counterpart
+ * in the reference. */
+int rebop_autotune(rebop_gillespie *g, double tmax, uint64_t nb_steps, uint64_t n_pilot,
+                   rebop_tune_result *out);
+
 /* Gillespie::set_species / get_species on the initial state -- src/gillespie.rs:260-267. */
 int rebop_gillespie_set_species(rebop_gillespie *g, const int64_t *species, uint32_t n);
 int rebop_gillespie_get_species(const rebop_gillespie *g, int64_t *species, uint32_t n);
@@ -179,6 +197,24 @@ typedef struct rebop_job {
  */
 int rebop_run_jobs_device(const rebop_job *jobs, uint32_t n_jobs, void *stream,
                           rebop_stats *stats, uint64_t *events_per_job);
+
+/*
+ * A parameter sweep over one network shape: n_points x traj_per_point
+ * trajectories in one launch.  Trajectory q = p * traj_per_point + j (global
+ * id traj_begin + q, stream (seed, traj_begin + q)) runs rebop_run_device's
+ * grid loop with rate constants d_rates[p][0..R) (NULL: the network's own)
+ * and initial counts d_x0[p][0..S) (NULL: the handle's species); d_out and
+ * d_final are laid out as for rebop_run_device with n_traj = n_points *
+ * traj_per_point.  Device pointers.  After rebop_gillespie_prepare the call
+ * makes no host-device copy and no allocation, so with stats == NULL it can
+ * be captured into a hipGraph and replayed (e.g. after rewriting d_rates).
+ * Register and specialised kernels only (S <= 32, R <= 64).  The reference's
+ * counterpart is one `Gillespie.run` per parameter point.
+ */
+int rebop_run_sweep_device(const rebop_gillespie *g, uint64_t n_points, uint64_t traj_per_point,
+                           const double *d_rates, const int64_t *d_x0, double tmax,
+                           uint64_t nb_steps, uint64_t traj_begin, int64_t *d_out,
+                           int64_t *d_final, int kernel, void *stream, rebop_stats *stats);
 
 /* Batched resumable state: N trajectories of one network, each with its own
  * (species, t, rng) -- the fields of gillespie::Gillespie (src/gillespie.rs:157-163)

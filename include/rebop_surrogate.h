@@ -1,0 +1,74 @@
+/*
+ * rebop_surrogate.h -- C ABI of the surrogate's device kernels.
+ *
+ * No counterpart in the reference (rebop has no learned component).  These
+ * are the building blocks the training and inference loop (Python,
+ * rebop_amd/surrogate.py) calls: a BF16 matrix-core GEMM with FP32
+ * accumulation, bias/activation epilogues, the ReLU derivative, column sums,
+ * Adam, the input featurisation of (parameter point, sample time) items, and
+ * the negative-binomial-mixture likelihood head with its gradient.  Every
+ * pointer is a device pointer; `stream` is a hipStream_t (NULL = default).
+ * Functions return 0, 3 (invalid shape) or 4 (HIP error; see
+ * rebop_sg_last_error).
+ */
+#ifndef REBOP_SURROGATE_H
+#define REBOP_SURROGATE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define REBOP_SG_COMPONENTS 8 /* mixture components per species */
+
+const char *rebop_sg_last_error(void);
+
+/* C[M][N] (FP32, ldc) = op(A) op(B) with BF16 A and B.  ta: A is stored
+ * [K][M] (else [M][K]); tb: B is stored [N][K] (else [K][N]).  M and N
+ * multiples of 64, K a multiple of 32; leading dimensions in elements,
+ * multiples of 8. */
+int rebop_sg_gemm_bf16(int ta, int tb, int M, int N, int K, const void *A, int lda, const void *B,
+                       int ldb, float *C, int ldc, void *stream);
+
+/* H = bf16(act(C + bias)) and/or F = act(C + bias) (either may be NULL). */
+int rebop_sg_bias_act(int M, int N, const float *C, int ldc, const float *bias, int relu, void *H,
+                      int ldh, float *F, int ldf, void *stream);
+
+/* dH *= (H > 0) in place (FP32), and dZ = bf16(dH). */
+int rebop_sg_relu_grad(int M, int N, float *dH, int ldd, const void *H, int ldh, void *dZ, int ldz,
+                       void *stream);
+
+int rebop_sg_cast_bf16(size_t n, const float *x, void *y, void *stream);
+
+/* out[n] = sum over m of X[m][n]. */
+int rebop_sg_colsum(int M, int N, const float *X, int ldx, float *out, void *stream);
+
+/* One Adam step on n FP32 weights; wb (nullable) receives their BF16 copy.
+ * bc1, bc2: the bias corrections 1 - b1^t and 1 - b2^t. */
+int rebop_sg_adam(size_t n, float *w, const float *g, float *m, float *v, void *wb, float lr,
+                  float b1, float b2, float eps, float bc1, float bc2, void *stream);
+
+/* X[b][0..ldx) (BF16) = features of item b: log10 of the 3 rate constants of
+ * parameter point item_point[b], its initial E and S counts and the time of
+ * sample item_sample[b] (tmax * i / nb_steps), scaled to about [-1, 1]. */
+int rebop_sg_features(int B, const uint32_t *item_point, const uint32_t *item_sample,
+                      const double *rates, const int64_t *x0, double tmax, int nb_steps, void *X,
+                      int ldx, void *stream);
+
+/* Negative log-likelihood of J SSA samples per item under the predicted
+ * per-species K-component negative-binomial mixtures (K = REBOP_SG_COMPONENTS),
+ * and its gradient: grad[b][...] = scale * d NLL / d head[b][...];
+ * *loss_sum += NLL.  samples: an SSA output [nb+1][S][n_traj_total] in which
+ * item (p, i) owns trajectories p*J .. p*J+J-1 of sample i. */
+int rebop_sg_nb_mixture_nll(int B, int S, int K, int J, const float *head, int ldh,
+                            const uint32_t *item_point, const uint32_t *item_sample,
+                            const int64_t *samples, uint64_t n_traj_total, float *grad,
+                            double *loss_sum, double scale, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,104 @@
+"""FP64 CPU reference of the surrogate's computations.
+
+TEST INFRASTRUCTURE ONLY: the checker of the device kernels in
+rebop_amd/csrc/surrogate.hip (loaded by tests/ only).
+
+- ``bf16``: round-to-nearest-even to bfloat16, as the device casts do.
+- ``forward`` / ``backward``: the ReLU MLP of rebop_amd.surrogate in FP64;
+  with ``emulate_bf16`` the operands the device stores in BF16 (inputs,
+  activations, weights, back-propagated gradients) are rounded the same way,
+  so the only remaining difference is FP32 vs FP64 accumulation.
+- ``nll``: the negative-binomial-mixture negative log-likelihood and its
+  gradient w.r.t. the head, with scipy's gammaln / digamma.
+- ``features``: the input scaling of features_kernel.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+from scipy.special import digamma, gammaln
+
+
+def bf16(x: np.ndarray) -> np.ndarray:
+    """Round (via float32) to the nearest bfloat16, ties to even; returned as float64."""
+    f = np.asarray(x, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    lsb = (u >> 16) & 1
+    u = ((u + 0x7FFF + lsb) >> 16) << 16
+    out = (u.astype(np.uint32)).view(np.float32).astype(np.float64)
+    nan = np.isnan(f)
+    out[nan] = np.nan
+    return out
+
+
+def features(rates: np.ndarray, x0: np.ndarray, sample: np.ndarray, tmax: float, nb: int,
+             width: int) -> np.ndarray:
+    B = len(sample)
+    X = np.zeros((B, width))
+    X[:, 0:3] = (np.log10(rates) + 1.0) * 0.5
+    X[:, 3] = (x0[:, 0] - 105.0) / 95.0
+    X[:, 4] = (x0[:, 1] - 525.0) / 475.0
+    X[:, 5] = 2.0 * (tmax * sample / nb) / tmax - 1.0
+    return X
+
+
+def forward(X, Ws, bs, emulate_bf16=True):
+    """Returns (head, activations) in FP64."""
+    r = bf16 if emulate_bf16 else (lambda a: np.asarray(a, dtype=np.float64))
+    H = [r(X)]
+    for l, (W, b) in enumerate(zip(Ws, bs)):
+        Z = H[-1] @ r(W) + b
+        if l < len(Ws) - 1:
+            H.append(r(np.maximum(Z, 0.0)))
+        else:
+            head = Z
+    return head, H
+
+
+def backward(dhead, H, Ws, emulate_bf16=True):
+    r = bf16 if emulate_bf16 else (lambda a: np.asarray(a, dtype=np.float64))
+    L = len(Ws)
+    dW, db = [None] * L, [None] * L
+    dH = np.asarray(dhead, dtype=np.float64)
+    for l in reversed(range(L)):
+        if l < L - 1:
+            dH = dH * (H[l + 1] > 0)
+        db[l] = dH.sum(axis=0)
+        dZ = r(dH)
+        dW[l] = H[l].T @ dZ
+        if l > 0:
+            dH = dZ @ r(Ws[l]).T
+    return dW, db
+
+
+def nll(head: np.ndarray, samples: np.ndarray, K: int):
+    """head [B][>= S*3K], samples [B][S][J] -> (total NLL, d NLL / d head)."""
+    B, S, J = samples.shape
+    grad = np.zeros_like(head, dtype=np.float64)
+    total = 0.0
+    for s in range(S):
+        h = head[:, s * 3 * K:(s + 1) * 3 * K].astype(np.float64)
+        logits, lmu_raw, lr_raw = h[:, :K], h[:, K:2 * K], h[:, 2 * K:]
+        lmu = np.clip(lmu_raw, -8.0, 9.0)
+        lr = np.clip(lr_raw, -6.0, 12.0)
+        mu, r = np.exp(lmu), np.exp(lr)
+        lz = np.logaddexp.reduce(logits, axis=1, keepdims=True)
+        lpi = logits - lz  # [B][K]
+        x = samples[:, s, :].astype(np.float64)[:, :, None]  # [B][J][1]
+        mu3, r3 = mu[:, None, :], r[:, None, :]
+        lrm = np.log(r3 + mu3)
+        ll = gammaln(x + r3) - gammaln(r3) - gammaln(x + 1.0) + r3 * (lr[:, None, :] - lrm) + x * (lmu[:, None, :] - lrm)
+        lt = lpi[:, None, :] + ll  # [B][J][K]
+        lp = np.logaddexp.reduce(lt, axis=2, keepdims=True)
+        total -= lp.sum()
+        gam = np.exp(lt - lp)
+        pi = np.exp(lpi)[:, None, :]
+        gl = (pi - gam).sum(axis=1)
+        gm = (-gam * r3 * (x - mu3) / (r3 + mu3)).sum(axis=1)
+        gr = (-gam * r3 * (digamma(x + r3) - digamma(r3) + (lr[:, None, :] - lrm) + (mu3 - x) / (r3 + mu3))).sum(axis=1)
+        gm[lmu_raw != lmu] = 0.0
+        gr[lr_raw != lr] = 0.0
+        grad[:, s * 3 * K:s * 3 * K + K] = gl
+        grad[:, s * 3 * K + K:s * 3 * K + 2 * K] = gm
+        grad[:, s * 3 * K + 2 * K:(s + 1) * 3 * K] = gr
+    return total, grad

@@ -65,6 +65,12 @@ SIGNATURES: dict[str, tuple] = {
         c_int,
         [c_void_p, c_double, c_uint64, c_uint64, POINTER(c_uint64), POINTER(c_double), POINTER(c_int64), c_void_p],
     ),
+    "rebop_autotune": (c_int, [c_void_p, c_double, c_uint64, c_uint64, c_void_p]),
+    "rebop_run_sweep_device": (
+        c_int,
+        [c_void_p, c_uint64, c_uint64, c_void_p, c_void_p, c_double, c_uint64, c_uint64, c_void_p,
+         c_void_p, c_int, c_void_p, c_void_p],
+    ),
     "rebop_run_jobs_device": (c_int, [c_void_p, c_uint32, c_void_p, c_void_p, POINTER(c_uint64)]),
     "rebop_batch_new": (c_int, [c_void_p, c_uint64, c_uint64, POINTER(c_int64), POINTER(c_void_p)]),
     "rebop_batch_free": (None, [c_void_p]),
@@ -87,6 +93,18 @@ class Stats(ctypes.Structure):
         ("trajectories", c_uint64),
         ("kernel", c_uint64),
         ("kernel_ms", c_double),
+    ]
+
+
+class TuneResult(ctypes.Structure):
+    """``rebop_tune_result`` of include/rebop_gpu.h."""
+
+    _fields_ = [
+        ("kernel", ctypes.c_int32),
+        ("threads", ctypes.c_int32),
+        ("occupancy", c_double),
+        ("ms", c_double),
+        ("n_tried", c_uint32),
     ]
 
 

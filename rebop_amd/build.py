@@ -16,6 +16,8 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "librebop_gpu.so"
+SG_LIB = HERE / "librebop_surrogate.so"
+SG_SOURCES = ["surrogate.hip"]
 ARCH = "gfx950"
 
 SOURCES = ["ssa_kernels.hip", "ssa_lds.hip", "rebop_gpu.cpp", "jit.cpp"]
@@ -81,6 +83,17 @@ def build(verbose: bool = False) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    # the surrogate's kernels (dense layers on the matrix cores; contraction is
+    # allowed there: they are not part of the bit-exact SSA path)
+    sg_tmp = SG_LIB.with_suffix(".so.tmp")
+    sg_flags = [f for f in FLAGS if f != "-ffp-contract=off"]
+    cmd = [hipcc(), *sg_flags, "-shared", "-o", str(sg_tmp), *[str(CSRC / s) for s in SG_SOURCES]]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stdout}\n{res.stderr}")
+    os.replace(sg_tmp, SG_LIB)
     return LIB
 
 

@@ -43,6 +43,7 @@ struct JitReaction {
 struct JitKernel {
   hipModule_t mod = nullptr;
   hipFunction_t fn = nullptr;
+  hipFunction_t fn_sweep = nullptr;
 };
 std::string jit_source(uint32_t S, const std::vector<JitReaction> &rx);
 int jit_get(const std::string &src, JitKernel *out, std::string *err);
@@ -156,6 +157,10 @@ struct rebop_gillespie {
   rebop_dev_net view{};
   uint32_t F = 0;
   DevBuf counters;
+  // the autotuner's choice for AUTO launches (kernel < 0: not tuned)
+  int tune_kernel = -1;
+  int tune_threads = 256;
+  double tune_occ = 1.0;
   // the network part of an LDS-kernel job (sparse jump, checkpoint blocks)
   rebop_dev_job job{};
   DevBuf djobs, ditems;
@@ -424,9 +429,17 @@ int finish_stats(rebop_stats *stats, unsigned long long *counters, hipEvent_t ev
 // Enqueues one SSA launch (plus the counter reset) on `stream`.  When
 // `stats` is set, synchronises and fills it.
 int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t stream,
-               rebop_stats *stats) {
+               rebop_stats *stats, int threads = 0, double occ = 1.0) {
   int rc = ensure_device();
   if (rc) return rc;
+  if (kernel == REBOP_KERNEL_AUTO && g->tune_kernel >= 0 && threads == 0) {
+    kernel = g->tune_kernel;
+    threads = g->tune_threads;
+    occ = g->tune_occ;
+  }
+  if (threads == 0) threads = kThreads;
+  if (threads != 64 && threads != 128 && threads != 256)
+    return fail(REBOP_ERR_VALUE, "workgroup size must be 64, 128 or 256");
   rc = upload_net(g);
   if (rc) return rc;
   const uint32_t S = g->S, R = (uint32_t)g->rx.size();
@@ -450,6 +463,9 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
                                            std::to_string(R) +
                                            " reactions exceeds the register kernels");
   }
+  if (use_lds && run.rates)
+    return fail(REBOP_ERR_UNSUPPORTED,
+                "parameter sweeps run on the register and specialised kernels (S <= 32, R <= 64)");
   if (use_lds) {
     // LDS kernel: one job, items of 64 trajectories
     rebop_dev_job job = g->job;
@@ -459,7 +475,7 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
     job.tmax = run.tmax;
     job.nb_steps = run.nb_steps;
     job.mode = run.mode;
-    job.x0_per_traj = run.x0_per_traj;
+    job.x0_div = run.x0_div;
     job.x0 = run.x0;
     job.out = run.out;
     job.final_x = run.final_x;
@@ -496,16 +512,17 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
   run.persistent = persistent ? 1 : 0;
   run.counters = (unsigned long long *)g->counters.p;
   const size_t lds = spec ? 0 : sizeof(double) * (size_t)R * S;
-  const uint64_t need_blocks = (run.n_traj + kThreads - 1) / kThreads;
+  const uint64_t need_blocks = (run.n_traj + threads - 1) / threads;
   uint64_t blocks = need_blocks;
   if (persistent) {
     int per_cu = 0;
     if (spec)
-      HIP_TRY(rebop::jit_occupancy(g->jit, kThreads, &per_cu));
+      HIP_TRY(rebop::jit_occupancy(g->jit, threads, &per_cu));
     else
-      HIP_TRY(rebop::reg_occupancy(shape, kThreads, lds, &per_cu));
+      HIP_TRY(rebop::reg_occupancy(shape, threads, lds, &per_cu));
     if (per_cu < 1) per_cu = 1;
-    const uint64_t resident = (uint64_t)per_cu * (uint64_t)g_dev.cus;
+    uint64_t resident = (uint64_t)((double)per_cu * (double)g_dev.cus * occ);
+    if (resident < (uint64_t)g_dev.cus) resident = (uint64_t)g_dev.cus;
     if (blocks > resident) blocks = resident;
   }
   if (blocks == 0) blocks = 1;
@@ -522,9 +539,9 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
   if (stats) HIP_TRY(hipEventRecord(ev0, stream));
   if (run.n_traj > 0) {
     if (spec)
-      HIP_TRY(rebop::jit_launch(g->jit, run, (int)blocks, kThreads, stream));
+      HIP_TRY(rebop::jit_launch(g->jit, run, (int)blocks, threads, stream));
     else
-      HIP_TRY(rebop::launch_reg(shape, g->view, run, (int)blocks, kThreads, lds, stream));
+      HIP_TRY(rebop::launch_reg(shape, g->view, run, (int)blocks, threads, lds, stream));
   }
   if (stats) HIP_TRY(hipEventRecord(ev1, stream));
 
@@ -688,6 +705,7 @@ int rebop_gillespie_prepare(const rebop_gillespie *gc, int kernel) {
     return REBOP_OK;
   }
   if (int rc = ensure_device()) return rc;
+  if (int rc = upload_net(g)) return rc;
   return prepare_jit(g);
 }
 
@@ -715,7 +733,111 @@ int rebop_run_device(const rebop_gillespie *gc, double tmax, uint64_t nb_steps,
   run.nb_steps = nb_steps;
   run.mode = REBOP_MODE_GRID;
   run.x0 = d_x0 ? d_x0 : (const int64_t *)g->dx0.p;
-  run.x0_per_traj = d_x0 ? 1 : 0;
+  run.x0_div = d_x0 ? 1 : 0;
+  run.out = d_out;
+  run.final_x = d_final;
+  return launch_run(g, run, kernel, (hipStream_t)stream, stats);
+}
+
+int rebop_autotune(rebop_gillespie *g, double tmax, uint64_t nb_steps, uint64_t n_pilot,
+                   rebop_tune_result *out) {
+  if (int rc = check_handle(g, "gillespie")) return rc;
+  if (nb_steps == 0) return fail(REBOP_ERR_VALUE, "nb_steps must be > 0");
+  if (n_pilot == 0) return fail(REBOP_ERR_VALUE, "n_pilot must be > 0");
+  if (int rc = check_x0(g->x0.data(), g->x0.size())) return rc;
+  int rc = ensure_device();
+  if (rc) return rc;
+  rc = upload_net(g);
+  if (rc) return rc;
+  const uint32_t S = g->S, R = (uint32_t)g->rx.size();
+  struct Cand {
+    int kernel, threads;
+    double occ;
+  };
+  std::vector<Cand> cands;
+  const bool jit_ok = jit_eligible(g) && !jit_disabled() && prepare_jit(g) == REBOP_OK;
+  const bool reg_ok = pick_reg_shape(S, R) >= 0;
+  for (int threads : {64, 128, 256}) {
+    for (double occ : {1.0, 0.5}) {
+      if (jit_ok)
+        cands.push_back({REBOP_KERNEL_PERSISTENT | REBOP_KERNEL_SPECIALIZED, threads, occ});
+      if (reg_ok) cands.push_back({REBOP_KERNEL_PERSISTENT, threads, occ});
+    }
+    if (jit_ok) cands.push_back({REBOP_KERNEL_PLAIN | REBOP_KERNEL_SPECIALIZED, threads, 1.0});
+    if (reg_ok) cands.push_back({REBOP_KERNEL_PLAIN, threads, 1.0});
+  }
+  cands.push_back({REBOP_KERNEL_LDS, 64, 1.0});
+  rebop_dev_run run{};
+  run.seed = g->seed;
+  run.traj_begin = 0;
+  run.n_traj = n_pilot;
+  run.tmax = tmax;
+  run.nb_steps = nb_steps;
+  run.mode = REBOP_MODE_GRID;
+  run.x0 = (const int64_t *)g->dx0.p;
+  const int saved = g->tune_kernel;
+  g->tune_kernel = -1;
+  double best = 1e300;
+  Cand pick = cands.back();
+  uint32_t tried = 0;
+  for (const Cand &c : cands) {
+    rebop_stats st{};
+    // one warm launch, then the timed one
+    rc = launch_run(g, run, c.kernel, nullptr, &st, c.threads, c.occ);
+    if (rc == REBOP_OK) rc = launch_run(g, run, c.kernel, nullptr, &st, c.threads, c.occ);
+    if (rc != REBOP_OK) continue; // a variant that does not apply
+    tried++;
+    if (st.kernel_ms < best) {
+      best = st.kernel_ms;
+      pick = c;
+    }
+  }
+  if (tried == 0) {
+    g->tune_kernel = saved;
+    return fail(REBOP_ERR_UNSUPPORTED, "no kernel variant ran for this network");
+  }
+  g->tune_kernel = pick.kernel;
+  g->tune_threads = pick.threads;
+  g->tune_occ = pick.occ;
+  if (out) {
+    out->kernel = pick.kernel;
+    out->threads = pick.threads;
+    out->occupancy = pick.occ;
+    out->ms = best;
+    out->n_tried = tried;
+  }
+  return REBOP_OK;
+}
+
+int rebop_run_sweep_device(const rebop_gillespie *gc, uint64_t n_points, uint64_t traj_per_point,
+                           const double *d_rates, const int64_t *d_x0, double tmax,
+                           uint64_t nb_steps, uint64_t traj_begin, int64_t *d_out,
+                           int64_t *d_final, int kernel, void *stream, rebop_stats *stats) {
+  if (int rc = check_handle(gc, "gillespie")) return rc;
+  rebop_gillespie *g = const_cast<rebop_gillespie *>(gc);
+  if (nb_steps == 0) return fail(REBOP_ERR_VALUE, "nb_steps must be > 0 for a grid run");
+  if (traj_per_point == 0) return fail(REBOP_ERR_VALUE, "traj_per_point must be > 0");
+  if (n_points && traj_per_point > UINT64_MAX / n_points)
+    return fail(REBOP_ERR_VALUE, "n_points * traj_per_point overflows");
+  if (!d_x0) {
+    if (int rc = check_x0(g->x0.data(), g->x0.size())) return rc;
+  }
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  int rc = ensure_device();
+  if (rc) return rc;
+  rc = upload_net(g);
+  if (rc) return rc;
+  rebop_dev_run run{};
+  run.seed = g->seed;
+  run.traj_begin = traj_begin;
+  run.n_traj = n_points * traj_per_point;
+  run.tmax = tmax;
+  run.nb_steps = nb_steps;
+  run.mode = REBOP_MODE_GRID;
+  run.x0 = d_x0 ? d_x0 : (const int64_t *)g->dx0.p;
+  run.x0_div = d_x0 ? traj_per_point : 0;
+  run.rates = d_rates;
+  run.rates_div = traj_per_point;
   run.out = d_out;
   run.final_x = d_final;
   return launch_run(g, run, kernel, (hipStream_t)stream, stats);
@@ -779,7 +901,7 @@ int rebop_run_jobs_device(const rebop_job *jobs, uint32_t n_jobs, void *stream_,
     d.tmax = J.tmax;
     d.nb_steps = J.nb_steps;
     d.mode = REBOP_MODE_GRID;
-    d.x0_per_traj = J.d_x0 ? 1 : 0;
+    d.x0_div = J.d_x0 ? 1 : 0;
     d.x0 = J.d_x0 ? J.d_x0 : (const int64_t *)g->dx0.p;
     d.out = J.d_out;
     d.final_x = J.d_final;
@@ -840,7 +962,7 @@ int rebop_run_all_events(const rebop_gillespie *gc, double tmax, uint64_t traj_b
   run.nb_steps = 0;
   run.mode = REBOP_MODE_EVENTS;
   run.x0 = (const int64_t *)g->dx0.p;
-  run.x0_per_traj = 0;
+  run.x0_div = 0;
   run.n_records = (uint64_t *)dcount.p;
   const bool want = times != nullptr || states != nullptr;
   rebop_stats local{};

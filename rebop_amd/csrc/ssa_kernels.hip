@@ -22,7 +22,14 @@ struct GenericNet {
   static constexpr int MR = MR_;
   rebop_dev_net net;
   const double *lds_diff; // [R][S]
+  double kr[MR];          // this trajectory's rate constants
 
+  // the network's constants, or (parameter sweeps) this trajectory's row
+  __device__ __forceinline__ void start(const rebop_dev_run &run, uint64_t j) {
+    const double *src = run.rates ? run.rates + (j / run.rates_div) * net.R : net.k;
+#pragma unroll
+    for (int r = 0; r < MR; ++r) kr[r] = r < (int)net.R ? src[r] : 0.0;
+  }
   __device__ __forceinline__ uint32_t S() const { return net.S; }
   __device__ __forceinline__ void load_x0(double (&x)[MS], const int64_t *src) const {
     reg_load_x0(x, src, net.S);
@@ -44,7 +51,7 @@ struct GenericNet {
 #pragma unroll
     for (int r = 0; r < MR; ++r) {
       if (r < (int)net.R) {
-        double a = net.k[r];
+        double a = kr[r];
         const uint32_t fb = net.fac_begin[r];
         const uint32_t fe = net.fac_begin[r + 1];
         for (uint32_t f = fb; f < fe; ++f) a = a * (x[net.fac_s[f]] - net.fac_c[f]);
@@ -80,7 +87,9 @@ __global__ void __launch_bounds__(256) ssa_reg_kernel(rebop_dev_net net, rebop_d
   for (uint32_t q = threadIdx.x; q < net.R * net.S; q += blockDim.x) lds_diff[q] = net.diff[q];
   if (threadIdx.x < 4) blk_ctr[threadIdx.x] = 0ull;
   __syncthreads();
-  const GenericNet<MS, MR> pol{net, lds_diff};
+  GenericNet<MS, MR> pol;
+  pol.net = net;
+  pol.lds_diff = lds_diff;
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
   const LaneTotals acc = run.mode == REBOP_MODE_EVENTS ? lane_events(pol, run, gtid, nthreads)

@@ -23,6 +23,7 @@
 // that the compiler keeps them in VGPRs even when x is indexed by a
 // wave-uniform runtime species index):
 //   uint32_t S() const;
+//   void start(const rebop_dev_run &run, uint64_t j);    per-trajectory setup (rates)
 //   void load_x0(X &x, const int64_t *src);                counts from an int64 row
 //   void load_state(X &x, const double *st, uint64_t stride);  st[s * stride]
 //   void store_state(const X &x, double *st, uint64_t stride);
@@ -145,12 +146,13 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
       const uint64_t nj = next_trajectory(run, started, first, queue_offset);
       if (nj >= n) break;
       jl = nj;
+      net.start(run, jl);
       if (run.st_x) {
         net.load_state(x, run.st_x + jl, n);
         t = run.st_t[jl];
         draw = run.st_draw[jl];
       } else {
-        net.load_x0(x, run.x0_per_traj ? run.x0 + jl * net.S() : run.x0);
+        net.load_x0(x, run.x0_div ? run.x0 + (jl / run.x0_div) * net.S() : run.x0);
         t = 0.0;
         draw = 0;
       }
@@ -237,7 +239,8 @@ __device__ __forceinline__ LaneTotals lane_events(Net &net, const rebop_dev_run 
   for (;;) {
     const uint64_t jl = next_trajectory(run, started, first, queue_offset);
     if (jl >= n) break;
-    net.load_x0(x, run.x0_per_traj ? run.x0 + jl * S : run.x0);
+    net.start(run, jl);
+    net.load_x0(x, run.x0_div ? run.x0 + (jl / run.x0_div) * S : run.x0);
     if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
     const bool write = run.rec_offsets != nullptr;
     const uint64_t base = write ? run.rec_offsets[jl] : 0ull;

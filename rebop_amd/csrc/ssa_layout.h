@@ -65,9 +65,10 @@ typedef struct {
   uint64_t nb_steps;
   int32_t mode;
   int32_t persistent;
-  const int64_t *x0;   /* [S] shared (x0_per_traj == 0) or [n][S] */
-  int32_t x0_per_traj;
-  int32_t pad;
+  const int64_t *x0;   /* [S] shared (x0_div == 0) or row j / x0_div of [*][S] */
+  uint64_t x0_div;
+  const double *rates; /* NULL, or row j / rates_div of [*][R]: per-trajectory rate constants */
+  uint64_t rates_div;
   int64_t *out;        /* [nb+1][S][n] or NULL */
   int64_t *final_x;    /* [S][n] or NULL */
   double *st_x;        /* resumable state [S][n], or NULL */
@@ -107,7 +108,8 @@ typedef struct {
   double tmax;
   uint64_t nb_steps;
   int32_t mode;
-  int32_t x0_per_traj;
+  int32_t pad2;
+  uint64_t x0_div;
   const int64_t *x0;
   int64_t *out;
   int64_t *final_x;

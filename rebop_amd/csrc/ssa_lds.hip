@@ -56,6 +56,7 @@ struct LdsNet {
   __device__ __forceinline__ double fcnt(uint32_t s) const { return (double)xl[(uint64_t)s * BL]; }
 
   __device__ __forceinline__ uint32_t S() const { return nS; }
+  __device__ __forceinline__ void start(const rebop_dev_run &, uint64_t) {}
 
   __device__ __forceinline__ void set_count(uint32_t s, double v) {
     if (!(__builtin_fabs(v) <= (double)REBOP_LDS_LIMIT)) {
@@ -202,7 +203,7 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
     run.mode = job.mode;
     run.persistent = 0;
     run.x0 = job.x0;
-    run.x0_per_traj = job.x0_per_traj;
+    run.x0_div = job.x0_div;
     run.out = job.out;
     run.final_x = job.final_x;
     run.st_x = job.st_x;

@@ -1,0 +1,356 @@
+// surrogate.hip -- kernels of the learned surrogate of the SSA.
+//
+// A multilayer perceptron maps (log rate constants, initial counts, t) of a
+// Michaelis-Menten network to a K-component negative-binomial mixture per
+// species.  Its dense layers run on the matrix cores: BF16 operands, FP32
+// accumulation (v_mfma_f32_16x16x32_bf16).  The loss head (mixture negative
+// log-likelihood of the SSA samples and its gradient) runs in FP64.  Adam
+// keeps FP32 master weights and writes the BF16 copy the GEMMs read.
+//
+// MFMA operand layout used below (16x16x32, wave64): lane l supplies
+//   A[row = l % 16][k = 8 (l / 16) .. +8]   (8 bf16, k contiguous)
+//   B[k = 8 (l / 16) .. +8][col = l % 16]
+// and receives D[row = 4 (l / 16) + q][col = l % 16], q = 0..3.
+// The GEMM test checks the product against a CPU reference.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "../../include/rebop_surrogate.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+namespace sg {
+
+// ---------------------------------------------------------------------------
+// C = op(A) op(B), M x N x K, FP32 out.  !TA: A[m][k] (lda); TA: A stored
+// [k][m].  !TB: B[k][n] (ldb); TB: B stored [n][k].  Tiles 64 x 64 x 32, four
+// waves, each 32 x 32 (2 x 2 MFMA tiles).  M, N multiples of 64, K of 32.
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256) gemm_kernel(int M, int N, int K, const __bf16 *__restrict__ A,
+                                                   int lda, const __bf16 *__restrict__ B, int ldb,
+                                                   float *__restrict__ C, int ldc) {
+  __shared__ __attribute__((aligned(16))) __bf16 As[64][40]; // [m][k], rows padded to 80 B
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[64][40]; // [n][k]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  floatx4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    if (!TA) {
+      const int r = tid >> 2, c = (tid & 3) * 8;
+      *(bf16x8 *)&As[r][c] = *(const bf16x8 *)&A[(size_t)(m0 + r) * lda + k0 + c];
+    } else {
+      const int kk = tid >> 3, c = (tid & 7) * 8;
+      const bf16x8 v = *(const bf16x8 *)&A[(size_t)(k0 + kk) * lda + m0 + c];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) As[c + q][kk] = v[q];
+    }
+    if (TB) {
+      const int r = tid >> 2, c = (tid & 3) * 8;
+      *(bf16x8 *)&Bs[r][c] = *(const bf16x8 *)&B[(size_t)(n0 + r) * ldb + k0 + c];
+    } else {
+      const int kk = tid >> 3, c = (tid & 7) * 8;
+      const bf16x8 v = *(const bf16x8 *)&B[(size_t)(k0 + kk) * ldb + n0 + c];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) Bs[c + q][kk] = v[q];
+    }
+    __syncthreads();
+    bf16x8 a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) a[i] = *(const bf16x8 *)&As[wm + i * 16 + (lane & 15)][(lane >> 4) * 8];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) b[j] = *(const bf16x8 *)&Bs[wn + j * 16 + (lane & 15)][(lane >> 4) * 8];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = m0 + wm + i * 16 + (lane >> 4) * 4 + q;
+        const int col = n0 + wn + j * 16 + (lane & 15);
+        C[(size_t)row * ldc + col] = acc[i][j][q];
+      }
+}
+
+// H = bf16(act(C + bias)), act = ReLU or identity; optionally the FP32 result too.
+__global__ void bias_act_kernel(int M, int N, const float *__restrict__ C, int ldc,
+                                const float *__restrict__ bias, int relu, __bf16 *__restrict__ H,
+                                int ldh, float *__restrict__ F, int ldf) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)M * N) return;
+  const int m = (int)(idx / N), n = (int)(idx % N);
+  float v = C[(size_t)m * ldc + n] + bias[n];
+  if (relu) v = v > 0.f ? v : 0.f;
+  if (H) H[(size_t)m * ldh + n] = (__bf16)v;
+  if (F) F[(size_t)m * ldf + n] = v;
+}
+
+// dZ = dH * (H > 0): the ReLU derivative from the stored activation; FP32 in
+// place (for the bias gradient) and BF16 (for the next GEMMs).
+__global__ void relu_grad_kernel(int M, int N, float *__restrict__ dH, int ldd,
+                                 const __bf16 *__restrict__ H, int ldh, __bf16 *__restrict__ dZ,
+                                 int ldz) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)M * N) return;
+  const int m = (int)(idx / N), n = (int)(idx % N);
+  const float h = (float)H[(size_t)m * ldh + n];
+  const float g = h > 0.f ? dH[(size_t)m * ldd + n] : 0.f;
+  dH[(size_t)m * ldd + n] = g;
+  dZ[(size_t)m * ldz + n] = (__bf16)g;
+}
+
+__global__ void cast_bf16_kernel(size_t n, const float *__restrict__ x, __bf16 *__restrict__ y) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (__bf16)x[i];
+}
+
+// out[n] = sum_m X[m][n]: one thread per column, rows strided (coalesced).
+__global__ void colsum_kernel(int M, int N, const float *__restrict__ X, int ldx,
+                              float *__restrict__ out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  float s = 0.f;
+  for (int m = 0; m < M; ++m) s += X[(size_t)m * ldx + n];
+  out[n] = s;
+}
+
+// Adam with FP32 master weights; writes the BF16 copy read by the GEMMs.
+__global__ void adam_kernel(size_t n, float *__restrict__ w, const float *__restrict__ g,
+                            float *__restrict__ m, float *__restrict__ v, __bf16 *__restrict__ wb,
+                            float lr, float b1, float b2, float eps, float bc1, float bc2) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+  w[i] = wi;
+  if (wb) wb[i] = (__bf16)wi;
+}
+
+// Inputs of item b: (point p, sample i) -> [log10 k (3), E0, S0, t] scaled to
+// about [-1, 1]; columns 6..ldx-1 are zero.
+__global__ void features_kernel(int B, const uint32_t *__restrict__ item_point,
+                                const uint32_t *__restrict__ item_sample, const double *__restrict__ rates,
+                                const int64_t *__restrict__ x0, double tmax, int nb_steps,
+                                __bf16 *__restrict__ X, int ldx) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const uint32_t p = item_point[b], i = item_sample[b];
+  float f[6];
+  for (int r = 0; r < 3; ++r) f[r] = (float)((log10(rates[3 * p + r]) + 1.0) * 0.5);
+  f[3] = (float)((x0[4 * p + 0] - 105.0) / 95.0);
+  f[4] = (float)((x0[4 * p + 1] - 525.0) / 475.0);
+  f[5] = (float)(2.0 * (tmax * (double)i / (double)nb_steps) / tmax - 1.0);
+  for (int c = 0; c < ldx; ++c) X[(size_t)b * ldx + c] = (__bf16)(c < 6 ? f[c] : 0.f);
+}
+
+// ---- special functions (FP64): recurrence to z >= 10, then the asymptotic
+// series of ln Gamma (Stirling) and of the digamma function, with the
+// Bernoulli-number coefficients B_2n / (2n (2n-1)) and B_2n / (2n).
+__device__ __forceinline__ double lgamma_pos(double z) {
+  double shift = 0.0;
+  while (z < 10.0) {
+    shift += log(z);
+    z += 1.0;
+  }
+  const double iz = 1.0 / z, iz2 = iz * iz;
+  const double series =
+      iz * (1.0 / 12.0 - iz2 * (1.0 / 360.0 - iz2 * (1.0 / 1260.0 - iz2 * (1.0 / 1680.0 - iz2 * (1.0 / 1188.0)))));
+  return (z - 0.5) * log(z) - z + 0.91893853320467274178 + series - shift;
+}
+
+__device__ __forceinline__ double digamma_pos(double z) {
+  double shift = 0.0;
+  while (z < 10.0) {
+    shift += 1.0 / z;
+    z += 1.0;
+  }
+  const double iz = 1.0 / z, iz2 = iz * iz;
+  const double series =
+      iz2 * (1.0 / 12.0 - iz2 * (1.0 / 120.0 - iz2 * (1.0 / 252.0 - iz2 * (1.0 / 240.0 - iz2 * (1.0 / 132.0)))));
+  return log(z) - 0.5 * iz - series - shift;
+}
+
+// Negative log-likelihood of the SSA samples under the predicted mixtures,
+// and its gradient w.r.t. the head outputs.  One wave per item, one sample
+// per lane (64 samples).  Head layout per species s (K components):
+// [s*3K + c] mixture logit, [s*3K + K + c] log mean, [s*3K + 2K + c] log
+// dispersion.  samples: the SSA output [nb+1][S][P * J], item (p, i) reads
+// samples[(i*S + s) * P*J + p*J + j].  grad = d(mean NLL over items and
+// samples)/d head (FP32, [B][ldh]); loss_sum += total NLL (FP64).
+template <int K>
+__global__ void __launch_bounds__(64) nb_mixture_nll_kernel(
+    int B, int S, int J, const float *__restrict__ head, int ldh,
+    const uint32_t *__restrict__ item_point, const uint32_t *__restrict__ item_sample,
+    const int64_t *__restrict__ samples, uint64_t n_traj_total, float *__restrict__ grad,
+    double *__restrict__ loss_sum, double scale) {
+  const int b = blockIdx.x;
+  const int j = threadIdx.x;
+  if (b >= B) return;
+  const uint32_t p = item_point[b], i = item_sample[b];
+  const bool active = j < J;
+  double lane_loss = 0.0;
+  for (int s = 0; s < S; ++s) {
+    const float *h = head + (size_t)b * ldh + (size_t)s * 3 * K;
+    const double x = active ? (double)samples[((size_t)i * S + s) * n_traj_total + (size_t)p * J + j] : 0.0;
+    // log softmax of the logits (same on every lane)
+    double lmax = -1e300;
+#pragma unroll
+    for (int c = 0; c < K; ++c) lmax = fmax(lmax, (double)h[c]);
+    double zs = 0.0;
+#pragma unroll
+    for (int c = 0; c < K; ++c) zs += exp((double)h[c] - lmax);
+    const double lz = lmax + log(zs);
+    double lterm[K], mu[K], r[K], lr[K], lmu[K];
+    const double lgx1 = lgamma_pos(x + 1.0);
+    double best = -1e300;
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      lmu[c] = fmin(fmax((double)h[K + c], -8.0), 9.0);
+      lr[c] = fmin(fmax((double)h[2 * K + c], -6.0), 12.0);
+      mu[c] = exp(lmu[c]);
+      r[c] = exp(lr[c]);
+      const double lrm = log(r[c] + mu[c]);
+      const double ll = lgamma_pos(x + r[c]) - lgamma_pos(r[c]) - lgx1 + r[c] * (lr[c] - lrm) +
+                        x * (lmu[c] - lrm);
+      lterm[c] = ((double)h[c] - lz) + ll;
+      best = fmax(best, lterm[c]);
+    }
+    double se = 0.0;
+#pragma unroll
+    for (int c = 0; c < K; ++c) se += exp(lterm[c] - best);
+    const double lp = best + log(se);
+    if (active) lane_loss -= lp;
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      const double gam = active ? exp(lterm[c] - lp) : 0.0;
+      const double pi = exp((double)h[c] - lz);
+      const double rm = r[c] + mu[c];
+      double gl = active ? (pi - gam) : 0.0;
+      double gm = -gam * r[c] * (x - mu[c]) / rm;
+      double gr = -gam * r[c] *
+                  (digamma_pos(x + r[c]) - digamma_pos(r[c]) + (lr[c] - log(rm)) + (mu[c] - x) / rm);
+      if ((double)h[K + c] != lmu[c]) gm = 0.0; // clamped
+      if ((double)h[2 * K + c] != lr[c]) gr = 0.0;
+      // sum over the wave's samples
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        gl += __shfl_xor(gl, off);
+        gm += __shfl_xor(gm, off);
+        gr += __shfl_xor(gr, off);
+      }
+      if (j == 0) {
+        float *g = grad + (size_t)b * ldh + (size_t)s * 3 * K;
+        g[c] = (float)(gl * scale);
+        g[K + c] = (float)(gm * scale);
+        g[2 * K + c] = (float)(gr * scale);
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) lane_loss += __shfl_xor(lane_loss, off);
+  if (j == 0) atomicAdd(loss_sum, lane_loss);
+}
+
+} // namespace sg
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+namespace {
+hipError_t g_err = hipSuccess;
+int ret(hipError_t e) {
+  g_err = e;
+  return e == hipSuccess ? 0 : 4;
+}
+unsigned nblk(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+} // namespace
+
+extern "C" {
+
+const char *rebop_sg_last_error(void) { return hipGetErrorString(g_err); }
+
+int rebop_sg_gemm_bf16(int ta, int tb, int M, int N, int K, const void *A, int lda, const void *B,
+                       int ldb, float *C, int ldc, void *stream) {
+  if (M % 64 || N % 64 || K % 32 || M <= 0 || N <= 0 || K <= 0) return 3;
+  const dim3 grid(N / 64, M / 64);
+  const __bf16 *a = (const __bf16 *)A, *b = (const __bf16 *)B;
+  hipStream_t s = (hipStream_t)stream;
+  if (!ta && !tb) hipLaunchKernelGGL((sg::gemm_kernel<false, false>), grid, dim3(256), 0, s, M, N, K, a, lda, b, ldb, C, ldc);
+  else if (!ta && tb) hipLaunchKernelGGL((sg::gemm_kernel<false, true>), grid, dim3(256), 0, s, M, N, K, a, lda, b, ldb, C, ldc);
+  else if (ta && !tb) hipLaunchKernelGGL((sg::gemm_kernel<true, false>), grid, dim3(256), 0, s, M, N, K, a, lda, b, ldb, C, ldc);
+  else hipLaunchKernelGGL((sg::gemm_kernel<true, true>), grid, dim3(256), 0, s, M, N, K, a, lda, b, ldb, C, ldc);
+  return ret(hipGetLastError());
+}
+
+int rebop_sg_bias_act(int M, int N, const float *C, int ldc, const float *bias, int relu, void *H,
+                      int ldh, float *F, int ldf, void *stream) {
+  hipLaunchKernelGGL(sg::bias_act_kernel, dim3(nblk((size_t)M * N, 256)), dim3(256), 0,
+                     (hipStream_t)stream, M, N, C, ldc, bias, relu, (__bf16 *)H, ldh, F, ldf);
+  return ret(hipGetLastError());
+}
+
+int rebop_sg_relu_grad(int M, int N, float *dH, int ldd, const void *H, int ldh, void *dZ, int ldz,
+                       void *stream) {
+  hipLaunchKernelGGL(sg::relu_grad_kernel, dim3(nblk((size_t)M * N, 256)), dim3(256), 0,
+                     (hipStream_t)stream, M, N, dH, ldd, (const __bf16 *)H, ldh, (__bf16 *)dZ, ldz);
+  return ret(hipGetLastError());
+}
+
+int rebop_sg_cast_bf16(size_t n, const float *x, void *y, void *stream) {
+  hipLaunchKernelGGL(sg::cast_bf16_kernel, dim3(nblk(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                     n, x, (__bf16 *)y);
+  return ret(hipGetLastError());
+}
+
+int rebop_sg_colsum(int M, int N, const float *X, int ldx, float *out, void *stream) {
+  hipLaunchKernelGGL(sg::colsum_kernel, dim3(nblk((size_t)N, 256)), dim3(256), 0,
+                     (hipStream_t)stream, M, N, X, ldx, out);
+  return ret(hipGetLastError());
+}
+
+int rebop_sg_adam(size_t n, float *w, const float *g, float *m, float *v, void *wb, float lr,
+                  float b1, float b2, float eps, float bc1, float bc2, void *stream) {
+  hipLaunchKernelGGL(sg::adam_kernel, dim3(nblk(n, 256)), dim3(256), 0, (hipStream_t)stream, n, w,
+                     g, m, v, (__bf16 *)wb, lr, b1, b2, eps, bc1, bc2);
+  return ret(hipGetLastError());
+}
+
+int rebop_sg_features(int B, const uint32_t *item_point, const uint32_t *item_sample,
+                      const double *rates, const int64_t *x0, double tmax, int nb_steps, void *X,
+                      int ldx, void *stream) {
+  hipLaunchKernelGGL(sg::features_kernel, dim3(nblk((size_t)B, 256)), dim3(256), 0,
+                     (hipStream_t)stream, B, item_point, item_sample, rates, x0, tmax, nb_steps,
+                     (__bf16 *)X, ldx);
+  return ret(hipGetLastError());
+}
+
+int rebop_sg_nb_mixture_nll(int B, int S, int K, int J, const float *head, int ldh,
+                            const uint32_t *item_point, const uint32_t *item_sample,
+                            const int64_t *samples, uint64_t n_traj_total, float *grad,
+                            double *loss_sum, double scale, void *stream) {
+  if (K != REBOP_SG_COMPONENTS || J > 64 || J <= 0) return 3;
+  hipLaunchKernelGGL((sg::nb_mixture_nll_kernel<REBOP_SG_COMPONENTS>), dim3(B), dim3(64), 0,
+                     (hipStream_t)stream, B, S, J, head, ldh, item_point, item_sample, samples,
+                     n_traj_total, grad, loss_sum, scale);
+  return ret(hipGetLastError());
+}
+
+} // extern "C"

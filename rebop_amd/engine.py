@@ -169,6 +169,12 @@ class GillespieNet:
         """Compile (and on a device, load) the kernel a run will use."""
         check(self._lib.rebop_gillespie_prepare(self._h, int(kernel)))
 
+    def autotune(self, tmax: float, nb_steps: int, n_pilot: int = 65536) -> "_native.TuneResult":
+        """``rebop_autotune``: AUTO runs of this network then use the fastest variant."""
+        res = _native.TuneResult()
+        check(self._lib.rebop_autotune(self._h, float(tmax), int(nb_steps), int(n_pilot), ctypes.byref(res)))
+        return res
+
     def get_species(self) -> np.ndarray:
         out = np.zeros(self.nb_species(), dtype=np.int64)
         check(self._lib.rebop_gillespie_get_species(self._h, _i64(out), len(out)))
@@ -261,6 +267,27 @@ class GillespieNet:
         if n == 1:
             return ts[0], xs[0]
         return ts, xs
+
+    def run_sweep_device(self, n_points: int, traj_per_point: int, tmax: float, nb_steps: int,
+                         out_ptr: int | None, rates_ptr: int | None = None,
+                         x0_ptr: int | None = None, final_ptr: int | None = None,
+                         traj_begin: int = 0, kernel: int = _native.REBOP_KERNEL_AUTO,
+                         stream: int | None = None, with_stats: bool = True) -> Stats | None:
+        """``rebop_run_sweep_device``: per-point rate constants / initial counts."""
+        st = Stats()
+        check(
+            self._lib.rebop_run_sweep_device(
+                self._h, int(n_points), int(traj_per_point),
+                c_void_p(rates_ptr) if rates_ptr else None,
+                c_void_p(x0_ptr) if x0_ptr else None,
+                float(tmax), int(nb_steps), int(traj_begin),
+                c_void_p(out_ptr) if out_ptr else None,
+                c_void_p(final_ptr) if final_ptr else None,
+                int(kernel), c_void_p(stream) if stream else None,
+                ctypes.byref(st) if with_stats else None,
+            )
+        )
+        return st if with_stats else None
 
     def batch(self, n_traj: int, traj_begin: int = 0, x0: np.ndarray | None = None) -> "Batch":
         return Batch(self, n_traj, traj_begin, x0)

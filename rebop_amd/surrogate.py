@@ -1,0 +1,477 @@
+"""Learned surrogate of the SSA for the Michaelis-Menten network.
+
+E + S -> ES @ kf, ES -> E + S @ kb, ES -> E + P @ kcat (species E, S, ES, P).
+A multilayer perceptron maps (log10 kf, log10 kb, log10 kcat, E0, S0, t) to
+a K-component negative-binomial mixture for each species.  It is trained on
+trajectories this engine generates (``rebop_run_sweep_device``: one
+launch over every parameter point) by the project's own device kernels
+(include/rebop_surrogate.h): BF16 matrix-core GEMMs with FP32 accumulation
+for the dense layers, an FP64 mixture-likelihood head, Adam.
+
+The SSA stays the judge and the fallback: ``TrustMap`` validates the
+surrogate against fresh SSA histograms (10^4 trajectories per point) on a
+grid of parameter cells; a query is answered by the surrogate only inside
+cells where every validated Wasserstein-1 distance was <= 2 % of the species
+mean, and by the SSA elsewhere.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from ctypes import c_double, c_float, c_int, c_size_t, c_uint64, c_void_p
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from . import _native
+from .engine import GillespieNet, Rate
+
+LIB_PATH = Path(__file__).resolve().parent / "librebop_surrogate.so"
+K = 8  # mixture components per species (REBOP_SG_COMPONENTS)
+S = 4  # species E, S, ES, P
+IN = 64  # input features, 6 used, padded for the matrix cores
+OUT = 128  # head width, S * 3 * K = 96 used
+WIDTH = 512
+TMAX = 50.0
+NB = 32  # samples at t = 50 i / 32, i = 1..32
+J = 64  # SSA trajectories per parameter point in the training data
+MLP_DIMS = [IN] + [WIDTH] * 4 + [OUT]  # five dense layers
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    _native.load()  # one HIP runtime with torch (see _native.load)
+    if not LIB_PATH.exists():
+        raise _native.RebopError(4, f"{LIB_PATH} is missing: build with `python -m rebop_amd.build`")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    i, u, v, d, f, z = c_int, ctypes.c_uint32, c_void_p, c_double, c_float, c_size_t
+    sig = {
+        "rebop_sg_last_error": (ctypes.c_char_p, []),
+        "rebop_sg_gemm_bf16": (i, [i, i, i, i, i, v, i, v, i, v, i, v]),
+        "rebop_sg_bias_act": (i, [i, i, v, i, v, i, v, i, v, i, v]),
+        "rebop_sg_relu_grad": (i, [i, i, v, i, v, i, v, i, v]),
+        "rebop_sg_cast_bf16": (i, [z, v, v, v]),
+        "rebop_sg_colsum": (i, [i, i, v, i, v, v]),
+        "rebop_sg_adam": (i, [z, v, v, v, v, v, f, f, f, f, f, f, v]),
+        "rebop_sg_features": (i, [i, v, v, v, v, d, i, v, i, v]),
+        "rebop_sg_nb_mixture_nll": (i, [i, i, i, i, v, i, v, v, v, c_uint64, v, v, d, v]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ck(rc: int) -> None:
+    if rc != 0:
+        raise _native.RebopError(rc, load().rebop_sg_last_error().decode())
+
+
+def _p(t) -> int:
+    return t.data_ptr()
+
+
+# ---------------------------------------------------------------------------
+# thin wrappers of the device kernels (torch tensors as device buffers)
+
+
+def gemm(A, B, C, ta=False, tb=False, stream=None):
+    """C[M][N] = op(A) op(B); A, B bfloat16, C float32, all row-major."""
+    M, N = C.shape
+    Kd = A.shape[0] if ta else A.shape[1]
+    _ck(load().rebop_sg_gemm_bf16(int(ta), int(tb), M, N, Kd, _p(A), A.shape[1], _p(B), B.shape[1],
+                                  _p(C), C.shape[1], stream))
+
+
+def bias_act(C, bias, relu, H=None, F=None, stream=None):
+    M, N = C.shape
+    _ck(load().rebop_sg_bias_act(M, N, _p(C), C.shape[1], _p(bias), int(relu),
+                                 _p(H) if H is not None else None, H.shape[1] if H is not None else 0,
+                                 _p(F) if F is not None else None, F.shape[1] if F is not None else 0,
+                                 stream))
+
+
+def relu_grad(dH, H, dZ, stream=None):
+    M, N = dH.shape
+    _ck(load().rebop_sg_relu_grad(M, N, _p(dH), dH.shape[1], _p(H), H.shape[1], _p(dZ), dZ.shape[1], stream))
+
+
+def cast_bf16(x, y, stream=None):
+    _ck(load().rebop_sg_cast_bf16(x.numel(), _p(x), _p(y), stream))
+
+
+def colsum(X, out, stream=None):
+    M, N = X.shape
+    _ck(load().rebop_sg_colsum(M, N, _p(X), X.shape[1], _p(out), stream))
+
+
+def adam(w, g, m, v, wb, lr, b1, b2, eps, t, stream=None):
+    bc1 = 1.0 - b1**t
+    bc2 = 1.0 - b2**t
+    _ck(load().rebop_sg_adam(w.numel(), _p(w), _p(g), _p(m), _p(v), _p(wb) if wb is not None else None,
+                             lr, b1, b2, eps, bc1, bc2, stream))
+
+
+def features(item_point, item_sample, rates, x0, X, stream=None):
+    B = item_point.numel()
+    _ck(load().rebop_sg_features(B, _p(item_point), _p(item_sample), _p(rates), _p(x0), TMAX, NB,
+                                 _p(X), X.shape[1], stream))
+
+
+def nb_mixture_nll(head, item_point, item_sample, samples, grad, loss_sum, scale, stream=None):
+    B = item_point.numel()
+    _ck(load().rebop_sg_nb_mixture_nll(B, S, K, J, _p(head), head.shape[1], _p(item_point),
+                                       _p(item_sample), _p(samples), samples.shape[2], _p(grad),
+                                       _p(loss_sum), float(scale), stream))
+
+
+# ---------------------------------------------------------------------------
+# training data
+
+
+def mm_network(seed: int = 0) -> GillespieNet:
+    """The Michaelis-Menten network; its constants are replaced per point."""
+    g = GillespieNet([100, 500, 0, 0], seed=seed)
+    g.add_reaction(Rate.lma(1.0, [1, 1, 0, 0]), [-1, -1, 1, 0])
+    g.add_reaction(Rate.lma(1.0, [0, 0, 1, 0]), [1, 1, -1, 0])
+    g.add_reaction(Rate.lma(1.0, [0, 0, 1, 0]), [1, 0, -1, 1])
+    return g
+
+
+def sample_points(n: int, seed: int):
+    """Rate constants log-uniform in [1e-3, 1e1]; E0 ~ U{10..200}, S0 ~ U{50..1000}."""
+    rng = np.random.default_rng(seed)
+    rates = 10.0 ** rng.uniform(-3.0, 1.0, size=(n, 3))
+    x0 = np.zeros((n, 4), dtype=np.int64)
+    x0[:, 0] = rng.integers(10, 201, n)
+    x0[:, 1] = rng.integers(50, 1001, n)
+    return rates, x0
+
+
+@dataclass
+class SweepData:
+    rates: "torch.Tensor"  # [P][3] float64
+    x0: "torch.Tensor"  # [P][4] int64
+    samples: "torch.Tensor"  # [NB+1][S][P*n_traj] int64
+    n_traj: int
+    stats: _native.Stats
+
+
+def generate(rates: np.ndarray, x0: np.ndarray, n_traj: int, seed: int, traj_begin: int = 0) -> SweepData:
+    """n_traj SSA trajectories per point, sampled at t = 50 i / 32 (one launch)."""
+    import torch
+
+    P = len(rates)
+    g = mm_network(seed)
+    d_rates = torch.as_tensor(np.ascontiguousarray(rates, dtype=np.float64), device="cuda")
+    d_x0 = torch.as_tensor(np.ascontiguousarray(x0, dtype=np.int64), device="cuda")
+    out = torch.empty((NB + 1, S, P * n_traj), dtype=torch.int64, device="cuda")
+    st = g.run_sweep_device(P, n_traj, TMAX, NB, out.data_ptr(), rates_ptr=d_rates.data_ptr(),
+                            x0_ptr=d_x0.data_ptr(), traj_begin=traj_begin,
+                            stream=torch.cuda.current_stream().cuda_stream)
+    return SweepData(d_rates, d_x0, out, n_traj, st)
+
+
+# ---------------------------------------------------------------------------
+# the network
+
+
+class MLP:
+    """Dense ReLU network IN -> WIDTH (x4) -> OUT with FP32 master weights."""
+
+    def __init__(self, seed: int = 0, width: int = WIDTH, depth: int = 5):
+        import torch
+
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        dims = [IN] + [width] * (depth - 1) + [OUT]
+        self.dims = dims
+        self.W, self.b, self.Wb = [], [], []
+        for l in range(depth):
+            fan_in = 6 if l == 0 else dims[l]
+            w = torch.randn(dims[l], dims[l + 1], generator=g, dtype=torch.float64) * math.sqrt(2.0 / fan_in)
+            if l == 0:
+                w[6:, :] = 0.0  # padded inputs
+            if l == depth - 1:
+                w *= 0.1
+                w[:, S * 3 * K:] = 0.0  # padded head columns
+            b = torch.zeros(dims[l + 1], dtype=torch.float64)
+            if l == depth - 1:
+                for s in range(S):
+                    b[s * 3 * K + K: s * 3 * K + 2 * K] = math.log(100.0)  # log mean
+                    b[s * 3 * K + 2 * K: s * 3 * K + 3 * K] = math.log(10.0)  # log dispersion
+            self.W.append(w.float().cuda())
+            self.b.append(b.float().cuda())
+            self.Wb.append(self.W[-1].to(torch.bfloat16))
+        self.mW = [torch.zeros_like(w) for w in self.W]
+        self.vW = [torch.zeros_like(w) for w in self.W]
+        self.mb = [torch.zeros_like(b) for b in self.b]
+        self.vb = [torch.zeros_like(b) for b in self.b]
+        self.t = 0
+
+    @property
+    def depth(self) -> int:
+        return len(self.W)
+
+    def forward(self, X, keep: bool = True):
+        """X [B][IN] bf16 -> head [B][OUT] float32 (activations kept for backward)."""
+        import torch
+
+        B = X.shape[0]
+        H = [X]
+        for l in range(self.depth):
+            C = torch.empty((B, self.dims[l + 1]), dtype=torch.float32, device="cuda")
+            gemm(H[-1], self.Wb[l], C)
+            if l < self.depth - 1:
+                h = torch.empty((B, self.dims[l + 1]), dtype=torch.bfloat16, device="cuda")
+                bias_act(C, self.b[l], True, H=h)
+                H.append(h)
+            else:
+                bias_act(C, self.b[l], False, F=C)
+                head = C
+        if keep:
+            self._H = H
+        return head
+
+    def backward(self, dhead):
+        """dhead [B][OUT] float32 -> (dW, db) per layer."""
+        import torch
+
+        H = self._H
+        B = dhead.shape[0]
+        dW = [None] * self.depth
+        db = [None] * self.depth
+        dH = dhead.clone()
+        for l in reversed(range(self.depth)):
+            dZ = torch.empty((B, self.dims[l + 1]), dtype=torch.bfloat16, device="cuda")
+            if l == self.depth - 1:
+                cast_bf16(dH, dZ)
+            else:
+                relu_grad(dH, H[l + 1], dZ)
+            db[l] = torch.empty(self.dims[l + 1], dtype=torch.float32, device="cuda")
+            colsum(dH, db[l])
+            dW[l] = torch.empty((self.dims[l], self.dims[l + 1]), dtype=torch.float32, device="cuda")
+            gemm(H[l], dZ, dW[l], ta=True)
+            if l > 0:
+                dH = torch.empty((B, self.dims[l]), dtype=torch.float32, device="cuda")
+                gemm(dZ, self.Wb[l], dH, tb=True)
+        return dW, db
+
+    def step(self, dW, db, lr: float, b1=0.9, b2=0.999, eps=1e-8):
+        self.t += 1
+        for l in range(self.depth):
+            adam(self.W[l], dW[l], self.mW[l], self.vW[l], self.Wb[l], lr, b1, b2, eps, self.t)
+            adam(self.b[l], db[l], self.mb[l], self.vb[l], None, lr, b1, b2, eps, self.t)
+
+
+def train(mlp: MLP, data: SweepData, steps: int, batch: int = 4096, lr: float = 1e-3,
+          seed: int = 0, log_every: int = 0):
+    """Adam on the mean mixture NLL of the SSA samples; returns the loss history."""
+    import torch
+
+    P = data.rates.shape[0]
+    n_items = P * NB
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    X = torch.empty((batch, IN), dtype=torch.bfloat16, device="cuda")
+    grad = torch.zeros((batch, OUT), dtype=torch.float32, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    hist = []
+    for it in range(steps):
+        q = torch.randint(0, n_items, (batch,), generator=gen, device="cuda")
+        ip = (q // NB).to(torch.int32)  # read as uint32 by the kernels
+        isamp = (q % NB + 1).to(torch.int32)
+        features(ip, isamp, data.rates, data.x0, X)
+        head = mlp.forward(X)
+        grad.zero_()
+        loss.zero_()
+        nb_mixture_nll(head, ip, isamp, data.samples, grad, loss, 1.0 / (batch * J))
+        dW, db = mlp.backward(grad)
+        lr_t = lr * 0.5 * (1.0 + math.cos(math.pi * it / max(steps, 1)))
+        mlp.step(dW, db, lr_t)
+        if log_every and (it % log_every == 0 or it == steps - 1):
+            hist.append((it, float(loss.item()) / (batch * J)))
+    return hist
+
+
+def predict(mlp: MLP, rates: np.ndarray, x0: np.ndarray, sample_idx: np.ndarray, chunk: int = 1 << 16):
+    """Mixture parameters for queries (point rows, sample index 1..32): [Q][S][3][K]."""
+    import torch
+
+    rates_t = torch.as_tensor(np.ascontiguousarray(rates, dtype=np.float64), device="cuda")
+    x0_t = torch.as_tensor(np.ascontiguousarray(x0, dtype=np.int64), device="cuda")
+    Q = len(sample_idx)
+    out = np.zeros((Q, S, 3, K))
+    for a in range(0, Q, chunk):
+        b = min(Q, a + chunk)
+        B = ((b - a + 63) // 64) * 64
+        ip = torch.zeros(B, dtype=torch.int32, device="cuda")
+        isamp = torch.ones(B, dtype=torch.int32, device="cuda")
+        ip[: b - a] = torch.arange(a, b, dtype=torch.int32, device="cuda")
+        isamp[: b - a] = torch.as_tensor(sample_idx[a:b].astype(np.int32), device="cuda")
+        X = torch.empty((B, IN), dtype=torch.bfloat16, device="cuda")
+        features(ip, isamp, rates_t, x0_t, X)
+        head = mlp.forward(X, keep=False)
+        out[a:b] = head[: b - a, : S * 3 * K].double().cpu().numpy().reshape(b - a, S, 3, K)
+    return out
+
+
+def infer_device(mlp: MLP, d_rates, d_x0, d_point, d_sample, B: int):
+    """Device-only inference for timing: head [B][OUT] for B queries."""
+    import torch
+
+    X = torch.empty((B, IN), dtype=torch.bfloat16, device="cuda")
+    features(d_point, d_sample, d_rates, d_x0, X)
+    return mlp.forward(X, keep=False)
+
+
+# ---------------------------------------------------------------------------
+# distributions and the Wasserstein-1 gate
+
+
+def mixture_cdf(params: np.ndarray, xmax: int) -> np.ndarray:
+    """CDF on 0..xmax of one species' mixture (params [3][K])."""
+    from scipy import stats
+
+    logits, lmu, lr = params
+    w = np.exp(logits - logits.max())
+    w /= w.sum()
+    mu = np.exp(np.clip(lmu, -8, 9))
+    r = np.exp(np.clip(lr, -6, 12))
+    xs = np.arange(xmax + 1)
+    pmf = np.zeros(xmax + 1)
+    for c in range(K):
+        pmf += w[c] * stats.nbinom.pmf(xs, r[c], r[c] / (r[c] + mu[c]))
+    return np.minimum(np.cumsum(pmf), 1.0)
+
+
+def w1_to_samples(params: np.ndarray, samples: np.ndarray) -> float:
+    """W1 between the mixture and an empirical distribution of integer samples."""
+    xmax = int(max(samples.max(), 1) * 2 + 50)
+    F = mixture_cdf(params, xmax)
+    emp = np.searchsorted(np.sort(samples), np.arange(xmax + 1), side="right") / len(samples)
+    return float(np.abs(F - emp).sum())
+
+
+@dataclass
+class TrustMap:
+    """Cells of (log kf, log kb, log kcat, E0, S0) where the surrogate passed."""
+
+    bins: int
+    passed: np.ndarray  # bool [bins]^3 x 2 x 2
+
+    @staticmethod
+    def cell(rates: np.ndarray, x0: np.ndarray, bins: int) -> tuple:
+        lk = np.clip(((np.log10(rates) + 3.0) / 4.0 * bins).astype(int), 0, bins - 1)
+        e = (x0[:, 0] > 105).astype(int)
+        s = (x0[:, 1] > 525).astype(int)
+        return lk[:, 0], lk[:, 1], lk[:, 2], e, s
+
+    def trusted(self, rates: np.ndarray, x0: np.ndarray) -> np.ndarray:
+        return self.passed[self.cell(rates, x0, self.bins)]
+
+
+def validate(mlp: MLP, rates: np.ndarray, x0: np.ndarray, n_ref: int, seed: int, tol: float = 0.02,
+             bins: int = 2, times=(4, 8, 16, 32)):
+    """W1 of the surrogate against fresh n_ref-trajectory SSA histograms.
+
+    Returns (relative W1 [P][len(times)][S], TrustMap): a cell is trusted if
+    every validated (point, time, species) in it has W1 <= tol * mean.
+    """
+    data = generate(rates, x0, n_ref, seed=seed, traj_begin=1 << 40)
+    samp = data.samples.cpu().numpy()
+    P = len(rates)
+    rel = np.zeros((P, len(times), S))
+    for ti, i in enumerate(times):
+        params = predict(mlp, rates, x0, np.full(P, i))
+        for p in range(P):
+            for s in range(S):
+                xs = samp[i, s, p * n_ref:(p + 1) * n_ref]
+                w1 = w1_to_samples(params[p, s], xs)
+                mean = xs.mean()
+                rel[p, ti, s] = w1 / mean if mean > 0 else (0.0 if w1 <= tol else np.inf)
+    ok = (rel <= tol).all(axis=(1, 2))
+    passed = np.ones((bins,) * 3 + (2, 2), dtype=bool)
+    seen = np.zeros_like(passed)
+    cells = TrustMap.cell(rates, x0, bins)
+    for p in range(P):
+        c = tuple(ci[p] for ci in cells)
+        seen[c] = True
+        passed[c] &= ok[p]
+    passed &= seen
+    return rel, TrustMap(bins, passed), data.stats
+
+
+def benchmark(n_train: int = 100_000, steps: int = 1500, batch: int = 4096, n_queries: int = 1_000_000,
+              n_valid: int = 128, n_ref: int = 10_000, seed: int = 2026, log=print) -> dict:
+    """End-to-end: SSA training data -> training -> W1 validation -> query timing.
+
+    Returns a dict with the surrogate's and the SSA's queries/s (a query is
+    the distribution of every species at one (parameter point, sample time));
+    the SSA's cost per query is that of n_ref trajectories of the point,
+    shared by the point's 32 sample times.
+    """
+    import time
+
+    import torch
+
+    t0 = time.perf_counter()
+    rates, x0 = sample_points(n_train, seed)
+    data = generate(rates, x0, J, seed=seed)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    log(f"training data: {n_train} points x {J} trajectories, {data.stats.events} events, {gen_s:.2f} s")
+    mlp = MLP(seed=seed)
+    t0 = time.perf_counter()
+    hist = train(mlp, data, steps=steps, batch=batch, lr=2e-3, seed=seed, log_every=max(1, steps // 10))
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    log(f"training: {steps} steps of {batch}, {train_s:.2f} s, NLL/sample {hist[0][1]:.3f} -> {hist[-1][1]:.3f}")
+    del data
+    torch.cuda.empty_cache()
+    # validation against fresh n_ref-trajectory histograms
+    vr, vx = sample_points(n_valid, seed + 1)
+    t0 = time.perf_counter()
+    rel, trust, vstats = validate(mlp, vr, vx, n_ref=n_ref, seed=seed + 2)
+    val_s = time.perf_counter() - t0
+    ssa_ms = vstats.kernel_ms
+    ssa_queries = n_valid * NB
+    ssa_qps = ssa_queries / (ssa_ms / 1e3)
+    pass_frac = float((rel <= 0.02).mean())
+    point_pass = float((rel <= 0.02).all(axis=(1, 2)).mean())
+    log(f"validation: {n_valid} points x {n_ref} trajectories ({ssa_ms:.1f} ms kernel); "
+        f"W1 <= 2% of mean for {pass_frac:.3f} of (point, time, species), {point_pass:.3f} of points")
+    # query timing (device-resident inputs, like the SSA's)
+    qr, qx = sample_points(n_queries, seed + 3)
+    d_r = torch.as_tensor(qr, device="cuda")
+    d_x = torch.as_tensor(qx, device="cuda")
+    qs = np.random.default_rng(seed + 4).integers(1, NB + 1, n_queries).astype(np.int32)
+    chunk = 1 << 17
+    pts = [torch.arange(a, min(n_queries, a + chunk), dtype=torch.int32, device="cuda") for a in range(0, n_queries, chunk)]
+    sps = [torch.as_tensor(qs[a:a + chunk], device="cuda") for a in range(0, n_queries, chunk)]
+    for p_, s_ in zip(pts, sps):  # warm
+        infer_device(mlp, d_r, d_x, p_, s_, p_.numel())
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for p_, s_ in zip(pts, sps):
+        infer_device(mlp, d_r, d_x, p_, s_, p_.numel())
+    ev1.record()
+    torch.cuda.synchronize()
+    sg_ms = ev0.elapsed_time(ev1)
+    sg_qps = n_queries / (sg_ms / 1e3)
+    flops = 2.0 * n_queries * sum(mlp.dims[l] * mlp.dims[l + 1] for l in range(mlp.depth))
+    log(f"surrogate: {n_queries} queries in {sg_ms:.2f} ms ({sg_qps:.3e} q/s, {flops / sg_ms / 1e9:.1f} TFLOP/s); "
+        f"SSA: {ssa_qps:.3e} q/s; speedup {sg_qps / ssa_qps:.1f}x")
+    return dict(surrogate_qps=sg_qps, ssa_qps=ssa_qps, speedup=sg_qps / ssa_qps, surrogate_ms=sg_ms,
+                ssa_ms=ssa_ms, w1_pass_fraction=pass_frac, point_pass_fraction=point_pass,
+                trusted_cells=float(trust.passed.mean()), train_s=train_s, gen_s=gen_s, val_s=val_s,
+                nll_first=hist[0][1], nll_last=hist[-1][1], gemm_tflops=flops / sg_ms / 1e9,
+                train_events=int(vstats.events), rel_w1_median=float(np.median(rel)))

@@ -1,0 +1,165 @@
+"""The surrogate's device kernels against the FP64 CPU reference
+(oracle/surrogate_ref.py), a training run that must learn, and the W1 gate."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sg():
+    from rebop_amd import surrogate
+
+    surrogate.load()
+    return surrogate
+
+
+def _bf(x):
+    import torch
+
+    return torch.as_tensor(x, dtype=torch.float32, device="cuda").to(torch.bfloat16)
+
+
+@pytest.mark.parametrize("ta,tb", [(False, False), (False, True), (True, False), (True, True)])
+def test_gemm_bf16_matches_fp64(sg, ta, tb):
+    """op(A) op(B) on the matrix cores equals the FP64 product of the same
+    BF16 values to FP32-accumulation accuracy (the lane layout of
+    v_mfma_f32_16x16x32_bf16 is checked here)."""
+    import torch
+
+    from oracle.surrogate_ref import bf16
+
+    rng = np.random.default_rng(0)
+    M, N, Kd = 128, 192, 96
+    A = bf16(rng.normal(size=(M, Kd)))
+    B = bf16(rng.normal(size=(Kd, N)))
+    dA = _bf(A.T.copy() if ta else A)
+    dB = _bf(B.T.copy() if tb else B)
+    C = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
+    sg.gemm(dA, dB, C, ta=ta, tb=tb)
+    torch.cuda.synchronize()
+    ref = A @ B
+    got = C.double().cpu().numpy()
+    assert np.isfinite(got).all()
+    assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
+
+
+def test_nll_kernel_matches_fp64_reference(sg):
+    import torch
+
+    from oracle import surrogate_ref as ref
+
+    rng = np.random.default_rng(1)
+    B, S, K, J, P = 64, 4, 8, 64, 64
+    head = np.zeros((B, 128))
+    for s in range(S):
+        head[:, s * 3 * K:s * 3 * K + K] = rng.normal(size=(B, K))
+        head[:, s * 3 * K + K:s * 3 * K + 2 * K] = rng.normal(np.log(60.0), 1.0, size=(B, K))
+        head[:, s * 3 * K + 2 * K:(s + 1) * 3 * K] = rng.normal(1.0, 1.5, size=(B, K))
+    head = head.astype(np.float32).astype(np.float64)
+    nb = 32
+    samples = rng.poisson(60, size=(nb + 1, S, P * J)).astype(np.int64)
+    ip = rng.integers(0, P, B).astype(np.int32)
+    isamp = rng.integers(1, nb + 1, B).astype(np.int32)
+    d_head = torch.as_tensor(head, dtype=torch.float32, device="cuda")
+    grad = torch.zeros((B, 128), dtype=torch.float32, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    sg.nb_mixture_nll(d_head, torch.as_tensor(ip, device="cuda"), torch.as_tensor(isamp, device="cuda"),
+                      torch.as_tensor(samples, device="cuda"), grad, loss, 1.0)
+    torch.cuda.synchronize()
+    obs = np.stack([samples[isamp[b], :, ip[b] * J:(ip[b] + 1) * J] for b in range(B)])
+    total, g = ref.nll(head, obs, K)
+    assert loss.item() == pytest.approx(total, rel=1e-10)
+    got = grad.double().cpu().numpy()
+    assert np.allclose(got, g, rtol=1e-5, atol=1e-5 * np.abs(g).max())
+
+
+def test_training_step_gradients_match_reference(sg):
+    """One forward + NLL + backward of the device MLP against the FP64
+    reference with the same BF16 roundings (what remains is FP32 vs FP64
+    accumulation), and the FP32-vs-FP64 error of the head stated."""
+    import torch
+
+    from oracle import surrogate_ref as ref
+
+    rng = np.random.default_rng(2)
+    mlp = sg.MLP(seed=3, width=128, depth=3)
+    B, P, J, nb = 128, 32, 64, 32
+    rates, x0 = sg.sample_points(P, seed=4)
+    samples = rng.poisson(50, size=(nb + 1, 4, P * J)).astype(np.int64)
+    ip = rng.integers(0, P, B).astype(np.int32)
+    isamp = rng.integers(1, nb + 1, B).astype(np.int32)
+    d_ip, d_is = torch.as_tensor(ip, device="cuda"), torch.as_tensor(isamp, device="cuda")
+    X = torch.empty((B, sg.IN), dtype=torch.bfloat16, device="cuda")
+    sg.features(d_ip, d_is, torch.as_tensor(rates, device="cuda"), torch.as_tensor(x0, device="cuda"), X)
+    head = mlp.forward(X)
+    grad = torch.zeros((B, sg.OUT), dtype=torch.float32, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    scale = 1.0 / (B * J)
+    sg.nb_mixture_nll(head, d_ip, d_is, torch.as_tensor(samples, device="cuda"), grad, loss, scale)
+    dW, db = mlp.backward(grad)
+    torch.cuda.synchronize()
+
+    Xr = ref.features(rates[ip], x0[ip], isamp.astype(np.float64), sg.TMAX, sg.NB, sg.IN)
+    assert np.array_equal(ref.bf16(Xr), X.double().cpu().numpy())
+    Ws = [w.double().cpu().numpy() for w in mlp.W]
+    bs = [b.double().cpu().numpy() for b in mlp.b]
+    rhead, H = ref.forward(Xr, Ws, bs, emulate_bf16=True)
+    dhead = head.double().cpu().numpy()
+    err = np.abs(dhead - rhead).max() / np.abs(rhead).max()
+    assert err < 1e-5, err
+    # FP32 head vs an all-FP64 network without BF16 roundings: the stated error
+    rhead64, _ = ref.forward(Xr, Ws, bs, emulate_bf16=False)
+    err64 = np.abs(dhead - rhead64).max() / np.abs(rhead64).max()
+    print(f"head error vs all-FP64 network: {err64:.3e} (vs BF16-emulating FP64: {err:.3e})")
+    assert err64 < 5e-2
+    obs = np.stack([samples[isamp[b], :, ip[b] * J:(ip[b] + 1) * J] for b in range(B)])
+    total, g = ref.nll(dhead, obs, 8)
+    assert loss.item() == pytest.approx(total, rel=1e-9)
+    rdW, rdb = ref.backward(g * scale, H, Ws, emulate_bf16=True)
+    for l in range(mlp.depth):
+        a = dW[l].double().cpu().numpy()
+        assert np.abs(a - rdW[l]).max() <= 2e-3 * np.abs(rdW[l]).max() + 1e-12, l
+        b = db[l].double().cpu().numpy()
+        assert np.abs(b - rdb[l]).max() <= 2e-3 * np.abs(rdb[l]).max() + 1e-12, l
+
+
+def test_adam_matches_reference(sg):
+    import torch
+
+    rng = np.random.default_rng(5)
+    n = 1000
+    w = rng.normal(size=n).astype(np.float32)
+    g = rng.normal(size=n).astype(np.float32)
+    m = rng.normal(size=n).astype(np.float32) * 0.1
+    v = np.abs(rng.normal(size=n).astype(np.float32)) * 0.01
+    dw, dg, dm, dv = (torch.as_tensor(a, device="cuda") for a in (w, g, m, v))
+    wb = torch.empty(n, dtype=torch.bfloat16, device="cuda")
+    sg.adam(dw, dg, dm, dv, wb, 1e-3, 0.9, 0.999, 1e-8, 3)
+    torch.cuda.synchronize()
+    m2 = 0.9 * m + 0.1 * g
+    v2 = 0.999 * v + 0.001 * g * g
+    w2 = w - 1e-3 * (m2 / (1 - 0.9**3)) / (np.sqrt(v2 / (1 - 0.999**3)) + 1e-8)
+    assert np.allclose(dw.cpu().numpy(), w2, rtol=1e-5, atol=1e-7)
+    assert np.array_equal(wb.float().cpu().numpy(),
+                          torch.as_tensor(dw.cpu()).to(torch.bfloat16).float().numpy())
+
+
+def test_training_learns_and_beats_untrained(sg):
+    """A short training run lowers the NLL, and the trained network's W1 to
+    fresh SSA histograms is far below the untrained network's: what the
+    trained surrogate gains over the method without it."""
+    P = 2048
+    rates, x0 = sg.sample_points(P, seed=11)
+    data = sg.generate(rates, x0, sg.J, seed=12)
+    untrained = sg.MLP(seed=1)
+    mlp = sg.MLP(seed=1)
+    hist = sg.train(mlp, data, steps=300, batch=4096, lr=2e-3, log_every=50)
+    assert hist[-1][1] < 0.7 * hist[0][1], hist
+    vr, vx = sg.sample_points(12, seed=13)
+    rel_t, _, _ = sg.validate(mlp, vr, vx, n_ref=4000, seed=14, times=(8, 32))
+    rel_u, _, _ = sg.validate(untrained, vr, vx, n_ref=4000, seed=14, times=(8, 32))
+    assert np.median(rel_t) < 0.5 * np.median(rel_u), (np.median(rel_t), np.median(rel_u))

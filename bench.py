@@ -156,10 +156,13 @@ class Heterogeneous(Workload):
 
         from rebop_amd import workloads
 
+        log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+        t0 = time.perf_counter()
         self.specs = workloads.heterogeneous_specs(self.n_networks, self.seed)
         self.nets = workloads.build_nets(self.specs, seed=0)
+        log(f"het: {self.n_networks} networks built in {time.perf_counter() - t0:.1f} s")
         self.targets = workloads.target_events(self.n_networks)
-        self.tmax = workloads.calibrate_tmax(self.nets, self.specs, self.targets)
+        self.tmax = workloads.calibrate_tmax(self.nets, self.specs, self.targets, log=log)
         self.outs = [torch.empty((self.nb + 1, sp[0], self.n_traj), dtype=torch.int64, device="cuda")
                      for sp in self.specs]
         self.stream = torch.cuda.current_stream().cuda_stream
@@ -323,6 +326,8 @@ def main() -> int:
 
     for w in range(args.warmup):
         wl.step(1000 + w)
+        torch.cuda.synchronize()
+        print(f"warmup step {w} done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     events = 0
     kernel_ms = []
@@ -333,6 +338,8 @@ def main() -> int:
         events += st.events
         kernel_ms.append(st.kernel_ms)
         kinds.add(int(st.kernel))
+        if st.kernel_ms > 2000.0:
+            print(f"step {k}: {st.events} events, {st.kernel_ms:.0f} ms", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
 

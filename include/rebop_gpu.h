@@ -98,6 +98,38 @@ int rebop_gillespie_add_reaction_lma_sparse(rebop_gillespie *g, double rate,
                                             uint32_t n_pairs, const int64_t *differences,
                                             uint32_t n_differences);
 
+/* Rate expressions (Rate::Expr, src/expr.rs:9-38): the expression tree in
+ * postfix order, one 32-bit token per node: op | (arg << 8), with
+ * arg the constant index (CONST) or species index (SPECIES), 0 otherwise.
+ * Evaluation follows Expr::eval (src/expr.rs:24-38): species counts as f64,
+ * IEEE + - * / and negation, pow, max/min (a NaN operand yields the other
+ * operand), exp; pow and exp are this library's own (host and device agree
+ * bit for bit; the reference's libm may differ in the last ulp). */
+#define REBOP_EXPR_CONST 1u
+#define REBOP_EXPR_SPECIES 2u
+#define REBOP_EXPR_NEG 3u
+#define REBOP_EXPR_ADD 4u
+#define REBOP_EXPR_SUB 5u
+#define REBOP_EXPR_MUL 6u
+#define REBOP_EXPR_DIV 7u
+#define REBOP_EXPR_POW 8u
+#define REBOP_EXPR_MAX 9u
+#define REBOP_EXPR_MIN 10u
+#define REBOP_EXPR_EXP 11u
+
+/* add_reaction(Rate::expr(expr), differences) -- src/gillespie.rs:225-244 with
+ * Rate::expr at :33-35.  The propensity is the expression's value (no
+ * mass-action factor).  The program must be well formed (one value left, at
+ * most 32 operands deep, at most 4096 tokens; REBOP_ERR_VALUE) and refer to
+ * existing constants and species (REBOP_ERR_INDEX).  Networks with rate
+ * expressions run on the network-specialised kernels (S <= 32, R <= 64);
+ * other kernels, fused job batches and rate tables return
+ * REBOP_ERR_UNSUPPORTED.  An expression that evaluates to +inf makes the run
+ * return REBOP_ERR_RANGE (time would not advance). */
+int rebop_gillespie_add_reaction_expr(rebop_gillespie *g, const uint32_t *code, uint32_t n_code,
+                                      const double *consts, uint32_t n_consts,
+                                      const int64_t *differences, uint32_t n_differences);
+
 /* Compiles and caches the kernel `kernel` (REBOP_KERNEL_*) will use for this
  * network, so that the first run does not pay for it.  Without a device it
  * only compiles, which checks the specialised kernel for gfx950.  No

@@ -19,6 +19,7 @@
 #include <string.h>
 #include <unistd.h>
 
+#include "../rebop_amd/csrc/rebop_expr.h"
 #include "../rebop_amd/csrc/rebop_rng.h"
 
 /* ------------------------------------------------------------------ */
@@ -34,6 +35,12 @@ struct orc_net {
   uint32_t *rate_off; /* R+1 */
   uint32_t *rate_idx, *rate_ord;
   uint32_t rate_len, rate_cap;
+  /* Rate::Expr (rate_sparse[r] == 2): postfix tokens expr_off[r]..expr_off[r+1]
+   * of expr_code; their constant indices are rebased into expr_const. */
+  uint32_t *expr_off; /* R+1 */
+  uint32_t *expr_code;
+  double *expr_const;
+  uint32_t expr_len, expr_cap, const_len, const_cap;
   /* Jump: Flat holds S diffs at jump_off[r]; Sparse holds (idx, diff) pairs. */
   int *jump_sparse;
   uint32_t *jump_off; /* R+1 */
@@ -49,7 +56,8 @@ orc_net *orc_net_new(uint32_t nb_species, int sparse) {
   n->sparse = sparse ? 1 : 0;
   n->rate_off = (uint32_t *)calloc(1, sizeof(uint32_t));
   n->jump_off = (uint32_t *)calloc(1, sizeof(uint32_t));
-  if (!n->rate_off || !n->jump_off) {
+  n->expr_off = (uint32_t *)calloc(1, sizeof(uint32_t));
+  if (!n->rate_off || !n->jump_off || !n->expr_off) {
     orc_net_free(n);
     return NULL;
   }
@@ -67,6 +75,9 @@ void orc_net_free(orc_net *n) {
   free(n->jump_off);
   free(n->jump_idx);
   free(n->jump_d);
+  free(n->expr_off);
+  free(n->expr_code);
+  free(n->expr_const);
   free(n);
 }
 
@@ -106,6 +117,35 @@ static int push_jump_entry(orc_net *n, uint32_t idx, int64_t d) {
   return 0;
 }
 
+/* room for reaction r in every per-reaction array (the offsets need r + 2) */
+static int reserve_reaction(orc_net *n, uint32_t r) {
+  uint32_t c1 = n->cap, c2 = n->cap, c3 = n->cap, c4 = n->cap, c5 = n->cap, c6 = n->cap;
+  if (grow((void **)&n->rate_sparse, sizeof(int), &c1, r + 2) ||
+      grow((void **)&n->k, sizeof(double), &c2, r + 2) ||
+      grow((void **)&n->jump_sparse, sizeof(int), &c3, r + 2) ||
+      grow((void **)&n->rate_off, sizeof(uint32_t), &c4, r + 2) ||
+      grow((void **)&n->jump_off, sizeof(uint32_t), &c5, r + 2) ||
+      grow((void **)&n->expr_off, sizeof(uint32_t), &c6, r + 2))
+    return -1;
+  n->cap = c1;
+  return 0;
+}
+
+static int push_jump(orc_net *n, uint32_t r, const int64_t *diffs, uint32_t ndiffs) {
+  if (!n->sparse) {
+    n->jump_sparse[r] = 0;
+    for (uint32_t s = 0; s < ndiffs; ++s)
+      if (push_jump_entry(n, s, diffs[s])) return ORC_ERR_ALLOC;
+  } else {
+    /* Jump::sparse: (index, difference) for difference != 0 */
+    n->jump_sparse[r] = 1;
+    for (uint32_t s = 0; s < ndiffs; ++s)
+      if (diffs[s] != 0 && push_jump_entry(n, s, diffs[s])) return ORC_ERR_ALLOC;
+  }
+  n->jump_off[r + 1] = n->jump_len;
+  return ORC_OK;
+}
+
 /* Gillespie::add_reaction (gillespie.rs:225-244) with Rate::sparse (:55-69)
  * and Jump::sparse (:126-140) applied when the net is sparse. */
 int orc_net_add_reaction(orc_net *n, int rate_sparse, double k, const uint32_t *idx,
@@ -124,14 +164,7 @@ int orc_net_add_reaction(orc_net *n, int rate_sparse, double k, const uint32_t *
   /* every per-reaction array is sized for R + 2 entries (the offsets need R + 1) */
   uint32_t r = n->R;
   if (r + 2 > n->cap) {
-    uint32_t c1 = n->cap, c2 = n->cap, c3 = n->cap, c4 = n->cap, c5 = n->cap;
-    if (grow((void **)&n->rate_sparse, sizeof(int), &c1, r + 2) ||
-        grow((void **)&n->k, sizeof(double), &c2, r + 2) ||
-        grow((void **)&n->jump_sparse, sizeof(int), &c3, r + 2) ||
-        grow((void **)&n->rate_off, sizeof(uint32_t), &c4, r + 2) ||
-        grow((void **)&n->jump_off, sizeof(uint32_t), &c5, r + 2))
-      return ORC_ERR_ALLOC;
-    n->cap = c1;
+    if (reserve_reaction(n, r)) return ORC_ERR_ALLOC;
   }
 
   n->k[r] = k;
@@ -152,20 +185,96 @@ int orc_net_add_reaction(orc_net *n, int rate_sparse, double k, const uint32_t *
       if (push_rate_entry(n, idx[i], orders[i])) return ORC_ERR_ALLOC;
   }
   n->rate_off[r + 1] = n->rate_len;
-
-  if (!n->sparse) {
-    n->jump_sparse[r] = 0;
-    for (uint32_t s = 0; s < ndiffs; ++s)
-      if (push_jump_entry(n, s, diffs[s])) return ORC_ERR_ALLOC;
-  } else {
-    /* Jump::sparse: (index, difference) for difference != 0 */
-    n->jump_sparse[r] = 1;
-    for (uint32_t s = 0; s < ndiffs; ++s)
-      if (diffs[s] != 0 && push_jump_entry(n, s, diffs[s])) return ORC_ERR_ALLOC;
-  }
-  n->jump_off[r + 1] = n->jump_len;
+  n->expr_off[r + 1] = n->expr_len;
+  int rc = push_jump(n, r, diffs, ndiffs);
+  if (rc) return rc;
   n->R = r + 1;
   return ORC_OK;
+}
+
+/* Gillespie::add_reaction(Rate::expr(e), diffs) (gillespie.rs:225-244, :33-35):
+ * Rate::sparse/dense leave an expression as it is (:51, :67); only the jump
+ * follows the net's sparse flag.  The postfix program is checked for stack
+ * discipline and in-range constant/species indices. */
+int orc_net_add_reaction_expr(orc_net *n, const uint32_t *code, uint32_t n_code,
+                              const double *consts, uint32_t n_consts, const int64_t *diffs,
+                              uint32_t ndiffs) {
+  if (ndiffs != n->S) return ORC_ERR_SHAPE;
+  if (n_code == 0 || n_code > REBOP_EXPR_MAX_CODE) return ORC_ERR_VALUE;
+  uint32_t depth = 0;
+  for (uint32_t i = 0; i < n_code; ++i) {
+    const uint32_t op = REBOP_EXPR_OPCODE(code[i]), arg = REBOP_EXPR_ARG(code[i]);
+    if (op == REBOP_OP_CONST) {
+      if (arg >= n_consts) return ORC_ERR_INDEX;
+      depth++;
+    } else if (op == REBOP_OP_SPECIES) {
+      if (arg >= n->S) return ORC_ERR_INDEX;
+      depth++;
+    } else if (op == REBOP_OP_NEG || op == REBOP_OP_EXP) {
+      if (depth < 1) return ORC_ERR_VALUE;
+    } else if (op >= REBOP_OP_ADD && op <= REBOP_OP_MIN) {
+      if (depth < 2) return ORC_ERR_VALUE;
+      depth--;
+    } else {
+      return ORC_ERR_VALUE;
+    }
+    if (depth > REBOP_EXPR_MAX_DEPTH) return ORC_ERR_VALUE;
+  }
+  if (depth != 1) return ORC_ERR_VALUE;
+  const uint32_t r = n->R;
+  if (r + 2 > n->cap && reserve_reaction(n, r)) return ORC_ERR_ALLOC;
+  uint32_t cc = n->expr_cap, kc = n->const_cap;
+  if (grow((void **)&n->expr_code, sizeof(uint32_t), &cc, n->expr_len + n_code) ||
+      grow((void **)&n->expr_const, sizeof(double), &kc, n->const_len + n_consts + 1))
+    return ORC_ERR_ALLOC;
+  n->expr_cap = cc;
+  n->const_cap = kc;
+  for (uint32_t i = 0; i < n_code; ++i) {
+    uint32_t tok = code[i];
+    if (REBOP_EXPR_OPCODE(tok) == REBOP_OP_CONST)
+      tok = REBOP_EXPR_TOKEN(REBOP_OP_CONST, REBOP_EXPR_ARG(tok) + n->const_len);
+    n->expr_code[n->expr_len++] = tok;
+  }
+  for (uint32_t i = 0; i < n_consts; ++i) n->expr_const[n->const_len++] = consts[i];
+  n->rate_sparse[r] = 2;
+  n->k[r] = 0.0;
+  n->rate_off[r + 1] = n->rate_len;
+  n->expr_off[r + 1] = n->expr_len;
+  int rc = push_jump(n, r, diffs, ndiffs);
+  if (rc) return rc;
+  n->R = r + 1;
+  return ORC_OK;
+}
+
+/* Expr::eval (expr.rs:24-38) over the postfix program: counts enter as f64. */
+static double eval_expr(const orc_net *net, uint32_t r, const int64_t *x) {
+  double st[REBOP_EXPR_MAX_DEPTH];
+  uint32_t sp = 0;
+  for (uint32_t i = net->expr_off[r]; i < net->expr_off[r + 1]; ++i) {
+    const uint32_t tok = net->expr_code[i];
+    const uint32_t op = REBOP_EXPR_OPCODE(tok), arg = REBOP_EXPR_ARG(tok);
+    switch (op) {
+    case REBOP_OP_CONST: st[sp++] = net->expr_const[arg]; break;
+    case REBOP_OP_SPECIES: st[sp++] = (double)x[arg]; break;
+    case REBOP_OP_NEG: st[sp - 1] = -st[sp - 1]; break;
+    case REBOP_OP_EXP: st[sp - 1] = rebop_exp64(st[sp - 1]); break;
+    default: {
+      const double b = st[--sp], a = st[sp - 1];
+      double v;
+      switch (op) {
+      case REBOP_OP_ADD: v = a + b; break;
+      case REBOP_OP_SUB: v = a - b; break;
+      case REBOP_OP_MUL: v = a * b; break;
+      case REBOP_OP_DIV: v = a / b; break;
+      case REBOP_OP_POW: v = rebop_pow64(a, b); break;
+      case REBOP_OP_MAX: v = rebop_fmax64(a, b); break;
+      default: v = rebop_fmin64(a, b); break;
+      }
+      st[sp - 1] = v;
+    }
+    }
+  }
+  return st[0];
 }
 
 /* ------------------------------------------------------------------ */
@@ -174,6 +283,7 @@ int orc_net_add_reaction(orc_net *n, int rate_sparse, double k, const uint32_t *
  * over the stored pairs, in stored order.  Both are the same loop here since
  * the dense form stores every species in ascending order. */
 double orc_rate(const orc_net *net, uint32_t r, const int64_t *x) {
+  if (net->rate_sparse[r] == 2) return eval_expr(net, r, x);
   double acc = net->k[r];
   for (uint32_t i = net->rate_off[r]; i < net->rate_off[r + 1]; ++i) {
     const int64_t n = x[net->rate_idx[i]];
@@ -447,6 +557,10 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 }
 
 double orc_ln64(double x) { return rebop_ln64(x); }
+double orc_exp64(double x) { return rebop_exp64(x); }
+double orc_pow64(double x, double y) { return rebop_pow64(x, y); }
+double orc_fmax64(double a, double b) { return rebop_fmax64(a, b); }
+double orc_fmin64(double a, double b) { return rebop_fmin64(a, b); }
 double orc_u01_open_closed(uint32_t lo, uint32_t hi) { return rebop_u01_open_closed(lo, hi); }
 double orc_u01_closed_open(uint32_t lo, uint32_t hi) { return rebop_u01_closed_open(lo, hi); }
 

@@ -11,6 +11,8 @@
  *   Rate::sparse / dense  src/gillespie.rs:37-69    orc_net_add_reaction() packing
  *   Jump::affect          src/gillespie.rs:142-152  orc_affect()
  *   Gillespie::add_reaction asserts  :225-244       orc_net_add_reaction()
+ *   Rate::expr / Expr::eval src/gillespie.rs:33-35, src/expr.rs:24-38
+ *                                                   orc_net_add_reaction_expr()
  *   make_cumrates         src/gillespie.rs:357-364  orc_make_cumrates()
  *   choose_cumrate_sum    src/gillespie.rs:402-407  orc_choose_cumrate_sum()
  *   advance_until         src/gillespie.rs:315-344  orc_advance_until()
@@ -53,6 +55,16 @@ uint32_t orc_net_nb_reactions(const orc_net *net);
 int orc_net_add_reaction(orc_net *net, int rate_sparse, double k, const uint32_t *idx,
                          const uint32_t *orders, uint32_t n, const int64_t *diffs,
                          uint32_t ndiffs);
+
+/*
+ * add_reaction(Rate::expr(e), diffs): e as postfix tokens (rebop_expr.h:
+ * op | arg << 8) over n_consts constants.  Its value at the counts (as f64)
+ * is the propensity.  ORC_ERR_VALUE for a malformed program, ORC_ERR_INDEX for
+ * an out-of-range constant or species index.
+ */
+int orc_net_add_reaction_expr(orc_net *net, const uint32_t *code, uint32_t n_code,
+                              const double *consts, uint32_t n_consts, const int64_t *diffs,
+                              uint32_t ndiffs);
 
 /* Rate::rate of reaction r at counts x (for the known-answer tables). */
 double orc_rate(const orc_net *net, uint32_t r, const int64_t *x);
@@ -107,6 +119,11 @@ int orc_run_all_events(const orc_net *net, uint64_t seed, uint64_t traj, const i
 /* RNG primitives, exported so tests can pin them. */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 double orc_ln64(double x);
+/* the rate-expression math of rebop_expr.h */
+double orc_exp64(double x);
+double orc_pow64(double x, double y);
+double orc_fmax64(double a, double b);
+double orc_fmin64(double a, double b);
 double orc_u01_open_closed(uint32_t lo, uint32_t hi);
 double orc_u01_closed_open(uint32_t lo, uint32_t hi);
 double orc_grid_time(double tmax, uint64_t i, uint64_t nb_steps);

@@ -47,6 +47,10 @@ def load() -> ctypes.CDLL:
             c_int,
             [c_void_p, c_int, c_double, P(c_uint32), P(c_uint32), c_uint32, P(c_int64), c_uint32],
         ),
+        "orc_net_add_reaction_expr": (
+            c_int,
+            [c_void_p, P(c_uint32), c_uint32, P(c_double), c_uint32, P(c_int64), c_uint32],
+        ),
         "orc_rate": (c_double, [c_void_p, c_uint32, P(c_int64)]),
         "orc_make_cumrates": (c_double, [c_void_p, P(c_int64), P(c_double)]),
         "orc_choose_cumrate_sum": (c_uint64, [c_double, P(c_double), c_uint64]),
@@ -63,6 +67,10 @@ def load() -> ctypes.CDLL:
         ),
         "orc_philox4x32_10": (None, [P(c_uint32), P(c_uint32), P(c_uint32)]),
         "orc_ln64": (c_double, [c_double]),
+        "orc_exp64": (c_double, [c_double]),
+        "orc_pow64": (c_double, [c_double, c_double]),
+        "orc_fmax64": (c_double, [c_double, c_double]),
+        "orc_fmin64": (c_double, [c_double, c_double]),
         "orc_u01_open_closed": (c_double, [c_uint32, c_uint32]),
         "orc_u01_closed_open": (c_double, [c_uint32, c_uint32]),
         "orc_grid_time": (c_double, [c_double, c_uint64, c_uint64]),
@@ -131,6 +139,16 @@ class OracleNet:
         o = np.ascontiguousarray([p[1] for p in pairs], dtype=np.uint32)
         d = np.ascontiguousarray(diffs, dtype=np.int64)
         rc = self.lib.orc_net_add_reaction(self.h, 1, float(k), _p(i, c_uint32), _p(o, c_uint32), len(o), _p(d, c_int64), len(d))
+        if rc:
+            raise OracleError(rc)
+
+    def add_expr(self, code, consts, diffs) -> None:
+        """Rate::expr reaction: postfix tokens (op | arg << 8) and constants."""
+        c = np.ascontiguousarray(code, dtype=np.uint32)
+        k = np.ascontiguousarray(consts, dtype=np.float64)
+        d = np.ascontiguousarray(diffs, dtype=np.int64)
+        rc = self.lib.orc_net_add_reaction_expr(self.h, _p(c, c_uint32), len(c), _p(k, c_double), len(k),
+                                                _p(d, c_int64), len(d))
         if rc:
             raise OracleError(rc)
 
@@ -223,6 +241,22 @@ def philox(ctr, key):
 
 def ln64(x: float) -> float:
     return float(load().orc_ln64(float(x)))
+
+
+def exp64(x: float) -> float:
+    return float(load().orc_exp64(float(x)))
+
+
+def pow64(x: float, y: float) -> float:
+    return float(load().orc_pow64(float(x), float(y)))
+
+
+def fmax64(a: float, b: float) -> float:
+    return float(load().orc_fmax64(float(a), float(b)))
+
+
+def fmin64(a: float, b: float) -> float:
+    return float(load().orc_fmin64(float(a), float(b)))
 
 
 def choose_cumrate_sum(chosen: float, cum) -> int:

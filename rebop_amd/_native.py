@@ -29,6 +29,19 @@ REBOP_KERNEL_PERSISTENT = 2
 REBOP_KERNEL_LDS = 3
 REBOP_KERNEL_SPECIALIZED = 16
 
+# rate-expression bytecode operations (token = op | arg << 8)
+REBOP_EXPR_CONST = 1
+REBOP_EXPR_SPECIES = 2
+REBOP_EXPR_NEG = 3
+REBOP_EXPR_ADD = 4
+REBOP_EXPR_SUB = 5
+REBOP_EXPR_MUL = 6
+REBOP_EXPR_DIV = 7
+REBOP_EXPR_POW = 8
+REBOP_EXPR_MAX = 9
+REBOP_EXPR_MIN = 10
+REBOP_EXPR_EXP = 11
+
 # every entry point of include/rebop_gpu.h: name -> (restype, argtypes)
 SIGNATURES: dict[str, tuple] = {
     "rebop_gillespie_new": (c_int, [POINTER(c_int64), c_uint32, c_int, POINTER(c_void_p)]),
@@ -47,6 +60,10 @@ SIGNATURES: dict[str, tuple] = {
     "rebop_gillespie_add_reaction_lma_sparse": (
         c_int,
         [c_void_p, c_double, POINTER(c_uint32), POINTER(c_uint32), c_uint32, POINTER(c_int64), c_uint32],
+    ),
+    "rebop_gillespie_add_reaction_expr": (
+        c_int,
+        [c_void_p, POINTER(c_uint32), c_uint32, POINTER(c_double), c_uint32, POINTER(c_int64), c_uint32],
     ),
     "rebop_gillespie_prepare": (c_int, [c_void_p, c_int]),
     "rebop_gillespie_set_species": (c_int, [c_void_p, POINTER(c_int64), c_uint32]),

@@ -22,7 +22,7 @@ ARCH = "gfx950"
 
 SOURCES = ["ssa_kernels.hip", "ssa_lds.hip", "rebop_gpu.cpp", "jit.cpp"]
 # headers whose text is embedded in the library for run-time compilation
-EMBEDDED_HEADERS = ["rebop_rng.h", "ssa_layout.h", "ssa_lane.h"]
+EMBEDDED_HEADERS = ["rebop_rng.h", "rebop_expr.h", "ssa_layout.h", "ssa_lane.h"]
 DELIM = "REBOPSRC"
 
 # -ffp-contract=off: the SSA's FP64 arithmetic must round exactly as the

@@ -1,0 +1,45 @@
+// jit.h -- the network-specialised kernels (jit.cpp): the per-reaction
+// description the source generator takes, the loaded kernel handles, and the
+// compile / occupancy / launch entry points used by the C ABI.
+#ifndef REBOP_JIT_H
+#define REBOP_JIT_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "ssa_layout.h"
+
+namespace rebop {
+
+struct JitReaction {
+  double k;
+  std::vector<std::pair<uint32_t, uint32_t>> factors; // (species, c): factor = x[s] - c
+  std::vector<int64_t> diff;                          // dense [S]
+  // Rate::Expr: postfix tokens (rebop_expr.h) and their constants; empty for LMA
+  std::vector<uint32_t> expr;
+  std::vector<double> consts;
+};
+
+struct JitKernel {
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+  hipFunction_t fn_sweep = nullptr;
+};
+
+// Kernel source for a network of S species and the given reactions.
+std::string jit_source(uint32_t S, const std::vector<JitReaction> &rx);
+// Compiles (or finds in the cache) and loads the kernel on the current device.
+int jit_get(const std::string &src, JitKernel *out, std::string *err);
+// Compiles for gfx950 without loading (no device needed).
+int jit_compile_only(const std::string &src, size_t *code_size, std::string *err);
+hipError_t jit_occupancy(const JitKernel &k, int threads, int *blocks_per_cu);
+hipError_t jit_launch(const JitKernel &k, const rebop_dev_run &run, int blocks, int threads,
+                      hipStream_t stream);
+
+} // namespace rebop
+
+#endif // REBOP_JIT_H

@@ -18,6 +18,8 @@
 #include <utility>
 #include <vector>
 
+#include "jit.h"
+#include "rebop_expr.h"
 #include "ssa_layout.h"
 
 namespace rebop {
@@ -35,22 +37,6 @@ hipError_t launch_lds(const rebop_dev_job *jobs, const rebop_dev_item *items, ui
                       unsigned long long *counters, unsigned long long *job_events, int blocks,
                       int threads, size_t lds, hipStream_t stream);
 
-struct JitReaction {
-  double k;
-  std::vector<std::pair<uint32_t, uint32_t>> factors;
-  std::vector<int64_t> diff;
-};
-struct JitKernel {
-  hipModule_t mod = nullptr;
-  hipFunction_t fn = nullptr;
-  hipFunction_t fn_sweep = nullptr;
-};
-std::string jit_source(uint32_t S, const std::vector<JitReaction> &rx);
-int jit_get(const std::string &src, JitKernel *out, std::string *err);
-int jit_compile_only(const std::string &src, size_t *code_size, std::string *err);
-hipError_t jit_occupancy(const JitKernel &k, int threads, int *blocks_per_cu);
-hipError_t jit_launch(const JitKernel &k, const rebop_dev_run &run, int blocks, int threads,
-                      hipStream_t stream);
 } // namespace rebop
 
 namespace {
@@ -117,6 +103,8 @@ struct Reaction {
   double k;
   std::vector<std::pair<uint32_t, uint32_t>> entries; // (species, order) in evaluation order
   std::vector<int64_t> diff;                          // dense [S]
+  std::vector<uint32_t> expr;                         // Rate::Expr postfix tokens (else empty)
+  std::vector<double> consts;                         // the expression's constants
 };
 
 // device buffer owned by RAII
@@ -149,6 +137,7 @@ struct rebop_gillespie {
   uint64_t seed = 0;
   std::vector<int64_t> x0;
   std::vector<Reaction> rx;
+  uint32_t n_expr = 0; // reactions with a rate expression (specialised kernels only)
   // packed device copy of the network, rebuilt when reactions change
   bool dirty = true;
   int dev_id = -1;
@@ -177,6 +166,14 @@ struct rebop_batch {
   uint64_t traj_begin = 0, n = 0;
   DevBuf x, t, draw;
 };
+
+static_assert(REBOP_EXPR_CONST == REBOP_OP_CONST && REBOP_EXPR_SPECIES == REBOP_OP_SPECIES &&
+                  REBOP_EXPR_NEG == REBOP_OP_NEG && REBOP_EXPR_ADD == REBOP_OP_ADD &&
+                  REBOP_EXPR_SUB == REBOP_OP_SUB && REBOP_EXPR_MUL == REBOP_OP_MUL &&
+                  REBOP_EXPR_DIV == REBOP_OP_DIV && REBOP_EXPR_POW == REBOP_OP_POW &&
+                  REBOP_EXPR_MAX == REBOP_OP_MAX && REBOP_EXPR_MIN == REBOP_OP_MIN &&
+                  REBOP_EXPR_EXP == REBOP_OP_EXP,
+              "public expression op codes match the kernels'");
 
 namespace {
 
@@ -305,8 +302,10 @@ int pick_reg_shape(uint32_t S, uint32_t R) {
 bool jit_eligible(const rebop_gillespie *g) {
   if (g->S > 32 || g->rx.size() > 64) return false;
   size_t F = 0;
-  for (auto &r : g->rx)
+  for (auto &r : g->rx) {
     for (auto &e : r.entries) F += e.second;
+    F += r.expr.size();
+  }
   return F <= 4096;
 }
 
@@ -324,6 +323,8 @@ std::string jit_source_of(const rebop_gillespie *g) {
     for (auto &e : r.entries)
       for (uint32_t q = 0; q < e.second; ++q) j.factors.emplace_back(e.first, e.second - 1u - q);
     j.diff = r.diff;
+    j.expr = r.expr;
+    j.consts = r.consts;
     jr.push_back(std::move(j));
   }
   std::string src = rebop::jit_source(g->S, jr);
@@ -422,7 +423,8 @@ int finish_stats(rebop_stats *stats, unsigned long long *counters, hipEvent_t ev
   stats->kernel = (uint64_t)kind;
   stats->kernel_ms = ms;
   if (ctr[REBOP_CTR_FLAGS] & REBOP_FLAG_RANGE)
-    return fail(REBOP_ERR_RANGE, "a count left the exactly simulated range");
+    return fail(REBOP_ERR_RANGE, "a count left the exactly simulated range (|x| <= 2^52), or a "
+                                 "total propensity was infinite");
   return REBOP_OK;
 }
 
@@ -447,11 +449,21 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
   // when the network is small enough; else the data-driven register kernel
   // when it fits, else the LDS kernel
   const int variant = kernel & 0xF;
+  if (g->n_expr) {
+    // rate expressions are compiled into the specialised kernel only
+    if (variant == REBOP_KERNEL_LDS)
+      return fail(REBOP_ERR_UNSUPPORTED,
+                  "networks with rate expressions run on the specialised kernels only");
+    kernel |= REBOP_KERNEL_SPECIALIZED;
+  }
   const bool fits_regs = pick_reg_shape(S, R) >= 0;
   bool spec = false, use_lds = variant == REBOP_KERNEL_LDS;
   if (!use_lds && (kernel & REBOP_KERNEL_SPECIALIZED)) {
     if (!jit_eligible(g))
-      return fail(REBOP_ERR_UNSUPPORTED, "network too large for a specialised kernel");
+      return fail(REBOP_ERR_UNSUPPORTED,
+                  g->n_expr ? "network with rate expressions too large for a specialised kernel "
+                              "(S <= 32, R <= 64)"
+                            : "network too large for a specialised kernel");
     rc = prepare_jit(g);
     if (rc) return rc;
     spec = true;
@@ -663,6 +675,76 @@ int rebop_gillespie_add_reaction_lma(rebop_gillespie *g, double rate, const uint
                              n_differences);
 }
 
+int rebop_gillespie_add_reaction_expr(rebop_gillespie *g, const uint32_t *code, uint32_t n_code,
+                                      const double *consts, uint32_t n_consts,
+                                      const int64_t *differences, uint32_t n_differences) {
+  if (int rc = check_handle(g, "gillespie")) return rc;
+  if (n_code && !code) return fail(REBOP_ERR_VALUE, "code is NULL");
+  if (n_consts && !consts) return fail(REBOP_ERR_VALUE, "consts is NULL");
+  if (n_differences && !differences) return fail(REBOP_ERR_VALUE, "differences is NULL");
+  if (n_code == 0) return fail(REBOP_ERR_VALUE, "empty rate expression");
+  if (n_code > REBOP_EXPR_MAX_CODE)
+    return fail(REBOP_ERR_VALUE, "rate expression longer than 4096 tokens");
+  // the postfix program must leave exactly one value, never underflow the
+  // stack nor exceed its depth, and refer to existing constants and species
+  uint32_t depth = 0;
+  for (uint32_t i = 0; i < n_code; ++i) {
+    const uint32_t op = REBOP_EXPR_OPCODE(code[i]), arg = REBOP_EXPR_ARG(code[i]);
+    const std::string at = " at token " + std::to_string(i);
+    switch (op) {
+    case REBOP_OP_CONST:
+      if (arg >= n_consts) return fail(REBOP_ERR_INDEX, "constant index out of range" + at);
+      depth += 1;
+      break;
+    case REBOP_OP_SPECIES:
+      if (arg >= g->S)
+        return fail(REBOP_ERR_INDEX, "assertion failed: species index " + std::to_string(arg) +
+                                         " out of range for " + std::to_string(g->S) +
+                                         " species" + at);
+      depth += 1;
+      break;
+    case REBOP_OP_NEG:
+    case REBOP_OP_EXP:
+      if (depth < 1) return fail(REBOP_ERR_VALUE, "rate expression stack underflow" + at);
+      break;
+    case REBOP_OP_ADD:
+    case REBOP_OP_SUB:
+    case REBOP_OP_MUL:
+    case REBOP_OP_DIV:
+    case REBOP_OP_POW:
+    case REBOP_OP_MAX:
+    case REBOP_OP_MIN:
+      if (depth < 2) return fail(REBOP_ERR_VALUE, "rate expression stack underflow" + at);
+      depth -= 1;
+      break;
+    default:
+      return fail(REBOP_ERR_VALUE, "unknown rate expression operation " + std::to_string(op) + at);
+    }
+    if (depth > REBOP_EXPR_MAX_DEPTH)
+      return fail(REBOP_ERR_VALUE, "rate expression deeper than 32 operands" + at);
+  }
+  if (depth != 1)
+    return fail(REBOP_ERR_VALUE, "rate expression leaves " + std::to_string(depth) + " values");
+  if (n_differences != g->S)
+    return fail(REBOP_ERR_SHAPE, "assertion `left == right` failed: " +
+                                     std::to_string(n_differences) + " differences for " +
+                                     std::to_string(g->S) + " species");
+  for (uint32_t s = 0; s < n_differences; ++s)
+    if (differences[s] > REBOP_DIFF_LIMIT || differences[s] < -REBOP_DIFF_LIMIT)
+      return fail(REBOP_ERR_VALUE, "stoichiometric difference " + std::to_string(differences[s]) +
+                                       " is beyond |d| <= 2^20");
+  Reaction r;
+  r.k = 0.0;
+  r.expr.assign(code, code + n_code);
+  r.consts.assign(consts, consts + n_consts);
+  r.diff.assign(differences, differences + n_differences);
+  g->rx.push_back(std::move(r));
+  g->n_expr += 1;
+  g->dirty = true;
+  g->jit_dirty = true;
+  return REBOP_OK;
+}
+
 int rebop_gillespie_add_reaction_lma_sparse(rebop_gillespie *g, double rate,
                                             const uint32_t *species_idx, const uint32_t *orders,
                                             uint32_t n_pairs, const int64_t *differences,
@@ -817,6 +899,9 @@ int rebop_run_sweep_device(const rebop_gillespie *gc, uint64_t n_points, uint64_
   rebop_gillespie *g = const_cast<rebop_gillespie *>(gc);
   if (nb_steps == 0) return fail(REBOP_ERR_VALUE, "nb_steps must be > 0 for a grid run");
   if (traj_per_point == 0) return fail(REBOP_ERR_VALUE, "traj_per_point must be > 0");
+  if (d_rates && g->n_expr)
+    return fail(REBOP_ERR_UNSUPPORTED,
+                "rate tables apply to mass-action rates; this network has rate expressions");
   if (n_points && traj_per_point > UINT64_MAX / n_points)
     return fail(REBOP_ERR_VALUE, "n_points * traj_per_point overflows");
   if (!d_x0) {
@@ -889,6 +974,10 @@ int rebop_run_jobs_device(const rebop_job *jobs, uint32_t n_jobs, void *stream_,
     if (int e = check_handle(J.net, "job net")) return e;
     if (J.nb_steps == 0) return fail(REBOP_ERR_VALUE, "job " + std::to_string(j) + ": nb_steps must be > 0");
     rebop_gillespie *g = const_cast<rebop_gillespie *>(J.net);
+    if (g->n_expr)
+      return fail(REBOP_ERR_UNSUPPORTED, "job " + std::to_string(j) +
+                                             ": networks with rate expressions run on the "
+                                             "specialised kernels, not in fused batches");
     if (!J.d_x0) {
       if (int e = check_x0(g->x0.data(), g->x0.size())) return e;
     }

@@ -170,6 +170,12 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
     if (need_props) {
       tot = net.props(x, cum);
       need_props = false;
+      // an infinite total would stop time (E / tot == 0) for good: flag the
+      // run and let the trajectory run out as an absorbing state
+      if (tot == __builtin_inf()) {
+        acc.flags |= REBOP_FLAG_RANGE;
+        tot = 0.0;
+      }
     }
 
     bool rec = true; // absorbing state: sample without drawing
@@ -257,6 +263,9 @@ __device__ __forceinline__ LaneTotals lane_events(Net &net, const rebop_dev_run 
     while (t < run.tmax) {
       const double tot = net.props(x, cum);
       if (!(0.0 < tot)) {
+        t = __builtin_inf();
+      } else if (tot == __builtin_inf()) {
+        acc.flags |= REBOP_FLAG_RANGE; // time would stop (E / tot == 0)
         t = __builtin_inf();
       } else {
         const rebop_u32x4 blk = rebop_draw_block(run.seed, run.traj_begin + jl, draw);

@@ -21,7 +21,9 @@
  * that range).  The host rejects |x0| > 2^52 and |diff| > 2^20; the kernel
  * checks |x| <= 2^52 at every sample and that fewer than 2^31 draws happened
  * since the previous check (so no count can have passed 2^53 in between),
- * and raises REBOP_FLAG_RANGE otherwise.
+ * and raises REBOP_FLAG_RANGE otherwise.  An infinite total propensity (an
+ * overflowing product or rate expression) raises it too: time would not
+ * advance, so that trajectory is run out as an absorbing state instead.
  */
 #ifndef REBOP_SSA_LAYOUT_H
 #define REBOP_SSA_LAYOUT_H

@@ -68,6 +68,27 @@ class Rate:
 
 
 @dataclass(frozen=True)
+class RateExpr:
+    """``Rate::Expr`` (src/gillespie.rs:18, :33-35): a rate expression in
+    postfix bytecode (tokens ``op | arg << 8``, REBOP_EXPR_*) over constants.
+    ``rebop_amd.expr.to_postfix`` builds one from a parsed string."""
+
+    code: tuple
+    consts: tuple
+
+    @staticmethod
+    def from_postfix(code: Sequence[int], consts: Sequence[float]) -> "RateExpr":
+        return RateExpr(tuple(int(c) for c in code), tuple(float(k) for k in consts))
+
+    # Rate::dense / Rate::sparse leave an expression unchanged (gillespie.rs:51, :67)
+    def dense(self) -> "RateExpr":
+        return self
+
+    def sparse_form(self) -> "RateExpr":
+        return self
+
+
+@dataclass(frozen=True)
 class Jump:
     """Net stoichiometry of a reaction: ``Jump::Flat`` or ``Jump::Sparse`` (src/gillespie.rs:93-100)."""
 
@@ -146,10 +167,18 @@ class GillespieNet:
     def nb_reactions(self) -> int:
         return int(self._lib.rebop_gillespie_nb_reactions(self._h))
 
-    def add_reaction(self, rate: Rate, differences: Sequence[int]) -> None:
+    def add_reaction(self, rate: "Rate | RateExpr", differences: Sequence[int]) -> None:
         """``Gillespie::add_reaction`` -- src/gillespie.rs:225-244."""
         d = np.ascontiguousarray(np.asarray(differences, dtype=np.int64).reshape(-1))
-        if rate.sparse:
+        if isinstance(rate, RateExpr):
+            code = np.ascontiguousarray(rate.code, dtype=np.uint32)
+            consts = np.ascontiguousarray(rate.consts, dtype=np.float64)
+            check(
+                self._lib.rebop_gillespie_add_reaction_expr(
+                    self._h, _u32(code), len(code), _f64(consts), len(consts), _i64(d), len(d)
+                )
+            )
+        elif rate.sparse:
             idx = np.ascontiguousarray([p[0] for p in rate.orders], dtype=np.uint32)
             orde = np.ascontiguousarray([p[1] for p in rate.orders], dtype=np.uint32)
             check(

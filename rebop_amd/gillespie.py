@@ -3,8 +3,10 @@
 Follows ``python/rebop/gillespie.py`` (class ``Gillespie``, :22-160) and the
 PyO3 class it wraps (``src/pyo3_gillespie.rs:11-253``): species are named,
 indexed in order of first appearance (reactants, then products, then species
-only seen in ``init``), rates are Law-of-Mass-Action constants, and ``run``
-samples ``nb_steps + 1`` equally spaced times.  ``run`` additionally accepts
+only seen in ``init``), a numeric rate is a Law-of-Mass-Action constant and a
+string rate is a rate expression (``Rate::Expr``, parsed as the reference's
+``PExpr``, names resolved to species counts or ``params`` at ``run``), and
+``run`` samples ``nb_steps + 1`` equally spaced times.  ``run`` additionally accepts
 ``n_trajectories`` and then simulates that many independent trajectories in
 one launch; trajectory j draws from the Philox stream (seed, j), where the
 seed is derived from ``rng`` exactly as the reference wrapper does
@@ -18,7 +20,8 @@ from typing import Mapping, Sequence
 
 import numpy as np
 
-from .engine import GillespieNet, Rate
+from . import expr as _expr
+from .engine import GillespieNet, Rate, RateExpr
 from .result import make_dataset
 
 _INIT_WARNING = (
@@ -32,8 +35,9 @@ class Gillespie:
     def __init__(self) -> None:
         self._species: dict[str, int] = {}
         self._init: dict[str, int] = {}
-        # (rate, reactants, products), as pyo3_gillespie.rs:15
-        self._reactions: list[tuple[float, list[str], list[str]]] = []
+        # (rate, reactants, products), as pyo3_gillespie.rs:15; rate is a float
+        # (PRate::Lma) or a parsed expression node (PRate::Expr)
+        self._reactions: list[tuple[float | _expr.Node, list[str], list[str]]] = []
 
     # -- model building (pyo3_gillespie.rs:76-117) ------------------------
     def add_species(self, species: str) -> None:
@@ -56,23 +60,19 @@ class Gillespie:
     ) -> None:
         """Add a reaction (pyo3_gillespie.rs:86-113).
 
-        ``rate`` is the Law-of-Mass-Action constant.  A string rate (an
-        arbitrary rate expression, ``Rate::Expr``) is outside the device path
-        and raises ``NotImplementedError``.
+        A number is the Law-of-Mass-Action constant (``PRate::Lma``); a
+        string is parsed as a rate expression (``PRate::Expr``), whose value
+        is the propensity.  An unparsable string raises
+        ``ValueError("Rate expression not understood")``.
         """
-        if isinstance(rate, str):
-            raise NotImplementedError(
-                "arbitrary rate expressions (Rate::Expr) are not supported by the device engine"
-            )
-        if isinstance(rate, bool) or not isinstance(rate, (int, float, np.integer, np.floating)):
-            raise TypeError(f"rate must be a number, not {type(rate).__name__}")
+        rate_ = _prate(rate)
         reactants = [str(r) for r in reactants]
         products = [str(p) for p in products]
         for r in reactants:
             self.add_species(r)
         for p in products:
             self.add_species(p)
-        self._reactions.append((float(rate), reactants, products))
+        self._reactions.append((rate_, reactants, products))
         if reverse_rate is not None:
             self.add_reaction(reverse_rate, products, reactants, None)
 
@@ -97,11 +97,12 @@ class Gillespie:
         """Textual form (pyo3_gillespie.rs:239-252)."""
         s = f"{len(self._species)} species and {len(self._reactions)} reactions\n"
         for rate, reactants, products in self._reactions:
-            s += " + ".join(reactants) + " --> " + " + ".join(products) + f" @ LMA({_rust_f64(rate)})\n"
+            shown = str(rate) if isinstance(rate, _expr.Node) else f"LMA({_expr.rust_f64(rate)})"
+            s += " + ".join(reactants) + " --> " + " + ".join(products) + f" @ {shown}\n"
         return s
 
     # -- network assembly (pyo3_gillespie.rs:161-196) -----------------------
-    def _network(self, seed: int, sparse: bool) -> GillespieNet:
+    def _network(self, seed: int, sparse: bool, params: Mapping[str, float]) -> GillespieNet:
         S = len(self._species)
         x0 = np.zeros(S, dtype=np.int64)
         for name, value in self._init.items():
@@ -116,7 +117,12 @@ class Gillespie:
                 actions[self._species[r]] -= 1
             for p in products:
                 actions[self._species[p]] += 1
-            net.add_reaction(Rate.lma(rate, orders), actions)
+            if isinstance(rate, _expr.Node):
+                # PRate::to_gillespie_rate -> PExpr::to_expr (pyo3_gillespie.rs:46-60)
+                code, consts = _expr.to_postfix(rate, self._species, params)
+                net.add_reaction(RateExpr.from_postfix(code, consts), actions)
+            else:
+                net.add_reaction(Rate.lma(rate, orders), actions)
         return net
 
     def run(
@@ -156,7 +162,7 @@ class Gillespie:
             save = [(v, self._species[v]) for v in var_names]
         else:
             save = list(self._species.items())
-        net = self._network(seed, bool(sparse) if sparse is not None else False)
+        net = self._network(seed, bool(sparse) if sparse is not None else False, params)
         nb = int(nb_steps)
         if nb < 0:
             raise OverflowError("can't convert negative int to unsigned")
@@ -175,17 +181,14 @@ class Gillespie:
         return make_dataset(times, data, batched=True)
 
 
-def _rust_f64(x: float) -> str:
-    """Format a float as Rust's ``Display`` for f64 does (shortest round-trip, no exponent)."""
-    if x != x:
-        return "NaN"
-    if x in (float("inf"), float("-inf")):
-        return "inf" if x > 0 else "-inf"
-    if x == int(x) and abs(x) < 1e16:
-        return str(int(x)) if x != 0 or str(x)[0] != "-" else "-0"
-    r = repr(x)
-    if "e" in r or "E" in r:
-        from decimal import Decimal
-
-        r = format(Decimal(r), "f")
-    return r
+def _prate(rate):
+    """``PRate::new`` (pyo3_gillespie.rs:39-44): a number is an LMA constant, a
+    string a rate expression."""
+    if isinstance(rate, str):
+        try:
+            return _expr.parse(rate)
+        except _expr.RateParseError:
+            raise ValueError("Rate expression not understood") from None
+    if isinstance(rate, (bool, int, float, np.integer, np.floating)):
+        return float(rate)
+    raise TypeError(f"rate must be a number or a string, not {type(rate).__name__}")

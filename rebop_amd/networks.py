@@ -37,20 +37,33 @@ def michaelis_menten(kf=0.01, kb=0.1, kcat=0.1, e0=50, s0=200):
     ], [e0, s0, 0, 0]
 
 
-def synthetic(n_species: int, n_reactions: int, seed: int, init_mean: float = 100.0):
+def synthetic(n_species: int, n_reactions: int, seed: int, init_mean: float = 100.0,
+              bounded: bool = False):
     """Random mass-action network: reactant order 0/1/2 with p = .1/.6/.3, reactants
     uniform over species (with replacement), 0-2 products uniform over species,
-    rates log-uniform in [1e-3, 1e1]; init counts Poisson(init_mean)."""
+    rates log-uniform in [1e-3, 1e1]; init counts Poisson(init_mean).
+
+    With ``bounded``, every species gets a mass in {1, 2, 3} and the products
+    of a reaction with reactants are redrawn until they weigh no more than
+    the reactants: the total mass then grows only through the order-0
+    (inflow) reactions, linearly in time, so no trajectory blows up
+    exponentially and the event count of a time span stays moderate.
+    """
     rng = np.random.default_rng(seed)
+    mass = rng.integers(1, 4, size=n_species) if bounded else None
     rx = []
     for _ in range(n_reactions):
         order = int(rng.choice(3, p=[0.1, 0.6, 0.3]))
         orders = np.zeros(n_species, dtype=np.int64)
         for s in rng.integers(0, n_species, size=order):
             orders[s] += 1
-        diffs = -orders.copy()
-        for s in rng.integers(0, n_species, size=int(rng.integers(0, 3))):
-            diffs[s] += 1
+        while True:
+            diffs = -orders.copy()
+            prods = rng.integers(0, n_species, size=int(rng.integers(0, 3)))
+            for s in prods:
+                diffs[s] += 1
+            if mass is None or order == 0 or mass[prods].sum() <= (mass * orders).sum():
+                break
         k = float(10.0 ** rng.uniform(-3, 1))
         rx.append((k, orders.tolist(), diffs.tolist()))
     x0 = rng.poisson(init_mean, size=n_species).astype(np.int64).tolist()

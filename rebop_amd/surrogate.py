@@ -273,7 +273,7 @@ class MLP:
 
 
 def train(mlp: MLP, data: SweepData, steps: int, batch: int = 4096, lr: float = 1e-3,
-          seed: int = 0, log_every: int = 0):
+          seed: int = 0, log_every: int = 0, log=None):
     """Adam on the mean mixture NLL of the SSA samples; returns the loss history."""
     import torch
 
@@ -298,6 +298,8 @@ def train(mlp: MLP, data: SweepData, steps: int, batch: int = 4096, lr: float = 
         mlp.step(dW, db, lr_t)
         if log_every and (it % log_every == 0 or it == steps - 1):
             hist.append((it, float(loss.item()) / (batch * J)))
+            if log:
+                log(f"train step {it}: NLL/sample {hist[-1][1]:.4f}")
     return hist
 
 
@@ -430,7 +432,8 @@ def benchmark(n_train: int = 100_000, steps: int = 1500, batch: int = 4096, n_qu
     log(f"training data: {n_train} points x {J} trajectories, {data.stats.events} events, {gen_s:.2f} s")
     mlp = MLP(seed=seed)
     t0 = time.perf_counter()
-    hist = train(mlp, data, steps=steps, batch=batch, lr=2e-3, seed=seed, log_every=max(1, steps // 10))
+    hist = train(mlp, data, steps=steps, batch=batch, lr=2e-3, seed=seed, log_every=max(1, steps // 10),
+                 log=log)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     log(f"training: {steps} steps of {batch}, {train_s:.2f} s, NLL/sample {hist[0][1]:.3f} -> {hist[-1][1]:.3f}")

@@ -162,12 +162,6 @@ def test_parameter_species_clash_message():
             s.run({}, 10, 10, params={"B": 4.2})
 
 
-def test_expression_rates_are_outside_the_device_path():
-    s = Gillespie()
-    with pytest.raises(NotImplementedError):
-        s.add_reaction("k * A", ["A"], [])
-
-
 def test_seed_derivation_matches_reference_wrapper():
     """gillespie.py:139-140: seed = default_rng(rng).integers(2**64 - 1, dtype=uint64)."""
     assert TABLES["test_fixed_seed"]["numpy_seed"] == str(

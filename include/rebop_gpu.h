@@ -59,7 +59,12 @@ enum rebop_kernel_kind {
   REBOP_KERNEL_LDS = 3,        /* counts in LDS (int32), any network up to 256 species */
   /* flag: the network compiled into the kernel (run-time compilation,
    * cached); combine with PLAIN or PERSISTENT */
-  REBOP_KERNEL_SPECIALIZED = 16
+  REBOP_KERNEL_SPECIALIZED = 16,
+  /* flags: 2 or 4 lanes per trajectory, sharing the Philox blocks and ln of
+   * the next draws (PLAIN | SPECIALIZED grid runs; AUTO picks them when the
+   * run has too few trajectories for more than one wave per SIMD) */
+  REBOP_KERNEL_GROUP2 = 32,
+  REBOP_KERNEL_GROUP4 = 64
 };
 
 /* Opaque network + initial state: rebop's `gillespie::Gillespie` (src/gillespie.rs:157-163). */

@@ -28,6 +28,8 @@ REBOP_KERNEL_PLAIN = 1
 REBOP_KERNEL_PERSISTENT = 2
 REBOP_KERNEL_LDS = 3
 REBOP_KERNEL_SPECIALIZED = 16
+REBOP_KERNEL_GROUP2 = 32
+REBOP_KERNEL_GROUP4 = 64
 
 # rate-expression bytecode operations (token = op | arg << 8)
 REBOP_EXPR_CONST = 1

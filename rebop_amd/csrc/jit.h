@@ -28,6 +28,8 @@ struct JitKernel {
   hipModule_t mod = nullptr;
   hipFunction_t fn = nullptr;
   hipFunction_t fn_sweep = nullptr;
+  hipFunction_t fn_g2 = nullptr; // 2 lanes per trajectory (plain launches)
+  hipFunction_t fn_g4 = nullptr; // 4 lanes per trajectory
 };
 
 // Kernel source for a network of S species and the given reactions.
@@ -37,8 +39,10 @@ int jit_get(const std::string &src, JitKernel *out, std::string *err);
 // Compiles for gfx950 without loading (no device needed).
 int jit_compile_only(const std::string &src, size_t *code_size, std::string *err);
 hipError_t jit_occupancy(const JitKernel &k, int threads, int *blocks_per_cu);
+// group = lanes per trajectory (1, 2 or 4; 2 and 4 only for plain, non-sweep
+// grid/target runs)
 hipError_t jit_launch(const JitKernel &k, const rebop_dev_run &run, int blocks, int threads,
-                      hipStream_t stream);
+                      hipStream_t stream, int group = 1);
 
 } // namespace rebop
 

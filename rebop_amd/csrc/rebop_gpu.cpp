@@ -438,6 +438,12 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
     kernel = g->tune_kernel;
     threads = g->tune_threads;
     occ = g->tune_occ;
+    // a grouped choice is kept only while the run still leaves at most one
+    // wave per SIMD (the pilot may have been smaller than this run)
+    const int grp = (kernel & REBOP_KERNEL_GROUP4) ? 4 : (kernel & REBOP_KERNEL_GROUP2) ? 2 : 1;
+    if (grp > 1 && (run.n_traj * (uint64_t)grp > (uint64_t)g_dev.cus * 256u || run.rates ||
+                    run.mode == REBOP_MODE_EVENTS))
+      kernel = REBOP_KERNEL_PERSISTENT | REBOP_KERNEL_SPECIALIZED;
   }
   if (threads == 0) threads = kThreads;
   if (threads != 64 && threads != 128 && threads != 256)
@@ -515,7 +521,24 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
     }
     return REBOP_OK;
   }
-  const bool persistent = variant != REBOP_KERNEL_PLAIN;
+  // lanes per trajectory: asked for, or (AUTO) when even 4 (2) lanes per
+  // trajectory leave at most one wave per SIMD
+  int group = (kernel & REBOP_KERNEL_GROUP4) ? 4 : (kernel & REBOP_KERNEL_GROUP2) ? 2 : 1;
+  if (group > 1) {
+    if (!spec)
+      return fail(REBOP_ERR_UNSUPPORTED,
+                  "lanes per trajectory > 1 needs the specialised kernel (REBOP_KERNEL_SPECIALIZED)");
+    if (variant == REBOP_KERNEL_PERSISTENT || run.mode == REBOP_MODE_EVENTS || run.rates)
+      return fail(REBOP_ERR_UNSUPPORTED,
+                  "lanes per trajectory > 1 applies to plain grid runs without rate tables");
+  } else if (variant == REBOP_KERNEL_AUTO && spec && run.mode != REBOP_MODE_EVENTS && !run.rates) {
+    const uint64_t lanes = (uint64_t)g_dev.cus * 4u * 64u;
+    if (run.n_traj * 4u <= lanes)
+      group = 4;
+    else if (run.n_traj * 2u <= lanes)
+      group = 2;
+  }
+  const bool persistent = group == 1 && variant != REBOP_KERNEL_PLAIN;
   const int shape = spec ? -1 : pick_reg_shape(S, R);
   if (!spec && shape < 0)
     return fail(REBOP_ERR_UNSUPPORTED, "network with " + std::to_string(S) + " species and " +
@@ -524,7 +547,7 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
   run.persistent = persistent ? 1 : 0;
   run.counters = (unsigned long long *)g->counters.p;
   const size_t lds = spec ? 0 : sizeof(double) * (size_t)R * S;
-  const uint64_t need_blocks = (run.n_traj + threads - 1) / threads;
+  const uint64_t need_blocks = (run.n_traj * (uint64_t)group + threads - 1) / threads;
   uint64_t blocks = need_blocks;
   if (persistent) {
     int per_cu = 0;
@@ -540,7 +563,8 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
   if (blocks == 0) blocks = 1;
   if (blocks > 0x7fffffffull) return fail(REBOP_ERR_VALUE, "too many trajectories for one launch");
   const int kind = (persistent ? REBOP_KERNEL_PERSISTENT : REBOP_KERNEL_PLAIN) |
-                   (spec ? REBOP_KERNEL_SPECIALIZED : 0);
+                   (spec ? REBOP_KERNEL_SPECIALIZED : 0) |
+                   (group == 4 ? REBOP_KERNEL_GROUP4 : group == 2 ? REBOP_KERNEL_GROUP2 : 0);
 
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (stats) {
@@ -551,7 +575,7 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
   if (stats) HIP_TRY(hipEventRecord(ev0, stream));
   if (run.n_traj > 0) {
     if (spec)
-      HIP_TRY(rebop::jit_launch(g->jit, run, (int)blocks, threads, stream));
+      HIP_TRY(rebop::jit_launch(g->jit, run, (int)blocks, threads, stream, group));
     else
       HIP_TRY(rebop::launch_reg(shape, g->view, run, (int)blocks, threads, lds, stream));
   }
@@ -840,12 +864,16 @@ int rebop_autotune(rebop_gillespie *g, double tmax, uint64_t nb_steps, uint64_t 
   const bool jit_ok = jit_eligible(g) && !jit_disabled() && prepare_jit(g) == REBOP_OK;
   const bool reg_ok = pick_reg_shape(S, R) >= 0;
   for (int threads : {64, 128, 256}) {
-    for (double occ : {1.0, 0.5}) {
+    for (double occ : {1.0, 0.6, 0.4, 0.25}) {
       if (jit_ok)
         cands.push_back({REBOP_KERNEL_PERSISTENT | REBOP_KERNEL_SPECIALIZED, threads, occ});
       if (reg_ok) cands.push_back({REBOP_KERNEL_PERSISTENT, threads, occ});
     }
-    if (jit_ok) cands.push_back({REBOP_KERNEL_PLAIN | REBOP_KERNEL_SPECIALIZED, threads, 1.0});
+    if (jit_ok) {
+      cands.push_back({REBOP_KERNEL_PLAIN | REBOP_KERNEL_SPECIALIZED, threads, 1.0});
+      cands.push_back({REBOP_KERNEL_PLAIN | REBOP_KERNEL_SPECIALIZED | REBOP_KERNEL_GROUP2, threads, 1.0});
+      cands.push_back({REBOP_KERNEL_PLAIN | REBOP_KERNEL_SPECIALIZED | REBOP_KERNEL_GROUP4, threads, 1.0});
+    }
     if (reg_ok) cands.push_back({REBOP_KERNEL_PLAIN, threads, 1.0});
   }
   cands.push_back({REBOP_KERNEL_LDS, 64, 1.0});

@@ -222,6 +222,140 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
   return acc;
 }
 
+// Lane q of each aligned group of K lanes (K = 2 or 4) reads `v` from lane Q
+// of its group: one DPP quad permutation per 32-bit half, no LDS traffic.
+template <int K, int Q>
+__device__ __forceinline__ double group_read(double v) {
+  static_assert(K == 2 || K == 4, "groups of 2 or 4 lanes");
+  constexpr int ctrl = K == 4 ? (Q | (Q << 2) | (Q << 4) | (Q << 6))
+                              : (Q | (Q << 2) | ((2 + Q) << 4) | ((2 + Q) << 6));
+  const uint64_t b = rebop_f64_bits(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)b, ctrl, 0xF, 0xF, false);
+  const uint32_t hi =
+      (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), ctrl, 0xF, 0xF, false);
+  return rebop_f64_from_bits(((uint64_t)hi << 32) | lo);
+}
+
+template <int K>
+__device__ __forceinline__ double group_read(double v, int q) {
+  switch (q) {
+  case 0: return group_read<K, 0>(v);
+  case 1: return group_read<K, 1>(v);
+  case 2: if constexpr (K == 4) return group_read<K, 2>(v); else return v;
+  default: if constexpr (K == 4) return group_read<K, 3>(v); else return v;
+  }
+}
+
+// K lanes per trajectory, for launches with too few trajectories to give
+// every SIMD more than one wave (there the time is one trajectory's serial
+// event chain, and the Philox block + ln of each draw is most of an event).
+// Lane `sub` of a group computes the block of draw (draw + sub) of the group's
+// trajectory; the group then runs up to K iterations of the state machine,
+// iteration q using block q (read from lane q by DPP).  That is exactly the
+// draw each iteration would take: every event or overshoot consumes the next
+// draw, and a trajectory in an absorbing state never draws again (its counts
+// no longer change), so the blocks of a round are used in order or not at
+// all.  All lanes of a group hold the same state; lane 0 of the group writes
+// and counts.  Plain launches only: trajectory j is group j.
+template <class Net, int K>
+__device__ __forceinline__ LaneTotals lane_loop_group(Net &net, const rebop_dev_run &run,
+                                                      uint64_t gtid) {
+  LaneTotals acc;
+  const uint64_t n = run.n_traj;
+  const uint64_t jl = gtid / K;
+  if (jl >= n) return acc;
+  const uint32_t sub = __lane_id() % K;
+  const bool lead = sub == 0;
+  const uint32_t nb = run.mode == REBOP_MODE_TARGET ? 0u : (uint32_t)run.nb_steps;
+  const uint64_t Sn = (uint64_t)net.S() * n;
+
+  double x[Net::MS];
+  double cum[Net::MR];
+#pragma unroll
+  for (int s = 0; s < Net::MS; ++s) x[s] = 0.0;
+#pragma unroll
+  for (int r = 0; r < Net::MR; ++r) cum[r] = 0.0;
+  net.start(run, jl);
+  double t;
+  uint64_t draw;
+  if (run.st_x) {
+    net.load_state(x, run.st_x + jl, n);
+    t = run.st_t[jl];
+    draw = run.st_draw[jl];
+  } else {
+    net.load_x0(x, run.x0_div ? run.x0 + (jl / run.x0_div) * net.S() : run.x0);
+    t = 0.0;
+    draw = 0;
+  }
+  if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
+  const uint64_t draw0 = draw;
+  uint64_t mark = draw;
+  uint32_t i = 0, ov = 0;
+  double T = grid_time(run, 0), tot = 0.0;
+  int64_t *orec = run.out && lead ? run.out + jl : nullptr;
+  bool need_props = true, done = false;
+
+  while (!done) {
+    const rebop_u32x4 blk = rebop_draw_block(run.seed, run.traj_begin + jl, draw + sub);
+    const double e_own = rebop_exp1_of(blk), u_own = rebop_unif_of(blk);
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const double e = group_read<K>(e_own, q);
+      const double u = group_read<K>(u_own, q);
+      if (!done) {
+        if (need_props) {
+          tot = net.props(x, cum);
+          need_props = false;
+          if (tot == __builtin_inf()) {
+            acc.flags |= REBOP_FLAG_RANGE;
+            tot = 0.0;
+          }
+        }
+        bool rec = true;
+        double tn = 0.0;
+        if (0.0 < tot) {
+          draw += 1;
+          tn = t + e / tot;
+          rec = tn > T;
+          ov += rec ? 1u : 0u;
+        }
+        if (rec) {
+          if (orec) {
+            net.write_counts(x, orec, n);
+            orec += Sn;
+          }
+          if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
+          if (draw - mark >= REBOP_DRAWS_PER_CHECK) acc.flags |= REBOP_FLAG_RANGE;
+          mark = draw;
+          t = T;
+          i += 1;
+          if (i > nb) {
+            if (lead && run.final_x) net.write_counts(x, run.final_x + jl, n);
+            if (lead && run.st_x) {
+              net.store_state(x, run.st_x + jl, n);
+              run.st_t[jl] = t;
+              run.st_draw[jl] = draw;
+            }
+            if (lead) {
+              acc.events += (draw - draw0) - ov;
+              acc.draws += draw - draw0;
+              acc.trajectories += 1;
+            }
+            done = true;
+          } else {
+            T = grid_time(run, i);
+          }
+        } else {
+          t = tn;
+          net.affect(x, net.choose(x, cum, tot * u));
+          need_props = true;
+        }
+      }
+    }
+  }
+  return acc;
+}
+
 // The nb_steps == 0 loop of Gillespie.run (src/pyo3_gillespie.rs:209-223):
 // record (t, counts) before the first event and after every
 // _advance_one_reaction (src/gillespie.rs:275-297) while t < tmax.  That step

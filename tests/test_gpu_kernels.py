@@ -22,6 +22,8 @@ VARIANTS = [
     K.REBOP_KERNEL_PERSISTENT,
     K.REBOP_KERNEL_PLAIN | K.REBOP_KERNEL_SPECIALIZED,
     K.REBOP_KERNEL_PERSISTENT | K.REBOP_KERNEL_SPECIALIZED,
+    K.REBOP_KERNEL_PLAIN | K.REBOP_KERNEL_SPECIALIZED | K.REBOP_KERNEL_GROUP2,
+    K.REBOP_KERNEL_PLAIN | K.REBOP_KERNEL_SPECIALIZED | K.REBOP_KERNEL_GROUP4,
     K.REBOP_KERNEL_LDS,
 ]
 # the data-driven register kernels hold at most 16 species and 32 reactions
@@ -254,3 +256,29 @@ def test_dimers_full_size_invariants():
     # the mean dimer count at t=1 sits in the range the reference's test brackets
     d = out[-1, 3, :].double().mean().item()
     assert 0 < d < 10000
+
+
+def test_auto_groups_lanes_only_for_small_runs():
+    """AUTO gives 4 (2) lanes per trajectory while that leaves at most one wave
+    per SIMD, else one lane per trajectory with refill."""
+    _, cus = K.device_info()
+    lanes = cus * 4 * 64
+    g = build_device(sir(), 5)
+    for n, want in [(1000, K.REBOP_KERNEL_GROUP4), (lanes // 4, K.REBOP_KERNEL_GROUP4),
+                    (lanes // 4 + 64, K.REBOP_KERNEL_GROUP2), (lanes // 2 + 64, 0)]:
+        out, st = _device_run(g, 10.0, 4, n, K.REBOP_KERNEL_AUTO)
+        assert st.kernel & (K.REBOP_KERNEL_GROUP2 | K.REBOP_KERNEL_GROUP4) == want, (n, st.kernel)
+        if want == 0:
+            assert st.kernel == K.REBOP_KERNEL_PERSISTENT | K.REBOP_KERNEL_SPECIALIZED
+
+
+def test_grouped_lanes_refuse_persistent_and_generic():
+    import torch
+
+    g = build_device(sir(), 5)
+    out = torch.zeros((5, 3, 64), dtype=torch.int64, device="cuda")
+    for kern in (K.REBOP_KERNEL_PERSISTENT | K.REBOP_KERNEL_SPECIALIZED | K.REBOP_KERNEL_GROUP4,
+                 K.REBOP_KERNEL_PLAIN | K.REBOP_KERNEL_GROUP2):
+        with pytest.raises(K.RebopError) as e:
+            g.run_device(10.0, 4, 64, out.data_ptr(), kernel=kern)
+        assert e.value.code == K.REBOP_ERR_UNSUPPORTED

@@ -68,19 +68,29 @@ def host_threads() -> int:
     return max(1, min(16, n))
 
 
+CPU_MIN_WALL_S = 1.0  # per random-stream variant: >= 16 s of CPU work on 16 threads
+
+
 def _time_oracle(jobs, threads: int, rng: int):
+    """Runs the job list (repeated with further seeds until it has taken
+    CPU_MIN_WALL_S of wall time); returns (events, seconds, repetitions)."""
     from oracle import pyoracle
 
-    events = 0
-    t0 = time.perf_counter()
+    nets = []
     for spec, seed, tmax, nb, n, tb in jobs:
         net = pyoracle.OracleNet(spec[0], sparse=True)
         for k, o, d in spec[1]:
             net.add_lma(k, o, d)
-        _, _, _, ev, _ = net.run_grid(seed, spec[2], tmax, nb, n_traj=n, traj_begin=tb,
-                                      nthreads=threads, want_out=False, rng=rng)
-        events += ev
-    return events, time.perf_counter() - t0
+        nets.append(net)
+    events, reps = 0, 0
+    t0 = time.perf_counter()
+    while reps == 0 or time.perf_counter() - t0 < CPU_MIN_WALL_S:
+        for net, (spec, seed, tmax, nb, n, tb) in zip(nets, jobs):
+            _, _, _, ev, _ = net.run_grid(seed + 1000 * reps, spec[2], tmax, nb, n_traj=n,
+                                          traj_begin=tb, nthreads=threads, want_out=False, rng=rng)
+            events += ev
+        reps += 1
+    return events, time.perf_counter() - t0, reps
 
 
 def cpu_baseline(jobs, note: str) -> dict:
@@ -91,8 +101,8 @@ def cpu_baseline(jobs, note: str) -> dict:
     from oracle import pyoracle
 
     threads = host_threads()
-    ev_p, dt_p = _time_oracle(jobs, threads, pyoracle.PHILOX)
-    ev_x, dt_x = _time_oracle(jobs, threads, pyoracle.XOSHIRO)
+    ev_p, dt_p, rep_p = _time_oracle(jobs, threads, pyoracle.PHILOX)
+    ev_x, dt_x, rep_x = _time_oracle(jobs, threads, pyoracle.XOSHIRO)
     rate_p, rate_x = ev_p / dt_p, ev_x / dt_x
     return {
         "value": max(rate_p, rate_x),
@@ -101,11 +111,12 @@ def cpu_baseline(jobs, note: str) -> dict:
         "kind": "port",
         "philox_ln64_events_per_s": rate_p,
         "xoshiro_libm_events_per_s": rate_x,
-        "sample": f"{note}; C restatement of advance_until over {threads} threads, sparse rate "
+        "sample": f"{note}, repeated with new seeds to >= {CPU_MIN_WALL_S:.0f} s wall per "
+                  f"stream; C restatement of advance_until over {threads} threads, sparse rate "
                   f"form (bit-identical to dense, faster), timed with the device's Philox + ln64 "
-                  f"stream ({ev_p} events, {dt_p:.2f} s) and with xoshiro256++ + libm log "
-                  f"({ev_x} events, {dt_x:.2f} s); value is the faster; the Rust reference "
-                  "cannot be built here",
+                  f"stream ({rep_p} x, {ev_p} events, {dt_p:.2f} s) and with xoshiro256++ + libm "
+                  f"log ({rep_x} x, {ev_x} events, {dt_x:.2f} s); value is the faster; the Rust "
+                  "reference cannot be built here",
     }
 
 
@@ -313,6 +324,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--kernel", type=int, default=0, help="REBOP_KERNEL_* (0 = auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="before timing, time every kernel variant and launch shape at the "
+                         "workload's full size (rebop_autotune); AUTO steps then use the fastest")
     ap.add_argument("--workload", default="dimer", choices=["dimer", "sir", "syn", "het", "surrogate"],
                     help="dimer (the headline configuration), sir, syn (128x512), het (4096 "
                          "networks), surrogate (Michaelis-Menten sweep -> MLP -> W1 gate)")
@@ -348,6 +362,13 @@ def main() -> int:
     wl = make_workload(args.workload, args.kernel)
     dev_name, cus = _native.device_info()
     wl.setup()
+    tuned = None
+    if args.autotune and isinstance(wl, SingleNetwork) and args.kernel == 0:
+        wl.g.seed(999)
+        res = wl.g.autotune(wl.tmax, wl.nb, n_pilot=wl.n)
+        tuned = {"kernel": int(res.kernel), "threads": int(res.threads),
+                 "occupancy": float(res.occupancy), "pilot_ms": float(res.ms), "tried": int(res.n_tried)}
+        print(f"autotune: {tuned}", file=sys.stderr, flush=True)
 
     for w in range(args.warmup):
         wl.step(1000 + w)
@@ -398,6 +419,7 @@ def main() -> int:
             "trajectories_per_s": trajs / wall,
             "output_gb_per_s": out_bytes * args.steps / wall / 1e9,
             "kernel": sorted(kinds),
+            "autotuned": tuned,
             "device": dev_name,
             "compute_units": cus,
         },

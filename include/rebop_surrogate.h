@@ -22,13 +22,16 @@ extern "C" {
 #endif
 
 #define REBOP_SG_COMPONENTS 8 /* mixture components per species */
+#define REBOP_SG_FEATURES 16   /* input features per item */
 
 const char *rebop_sg_last_error(void);
 
 /* C[M][N] (FP32, ldc) = op(A) op(B) with BF16 A and B.  ta: A is stored
  * [K][M] (else [M][K]); tb: B is stored [N][K] (else [K][N]).  M and N
  * multiples of 64, K a multiple of 32; leading dimensions in elements,
- * multiples of 8. */
+ * multiples of 8.  When the output has fewer than 512 tiles of 64 x 64, K is
+ * split over several workgroups whose partial products are added into C
+ * atomically (so the FP32 summation order varies from run to run). */
 int rebop_sg_gemm_bf16(int ta, int tb, int M, int N, int K, const void *A, int lda, const void *B,
                        int ldb, float *C, int ldc, void *stream);
 
@@ -50,22 +53,36 @@ int rebop_sg_colsum(int M, int N, const float *X, int ldx, float *out, void *str
 int rebop_sg_adam(size_t n, float *w, const float *g, float *m, float *v, void *wb, float lr,
                   float b1, float b2, float eps, float bc1, float bc2, void *stream);
 
-/* X[b][0..ldx) (BF16) = features of item b: log10 of the 3 rate constants of
- * parameter point item_point[b], its initial E and S counts and the time of
- * sample item_sample[b] (tmax * i / nb_steps), scaled to about [-1, 1]. */
+/* X[b][0..ldx) (BF16) = the REBOP_SG_FEATURES features of item b, scaled to
+ * about [-1, 1], then zeros: for parameter point item_point[b] and the time of
+ * sample item_sample[b] (t = tmax * i / nb_steps), log10 of the three rate
+ * constants, E0, S0, t, their log10, and log10 of kcat t, kf E0 t, kf S0 t,
+ * K_M / S0 (K_M = (kb + kcat) / kf), kb / kcat, E0 / S0 and kcat E0 t / S0.
+ * bound (nullable, [B][4]) receives E0, S0, E0, S0: the counts that bound
+ * species E, S, ES, P (E + ES = E0 and S + ES + P = S0).  ldx >=
+ * REBOP_SG_FEATURES. */
 int rebop_sg_features(int B, const uint32_t *item_point, const uint32_t *item_sample,
                       const double *rates, const int64_t *x0, double tmax, int nb_steps, void *X,
-                      int ldx, void *stream);
+                      int ldx, float *bound, void *stream);
+
+/* head[b][s*3K + K + c] += ln bound[b][s] for every component c: turns the
+ * head into the mixtures' parameters (logits, log means, log dispersions). */
+int rebop_sg_head_offsets(int B, int S, int K, float *head, int ldh, const float *bound,
+                          void *stream);
 
 /* Negative log-likelihood of J SSA samples per item under the predicted
  * per-species K-component negative-binomial mixtures (K = REBOP_SG_COMPONENTS),
  * and its gradient: grad[b][...] = scale * d NLL / d head[b][...];
  * *loss_sum += NLL.  samples: an SSA output [nb+1][S][n_traj_total] in which
- * item (p, i) owns trajectories p*J .. p*J+J-1 of sample i. */
+ * item (p, i) owns trajectories p*J .. p*J+J-1 of sample i.  With bound
+ * ([B][S], nullable): the head's log means are relative to ln bound, and
+ * components K/2 .. K-1 are negative binomials of bound - x instead of x (a
+ * count just below its bound has less variance than its mean, which no
+ * negative binomial of x has). */
 int rebop_sg_nb_mixture_nll(int B, int S, int K, int J, const float *head, int ldh,
                             const uint32_t *item_point, const uint32_t *item_sample,
                             const int64_t *samples, uint64_t n_traj_total, float *grad,
-                            double *loss_sum, double scale, void *stream);
+                            double *loss_sum, double scale, const float *bound, void *stream);
 
 #ifdef __cplusplus
 }

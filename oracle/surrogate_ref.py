@@ -10,7 +10,8 @@ rebop_amd/csrc/surrogate.hip (loaded by tests/ only).
   so the only remaining difference is FP32 vs FP64 accumulation.
 - ``nll``: the negative-binomial-mixture negative log-likelihood and its
   gradient w.r.t. the head, with scipy's gammaln / digamma.
-- ``features``: the input scaling of features_kernel.
+- ``features``: the input features of features_kernel (and the bound of
+  each species: E0 for E and ES, S0 for S and P).
 """
 
 from __future__ import annotations
@@ -32,14 +33,36 @@ def bf16(x: np.ndarray) -> np.ndarray:
 
 
 def features(rates: np.ndarray, x0: np.ndarray, sample: np.ndarray, tmax: float, nb: int,
-             width: int) -> np.ndarray:
+             width: int, with_offsets: bool = False):
+    """The 16 input features of features_kernel (and optionally the log-mean offsets)."""
     B = len(sample)
     X = np.zeros((B, width))
-    X[:, 0:3] = (np.log10(rates) + 1.0) * 0.5
-    X[:, 3] = (x0[:, 0] - 105.0) / 95.0
-    X[:, 4] = (x0[:, 1] - 525.0) / 475.0
-    X[:, 5] = 2.0 * (tmax * sample / nb) / tmax - 1.0
-    return X
+    kf, kb, kc = rates[:, 0], rates[:, 1], rates[:, 2]
+    e0, s0 = x0[:, 0].astype(np.float64), x0[:, 1].astype(np.float64)
+    t = tmax * sample / nb
+    lf, lb, lc = np.log10(kf), np.log10(kb), np.log10(kc)
+    le, ls, lt = np.log10(e0), np.log10(s0), np.log10(t)
+    lkm = np.log10(kb + kc) - lf
+    X[:, 0] = (lf + 1.0) * 0.5
+    X[:, 1] = (lb + 1.0) * 0.5
+    X[:, 2] = (lc + 1.0) * 0.5
+    X[:, 3] = (e0 - 105.0) / 95.0
+    X[:, 4] = (s0 - 525.0) / 475.0
+    X[:, 5] = 2.0 * t / tmax - 1.0
+    X[:, 6] = (le - 1.65) / 0.65
+    X[:, 7] = (ls - 2.35) / 0.65
+    X[:, 8] = (lt - 0.95) / 0.75
+    X[:, 9] = (lc + lt) / 3.0
+    X[:, 10] = (lf + le + lt - 1.6) / 3.4
+    X[:, 11] = (lf + ls + lt - 2.3) / 3.4
+    X[:, 12] = (lkm - ls + 2.05) / 4.65
+    X[:, 13] = (lb - lc) / 4.0
+    X[:, 14] = (le - ls + 0.7) / 1.3
+    X[:, 15] = (lc + le + lt - ls + 0.75) / 4.05
+    if not with_offsets:
+        return X
+    bound = np.stack([e0, s0, e0, s0], axis=1)
+    return X, bound
 
 
 def forward(X, Ws, bs, emulate_bf16=True):
@@ -71,20 +94,28 @@ def backward(dhead, H, Ws, emulate_bf16=True):
     return dW, db
 
 
-def nll(head: np.ndarray, samples: np.ndarray, K: int):
-    """head [B][>= S*3K], samples [B][S][J] -> (total NLL, d NLL / d head)."""
+def nll(head: np.ndarray, samples: np.ndarray, K: int, bound: np.ndarray | None = None):
+    """head [B][>= S*3K], samples [B][S][J] -> (total NLL, d NLL / d head).
+
+    With bound [B][S]: log means are relative to ln bound, and components
+    K/2.. are negative binomials of bound - x."""
     B, S, J = samples.shape
     grad = np.zeros_like(head, dtype=np.float64)
     total = 0.0
     for s in range(S):
         h = head[:, s * 3 * K:(s + 1) * 3 * K].astype(np.float64)
         logits, lmu_raw, lr_raw = h[:, :K], h[:, K:2 * K], h[:, 2 * K:]
+        xs = samples[:, s, :].astype(np.float64)[:, :, None]  # [B][J][1]
+        x = np.broadcast_to(xs, (B, J, K)).copy()
+        if bound is not None:
+            bs = np.asarray(bound, dtype=np.float64)[:, s]
+            lmu_raw = lmu_raw + np.log(bs)[:, None]
+            x[:, :, K // 2:] = np.maximum(bs[:, None, None] - xs, 0.0)
         lmu = np.clip(lmu_raw, -8.0, 9.0)
         lr = np.clip(lr_raw, -6.0, 12.0)
         mu, r = np.exp(lmu), np.exp(lr)
         lz = np.logaddexp.reduce(logits, axis=1, keepdims=True)
         lpi = logits - lz  # [B][K]
-        x = samples[:, s, :].astype(np.float64)[:, :, None]  # [B][J][1]
         mu3, r3 = mu[:, None, :], r[:, None, :]
         lrm = np.log(r3 + mu3)
         ll = gammaln(x + r3) - gammaln(r3) - gammaln(x + 1.0) + r3 * (lr[:, None, :] - lrm) + x * (lmu[:, None, :] - lrm)

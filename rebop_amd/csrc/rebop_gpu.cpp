@@ -35,7 +35,7 @@ int lds_checkpoints();
 hipError_t lds_occupancy(int threads, size_t lds, int *blocks_per_cu);
 hipError_t launch_lds(const rebop_dev_job *jobs, const rebop_dev_item *items, uint32_t n_items,
                       unsigned long long *counters, unsigned long long *job_events, int blocks,
-                      int threads, size_t lds, hipStream_t stream);
+                      int threads, size_t lds, uint32_t rec_word_off, hipStream_t stream);
 
 } // namespace rebop
 
@@ -63,6 +63,7 @@ struct DeviceInfo {
   std::string error;
   int device = -1;
   int cus = 0;
+  size_t lds_max = 65536; // LDS bytes a workgroup may allocate
   std::string name;
 };
 
@@ -89,6 +90,7 @@ int ensure_device() {
     return fail(g_dev.status, g_dev.error);
   }
   g_dev.cus = prop.multiProcessorCount;
+  if (prop.sharedMemPerBlock > 0) g_dev.lds_max = std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024);
   g_dev.name = prop.gcnArchName;
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
     g_dev.status = REBOP_ERR_DEVICE;
@@ -238,6 +240,33 @@ int upload_net(rebop_gillespie *g) {
     jb[r + 1] = (uint32_t)js.size();
   }
   const uint32_t J = (uint32_t)js.size();
+  // LDS-kernel reaction records (layout in ssa_layout.h)
+  std::vector<uint32_t> rec((size_t)REBOP_REC_WORDS * R, 0u);
+  for (uint32_t r = 0; r < R; ++r) {
+    uint32_t *w = rec.data() + (size_t)REBOP_REC_WORDS * r;
+    uint64_t kb;
+    std::memcpy(&kb, &k[r], sizeof kb);
+    w[0] = (uint32_t)kb;
+    w[1] = (uint32_t)(kb >> 32);
+    const uint32_t nf = fb[r + 1] - fb[r];
+    bool slow_f = nf > 2;
+    for (uint32_t q = 0; q < nf && q < 2; ++q) {
+      const uint32_t f = fb[r] + q;
+      const uint32_t c = (uint32_t)fc[f];
+      if (c >= 4096u || fs[f] >= 65536u) slow_f = true;
+      w[2 + q] = (fs[f] & 0xffffu) | ((c & 0xfffu) << 16) | REBOP_REC_PRESENT;
+    }
+    if (slow_f) w[3] |= REBOP_REC_SLOW_FACTORS;
+    const uint32_t nj = jb[r + 1] - jb[r];
+    if (nj > 4) {
+      w[3] |= REBOP_REC_SLOW_JUMP;
+    } else {
+      for (uint32_t q = 0; q < nj; ++q) {
+        const uint32_t e = jb[r] + q;
+        w[4 + q] = (js[e] & 0xffu) | ((uint32_t)jd[e] << 8);
+      }
+    }
+  }
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t ok = 0;
   const size_t ofb = ok + al(sizeof(double) * R);
@@ -247,11 +276,13 @@ int upload_net(rebop_gillespie *g) {
   const size_t ojb = odf + al(sizeof(double) * R * S);
   const size_t ojs = ojb + al(sizeof(uint32_t) * (R + 1));
   const size_t ojd = ojs + al(sizeof(uint32_t) * J);
-  const size_t total = ojd + al(sizeof(int32_t) * J);
+  const size_t orc = ojd + al(sizeof(int32_t) * J);
+  const size_t total = orc + al(sizeof(uint32_t) * rec.size());
   std::vector<unsigned char> host(total, 0);
   std::memcpy(host.data() + ojb, jb.data(), sizeof(uint32_t) * (R + 1));
   if (J) std::memcpy(host.data() + ojs, js.data(), sizeof(uint32_t) * J);
   if (J) std::memcpy(host.data() + ojd, jd.data(), sizeof(int32_t) * J);
+  if (R) std::memcpy(host.data() + orc, rec.data(), sizeof(uint32_t) * rec.size());
   if (R) std::memcpy(host.data() + ok, k.data(), sizeof(double) * R);
   std::memcpy(host.data() + ofb, fb.data(), sizeof(uint32_t) * (R + 1));
   if (F) std::memcpy(host.data() + ofs, fs.data(), sizeof(uint32_t) * F);
@@ -286,6 +317,7 @@ int upload_net(rebop_gillespie *g) {
   jt.jump_begin = (const uint32_t *)(base + ojb);
   jt.jump_s = (const uint32_t *)(base + ojs);
   jt.jump_d = (const int32_t *)(base + ojd);
+  jt.rec = (const uint32_t *)(base + orc);
   g->job = jt;
   g->dirty = false;
   g->dev_id = cur;
@@ -379,12 +411,19 @@ std::vector<rebop_dev_item> make_items(const std::vector<rebop_dev_job> &jobs,
 int enqueue_lds(const std::vector<rebop_dev_job> &jobs, const std::vector<double> &cost,
                 DevBuf &djobs, DevBuf &ditems, unsigned long long *counters,
                 unsigned long long *job_events, hipStream_t stream) {
-  uint32_t smax = 1;
-  for (auto &j : jobs) smax = std::max(smax, j.S);
-  const size_t lds = (size_t)smax * kLdsThreads * sizeof(int32_t);
-  if (lds > 64 * 1024)
-    return fail(REBOP_ERR_UNSUPPORTED, "LDS kernel: " + std::to_string(smax) +
-                                           " species exceed the 64 KiB workgroup LDS budget");
+  uint32_t smax = 1, rmax = 1;
+  for (auto &j : jobs) {
+    smax = std::max(smax, j.S);
+    rmax = std::max(rmax, j.R);
+  }
+  // counts [S_max][64] int32, then the reaction records (16-byte aligned)
+  const size_t xbytes = ((size_t)smax * kLdsThreads * sizeof(int32_t) + 15) & ~(size_t)15;
+  const size_t lds = xbytes + (size_t)rmax * REBOP_REC_WORDS * sizeof(uint32_t);
+  if (lds > g_dev.lds_max)
+    return fail(REBOP_ERR_UNSUPPORTED,
+                "LDS kernel: " + std::to_string(smax) + " species and " + std::to_string(rmax) +
+                    " reactions need " + std::to_string(lds) + " bytes of LDS per workgroup (" +
+                    std::to_string(g_dev.lds_max) + " available)");
   const std::vector<rebop_dev_item> items = make_items(jobs, cost);
   if (items.empty()) return REBOP_OK;
   if (items.size() > 0xffffffffull) return fail(REBOP_ERR_VALUE, "too many work items");
@@ -403,7 +442,7 @@ int enqueue_lds(const std::vector<rebop_dev_job> &jobs, const std::vector<double
   if (blocks > need) blocks = need;
   HIP_TRY(rebop::launch_lds((const rebop_dev_job *)djobs.p, (const rebop_dev_item *)ditems.p,
                             (uint32_t)items.size(), counters, job_events, (int)blocks,
-                            kLdsThreads, lds, stream));
+                            kLdsThreads, lds, (uint32_t)(xbytes / sizeof(uint32_t)), stream));
   return REBOP_OK;
 }
 

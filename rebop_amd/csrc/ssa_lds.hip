@@ -18,6 +18,15 @@
 // its rate's order, a negative rate constant, or a count that went negative)
 // uses the full linear count instead, which is the reference's definition.
 //
+// The network is read from LDS too: when a wave starts an item it copies the
+// job's reaction records (rate constant, the first two propensity factors,
+// up to four jump entries; 32 bytes each, see ssa_layout.h) next to the
+// counts.  A make_cumrates pass then costs one uniform 16-byte LDS read and
+// two count reads per reaction, independent from one reaction to the next
+// (the loop is unrolled so the reads of later reactions overlap the
+// arithmetic of earlier ones); a missing factor multiplies by exactly 1.0.
+// Reactions whose factors or jump do not fit a record use the global lists.
+//
 // Work items are (job, up to 64 trajectories); a wave takes items from a
 // device-wide queue, so one launch can run thousands of different networks
 // (heterogeneous batch) with the items ordered largest-first by the host.
@@ -48,6 +57,8 @@ struct LdsNet {
   // this lane's counts
   int32_t *xl;
   uint32_t BL;
+  // the job's reaction records in LDS
+  const uint32_t *rec;
   // per-lane flags
   bool linear;
   bool bad;
@@ -84,11 +95,27 @@ struct LdsNet {
   }
   __device__ __forceinline__ bool in_range(const double (&)[MS]) const { return !bad; }
 
-  // Rate::rate of reaction r (src/gillespie.rs:71-90), factors in stored order
-  __device__ __forceinline__ double rate(uint32_t r) const {
+  // Rate::rate of reaction r from the global factor lists (src/gillespie.rs:71-90)
+  __device__ __forceinline__ double rate_slow(uint32_t r) const {
     double a = k[r];
     const uint32_t fe = fb[r + 1];
     for (uint32_t f = fb[r]; f < fe; ++f) a = a * (fcnt(fs[f]) - fc[f]);
+    return a;
+  }
+
+  // factor (x[s] - c) of a record word, or exactly 1.0 when absent
+  __device__ __forceinline__ double factor(uint32_t w) const {
+    const double v = fcnt(w & 0xffffu) - (double)((w >> 16) & 0xfffu);
+    return (w & REBOP_REC_PRESENT) ? v : 1.0;
+  }
+
+  // Rate::rate of reaction r, factors in stored order: k * f1 * f2
+  __device__ __forceinline__ double rate(uint32_t r) const {
+    const uint4 q = *(const uint4 *)(rec + (size_t)r * REBOP_REC_WORDS);
+    if (q.w & REBOP_REC_SLOW_FACTORS) return rate_slow(r);
+    double a = rebop_f64_from_bits(((uint64_t)q.y << 32) | q.x);
+    a = a * factor(q.z);
+    a = a * factor(q.w);
     return a;
   }
 
@@ -100,6 +127,7 @@ struct LdsNet {
     for (int b = 0; b < MB; ++b) { // static: cum[] stays in VGPRs
       if ((uint32_t)b < nblk) {
         const uint32_t re = (r + B < R) ? r + B : R;
+#pragma unroll 4
         for (; r < re; ++r) acc = acc + rate(r);
         cum[b] = acc;
       }
@@ -139,15 +167,27 @@ struct LdsNet {
     return r < re ? r : re - 1u;
   }
 
+  __device__ __forceinline__ void add_count(uint32_t s, int32_t d) {
+    const int32_t v = (int32_t)((uint32_t)cnt(s) + (uint32_t)d);
+    cnt(s) = v;
+    if (v < 0) linear = true;
+    if (v > REBOP_LDS_LIMIT || v < -REBOP_LDS_LIMIT) bad = true;
+  }
+
   // Jump::affect (src/gillespie.rs:142-152), sparse form
   __device__ __forceinline__ void affect(double (&)[MS], uint32_t kk) {
-    const uint32_t e1 = jb[kk + 1];
-    for (uint32_t e = jb[kk]; e < e1; ++e) {
-      const uint32_t s = js[e];
-      const int32_t v = (int32_t)((uint32_t)cnt(s) + (uint32_t)jd[e]);
-      cnt(s) = v;
-      if (v < 0) linear = true;
-      if (v > REBOP_LDS_LIMIT || v < -REBOP_LDS_LIMIT) bad = true;
+    const uint32_t *w = rec + (size_t)kk * REBOP_REC_WORDS;
+    if (w[3] & REBOP_REC_SLOW_JUMP) {
+      const uint32_t e1 = jb[kk + 1];
+      for (uint32_t e = jb[kk]; e < e1; ++e) add_count(js[e], jd[e]);
+      return;
+    }
+    const uint4 j = *(const uint4 *)(w + 4);
+    const uint32_t e[4] = {j.x, j.y, j.z, j.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int32_t d = (int32_t)e[q] >> 8; // arithmetic shift: the signed 24-bit difference
+      if (d != 0) add_count(e[q] & 0xffu, d);
     }
   }
 };
@@ -157,8 +197,9 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
                                                       const rebop_dev_item *__restrict__ items,
                                                       uint32_t n_items,
                                                       unsigned long long *counters,
-                                                      unsigned long long *job_events) {
-  extern __shared__ int32_t lds_x[]; // [S_max][blockDim.x]
+                                                      unsigned long long *job_events,
+                                                      uint32_t rec_word_off) {
+  extern __shared__ int32_t lds_x[]; // [S_max][blockDim.x], then the records
   __shared__ unsigned long long blk_ctr[4];
   if (threadIdx.x < 4) blk_ctr[threadIdx.x] = 0ull;
   __syncthreads();
@@ -175,6 +216,13 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
     if (it >= n_items) break;
     const rebop_dev_item item = items[it];
     const rebop_dev_job &job = jobs[item.job];
+    // the job's reaction records into LDS (every item: a wave's consecutive
+    // items may belong to different jobs)
+    uint32_t *lrec = (uint32_t *)lds_x + rec_word_off;
+    __syncthreads(); // the previous item's reads are done
+    for (uint32_t w = threadIdx.x * 4u; w < job.R * REBOP_REC_WORDS; w += blockDim.x * 4u)
+      *(uint4 *)(lrec + w) = *(const uint4 *)(job.rec + w);
+    __syncthreads();
 
     LdsNet<MB> net;
     net.nS = job.S;
@@ -191,6 +239,7 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
     net.jd = job.jump_d;
     net.xl = lds_x + threadIdx.x;
     net.BL = blockDim.x;
+    net.rec = lrec;
     net.linear = net.job_linear;
     net.bad = false;
 
@@ -249,9 +298,14 @@ hipError_t lds_occupancy(int threads, size_t lds, int *blocks_per_cu) {
 
 hipError_t launch_lds(const rebop_dev_job *jobs, const rebop_dev_item *items, uint32_t n_items,
                       unsigned long long *counters, unsigned long long *job_events, int blocks,
-                      int threads, size_t lds, hipStream_t stream) {
+                      int threads, size_t lds, uint32_t rec_word_off, hipStream_t stream) {
+  if (lds > 64 * 1024) {
+    const hipError_t e = hipFuncSetAttribute((const void *)&ssa_lds_kernel<kLdsCheckpoints>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL((ssa_lds_kernel<kLdsCheckpoints>), dim3(blocks), dim3(threads), lds, stream,
-                     jobs, items, n_items, counters, job_events);
+                     jobs, items, n_items, counters, job_events, rec_word_off);
   return hipGetLastError();
 }
 

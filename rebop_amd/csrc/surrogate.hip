@@ -27,6 +27,8 @@ namespace sg {
 // C = op(A) op(B), M x N x K, FP32 out.  !TA: A[m][k] (lda); TA: A stored
 // [k][m].  !TB: B[k][n] (ldb); TB: B stored [n][k].  Tiles 64 x 64 x 32, four
 // waves, each 32 x 32 (2 x 2 MFMA tiles).  M, N multiples of 64, K of 32.
+// With gridDim.z > 1 (split K), block z covers k in [z*K/gz, (z+1)*K/gz) and
+// adds its partial product into C (zeroed beforehand) atomically.
 template <bool TA, bool TB>
 __global__ void __launch_bounds__(256) gemm_kernel(int M, int N, int K, const __bf16 *__restrict__ A,
                                                    int lda, const __bf16 *__restrict__ B, int ldb,
@@ -42,7 +44,9 @@ __global__ void __launch_bounds__(256) gemm_kernel(int M, int N, int K, const __
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  for (int k0 = 0; k0 < K; k0 += 32) {
+  const int ksplit = K / (int)gridDim.z; // a multiple of 32 (checked by the host)
+  const int kbeg = (int)blockIdx.z * ksplit, kend = kbeg + ksplit;
+  for (int k0 = kbeg; k0 < kend; k0 += 32) {
     if (!TA) {
       const int r = tid >> 2, c = (tid & 3) * 8;
       *(bf16x8 *)&As[r][c] = *(const bf16x8 *)&A[(size_t)(m0 + r) * lda + k0 + c];
@@ -82,8 +86,16 @@ __global__ void __launch_bounds__(256) gemm_kernel(int M, int N, int K, const __
       for (int q = 0; q < 4; ++q) {
         const int row = m0 + wm + i * 16 + (lane >> 4) * 4 + q;
         const int col = n0 + wn + j * 16 + (lane & 15);
-        C[(size_t)row * ldc + col] = acc[i][j][q];
+        if (gridDim.z == 1)
+          C[(size_t)row * ldc + col] = acc[i][j][q];
+        else
+          atomicAdd(&C[(size_t)row * ldc + col], acc[i][j][q]);
       }
+}
+
+__global__ void zero_kernel(int M, int N, float *__restrict__ C, int ldc) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (size_t)M * N) C[(i / N) * ldc + i % N] = 0.f;
 }
 
 // H = bf16(act(C + bias)), act = ReLU or identity; optionally the FP32 result too.
@@ -146,19 +158,58 @@ __global__ void adam_kernel(size_t n, float *__restrict__ w, const float *__rest
 
 // Inputs of item b: (point p, sample i) -> [log10 k (3), E0, S0, t] scaled to
 // about [-1, 1]; columns 6..ldx-1 are zero.
+// Input features of item b (columns 0..REBOP_SG_FEATURES-1, the rest zero),
+// scaled to about [-1, 1]: log10 of the rate constants, E0, S0, t, their
+// logs, and the logs of the kinetics' dimensionless groups (kcat t, kf E0 t,
+// kf S0 t, K_M / S0 with K_M = (kb + kcat) / kf, kb / kcat, E0 / S0,
+// kcat E0 t / S0).  bound[b][s] (nullable) = the count that bounds species s
+// (E0 for E and ES, S0 for S and P; exact in FP32).
 __global__ void features_kernel(int B, const uint32_t *__restrict__ item_point,
                                 const uint32_t *__restrict__ item_sample, const double *__restrict__ rates,
                                 const int64_t *__restrict__ x0, double tmax, int nb_steps,
-                                __bf16 *__restrict__ X, int ldx) {
+                                __bf16 *__restrict__ X, int ldx, float *__restrict__ bound) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
   const uint32_t p = item_point[b], i = item_sample[b];
-  float f[6];
-  for (int r = 0; r < 3; ++r) f[r] = (float)((log10(rates[3 * p + r]) + 1.0) * 0.5);
-  f[3] = (float)((x0[4 * p + 0] - 105.0) / 95.0);
-  f[4] = (float)((x0[4 * p + 1] - 525.0) / 475.0);
-  f[5] = (float)(2.0 * (tmax * (double)i / (double)nb_steps) / tmax - 1.0);
-  for (int c = 0; c < ldx; ++c) X[(size_t)b * ldx + c] = (__bf16)(c < 6 ? f[c] : 0.f);
+  const double kf = rates[3 * p + 0], kb = rates[3 * p + 1], kc = rates[3 * p + 2];
+  const double e0 = (double)x0[4 * p + 0], s0 = (double)x0[4 * p + 1];
+  const double t = tmax * (double)i / (double)nb_steps;
+  const double lf = log10(kf), lb = log10(kb), lc = log10(kc);
+  const double le = log10(e0), ls = log10(s0), lt = log10(t);
+  const double lkm = log10(kb + kc) - lf;
+  float f[REBOP_SG_FEATURES];
+  f[0] = (float)((lf + 1.0) * 0.5);
+  f[1] = (float)((lb + 1.0) * 0.5);
+  f[2] = (float)((lc + 1.0) * 0.5);
+  f[3] = (float)((e0 - 105.0) / 95.0);
+  f[4] = (float)((s0 - 525.0) / 475.0);
+  f[5] = (float)(2.0 * t / tmax - 1.0);
+  f[6] = (float)((le - 1.65) / 0.65);
+  f[7] = (float)((ls - 2.35) / 0.65);
+  f[8] = (float)((lt - 0.95) / 0.75);
+  f[9] = (float)((lc + lt) / 3.0);
+  f[10] = (float)((lf + le + lt - 1.6) / 3.4);
+  f[11] = (float)((lf + ls + lt - 2.3) / 3.4);
+  f[12] = (float)((lkm - ls + 2.05) / 4.65);
+  f[13] = (float)((lb - lc) / 4.0);
+  f[14] = (float)((le - ls + 0.7) / 1.3);
+  f[15] = (float)((lc + le + lt - ls + 0.75) / 4.05);
+  for (int c = 0; c < ldx; ++c) X[(size_t)b * ldx + c] = (__bf16)(c < REBOP_SG_FEATURES ? f[c] : 0.f);
+  if (bound) {
+    bound[4 * b + 0] = (float)e0; // E  = E0 - ES
+    bound[4 * b + 1] = (float)s0; // S <= S0
+    bound[4 * b + 2] = (float)e0; // ES <= E0
+    bound[4 * b + 3] = (float)s0; // P <= S0
+  }
+}
+
+// head[b][s*3K + K + c] += ln bound[b][s]: the log means of the mixtures.
+__global__ void head_offsets_kernel(int B, int S, int K, float *__restrict__ head, int ldh,
+                                    const float *__restrict__ bound) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= B * S * K) return;
+  const int b = idx / (S * K), s = (idx / K) % S, c = idx % K;
+  head[(size_t)b * ldh + (size_t)s * 3 * K + K + c] += logf(bound[(size_t)b * S + s]);
 }
 
 // ---- special functions (FP64): recurrence to z >= 10, then the asymptotic
@@ -192,15 +243,19 @@ __device__ __forceinline__ double digamma_pos(double z) {
 // and its gradient w.r.t. the head outputs.  One wave per item, one sample
 // per lane (64 samples).  Head layout per species s (K components):
 // [s*3K + c] mixture logit, [s*3K + K + c] log mean, [s*3K + 2K + c] log
-// dispersion.  samples: the SSA output [nb+1][S][P * J], item (p, i) reads
-// samples[(i*S + s) * P*J + p*J + j].  grad = d(mean NLL over items and
-// samples)/d head (FP32, [B][ldh]); loss_sum += total NLL (FP64).
+// dispersion.  With bounds, components c < K/2 are negative binomials of the
+// count x, components c >= K/2 negative binomials of bound - x (a count
+// just below its bound is narrower than any negative binomial of x can be),
+// and every log mean is relative to ln bound.  samples: the SSA output
+// [nb+1][S][P * J], item (p, i) reads samples[(i*S + s) * P*J + p*J + j].
+// grad = d(mean NLL over items and samples)/d head (FP32, [B][ldh]);
+// loss_sum += total NLL (FP64).
 template <int K>
 __global__ void __launch_bounds__(64) nb_mixture_nll_kernel(
     int B, int S, int J, const float *__restrict__ head, int ldh,
     const uint32_t *__restrict__ item_point, const uint32_t *__restrict__ item_sample,
     const int64_t *__restrict__ samples, uint64_t n_traj_total, float *__restrict__ grad,
-    double *__restrict__ loss_sum, double scale) {
+    double *__restrict__ loss_sum, double scale, const float *__restrict__ bound) {
   const int b = blockIdx.x;
   const int j = threadIdx.x;
   if (b >= B) return;
@@ -209,7 +264,9 @@ __global__ void __launch_bounds__(64) nb_mixture_nll_kernel(
   double lane_loss = 0.0;
   for (int s = 0; s < S; ++s) {
     const float *h = head + (size_t)b * ldh + (size_t)s * 3 * K;
-    const double x = active ? (double)samples[((size_t)i * S + s) * n_traj_total + (size_t)p * J + j] : 0.0;
+    const double xs = active ? (double)samples[((size_t)i * S + s) * n_traj_total + (size_t)p * J + j] : 0.0;
+    const double bnd = bound ? (double)bound[(size_t)b * S + s] : 0.0;
+    const double off = bound ? log(bnd) : 0.0;
     // log softmax of the logits (same on every lane)
     double lmax = -1e300;
 #pragma unroll
@@ -219,17 +276,22 @@ __global__ void __launch_bounds__(64) nb_mixture_nll_kernel(
     for (int c = 0; c < K; ++c) zs += exp((double)h[c] - lmax);
     const double lz = lmax + log(zs);
     double lterm[K], mu[K], r[K], lr[K], lmu[K];
-    const double lgx1 = lgamma_pos(x + 1.0);
+    // the variable of component c: x, or (bounded species) bound - x
+    const double xr = bound ? fmax(bnd - xs, 0.0) : xs;
+    const double lgx1 = lgamma_pos(xs + 1.0);
+    const double lgr1 = bound ? lgamma_pos(xr + 1.0) : lgx1;
     double best = -1e300;
 #pragma unroll
     for (int c = 0; c < K; ++c) {
-      lmu[c] = fmin(fmax((double)h[K + c], -8.0), 9.0);
+      const bool refl = bound && c >= K / 2;
+      const double x = refl ? xr : xs;
+      lmu[c] = fmin(fmax((double)h[K + c] + off, -8.0), 9.0);
       lr[c] = fmin(fmax((double)h[2 * K + c], -6.0), 12.0);
       mu[c] = exp(lmu[c]);
       r[c] = exp(lr[c]);
       const double lrm = log(r[c] + mu[c]);
-      const double ll = lgamma_pos(x + r[c]) - lgamma_pos(r[c]) - lgx1 + r[c] * (lr[c] - lrm) +
-                        x * (lmu[c] - lrm);
+      const double ll = lgamma_pos(x + r[c]) - lgamma_pos(r[c]) - (refl ? lgr1 : lgx1) +
+                        r[c] * (lr[c] - lrm) + x * (lmu[c] - lrm);
       lterm[c] = ((double)h[c] - lz) + ll;
       best = fmax(best, lterm[c]);
     }
@@ -240,6 +302,7 @@ __global__ void __launch_bounds__(64) nb_mixture_nll_kernel(
     if (active) lane_loss -= lp;
 #pragma unroll
     for (int c = 0; c < K; ++c) {
+      const double x = (bound && c >= K / 2) ? xr : xs;
       const double gam = active ? exp(lterm[c] - lp) : 0.0;
       const double pi = exp((double)h[c] - lz);
       const double rm = r[c] + mu[c];
@@ -247,7 +310,7 @@ __global__ void __launch_bounds__(64) nb_mixture_nll_kernel(
       double gm = -gam * r[c] * (x - mu[c]) / rm;
       double gr = -gam * r[c] *
                   (digamma_pos(x + r[c]) - digamma_pos(r[c]) + (lr[c] - log(rm)) + (mu[c] - x) / rm);
-      if ((double)h[K + c] != lmu[c]) gm = 0.0; // clamped
+      if ((double)h[K + c] + off != lmu[c]) gm = 0.0; // clamped
       if ((double)h[2 * K + c] != lr[c]) gr = 0.0;
       // sum over the wave's samples
 #pragma unroll
@@ -290,9 +353,15 @@ const char *rebop_sg_last_error(void) { return hipGetErrorString(g_err); }
 int rebop_sg_gemm_bf16(int ta, int tb, int M, int N, int K, const void *A, int lda, const void *B,
                        int ldb, float *C, int ldc, void *stream) {
   if (M % 64 || N % 64 || K % 32 || M <= 0 || N <= 0 || K <= 0) return 3;
-  const dim3 grid(N / 64, M / 64);
+  // split K when the output has too few 64 x 64 tiles to fill the chip
+  int splits = 1;
+  while ((M / 64) * (N / 64) * splits < 512 && K % (64 * splits) == 0 && K / (32 * splits) >= 8)
+    splits *= 2;
+  const dim3 grid(N / 64, M / 64, splits);
   const __bf16 *a = (const __bf16 *)A, *b = (const __bf16 *)B;
   hipStream_t s = (hipStream_t)stream;
+  if (splits > 1)
+    hipLaunchKernelGGL(sg::zero_kernel, dim3(nblk((size_t)M * N, 256)), dim3(256), 0, s, M, N, C, ldc);
   if (!ta && !tb) hipLaunchKernelGGL((sg::gemm_kernel<false, false>), grid, dim3(256), 0, s, M, N, K, a, lda, b, ldb, C, ldc);
   else if (!ta && tb) hipLaunchKernelGGL((sg::gemm_kernel<false, true>), grid, dim3(256), 0, s, M, N, K, a, lda, b, ldb, C, ldc);
   else if (ta && !tb) hipLaunchKernelGGL((sg::gemm_kernel<true, false>), grid, dim3(256), 0, s, M, N, K, a, lda, b, ldb, C, ldc);
@@ -335,21 +404,30 @@ int rebop_sg_adam(size_t n, float *w, const float *g, float *m, float *v, void *
 
 int rebop_sg_features(int B, const uint32_t *item_point, const uint32_t *item_sample,
                       const double *rates, const int64_t *x0, double tmax, int nb_steps, void *X,
-                      int ldx, void *stream) {
+                      int ldx, float *bound, void *stream) {
+  if (ldx < REBOP_SG_FEATURES) return 3;
   hipLaunchKernelGGL(sg::features_kernel, dim3(nblk((size_t)B, 256)), dim3(256), 0,
                      (hipStream_t)stream, B, item_point, item_sample, rates, x0, tmax, nb_steps,
-                     (__bf16 *)X, ldx);
+                     (__bf16 *)X, ldx, bound);
+  return ret(hipGetLastError());
+}
+
+int rebop_sg_head_offsets(int B, int S, int K, float *head, int ldh, const float *bound,
+                          void *stream) {
+  if (ldh < S * 3 * K) return 3;
+  hipLaunchKernelGGL(sg::head_offsets_kernel, dim3(nblk((size_t)B * S * K, 256)), dim3(256), 0,
+                     (hipStream_t)stream, B, S, K, head, ldh, bound);
   return ret(hipGetLastError());
 }
 
 int rebop_sg_nb_mixture_nll(int B, int S, int K, int J, const float *head, int ldh,
                             const uint32_t *item_point, const uint32_t *item_sample,
                             const int64_t *samples, uint64_t n_traj_total, float *grad,
-                            double *loss_sum, double scale, void *stream) {
+                            double *loss_sum, double scale, const float *bound, void *stream) {
   if (K != REBOP_SG_COMPONENTS || J > 64 || J <= 0) return 3;
   hipLaunchKernelGGL((sg::nb_mixture_nll_kernel<REBOP_SG_COMPONENTS>), dim3(B), dim3(64), 0,
                      (hipStream_t)stream, B, S, J, head, ldh, item_point, item_sample, samples,
-                     n_traj_total, grad, loss_sum, scale);
+                     n_traj_total, grad, loss_sum, scale, bound);
   return ret(hipGetLastError());
 }
 

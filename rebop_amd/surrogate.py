@@ -32,7 +32,8 @@ from .engine import GillespieNet, Rate
 LIB_PATH = Path(__file__).resolve().parent / "librebop_surrogate.so"
 K = 8  # mixture components per species (REBOP_SG_COMPONENTS)
 S = 4  # species E, S, ES, P
-IN = 64  # input features, 6 used, padded for the matrix cores
+IN = 64  # input width: NF features, zero-padded for the matrix cores
+NF = 16  # features per item (REBOP_SG_FEATURES)
 OUT = 128  # head width, S * 3 * K = 96 used
 WIDTH = 512
 TMAX = 50.0
@@ -60,8 +61,9 @@ def load():
         "rebop_sg_cast_bf16": (i, [z, v, v, v]),
         "rebop_sg_colsum": (i, [i, i, v, i, v, v]),
         "rebop_sg_adam": (i, [z, v, v, v, v, v, f, f, f, f, f, f, v]),
-        "rebop_sg_features": (i, [i, v, v, v, v, d, i, v, i, v]),
-        "rebop_sg_nb_mixture_nll": (i, [i, i, i, i, v, i, v, v, v, c_uint64, v, v, d, v]),
+        "rebop_sg_features": (i, [i, v, v, v, v, d, i, v, i, v, v]),
+        "rebop_sg_head_offsets": (i, [i, i, i, v, i, v, v]),
+        "rebop_sg_nb_mixture_nll": (i, [i, i, i, i, v, i, v, v, v, c_uint64, v, v, d, v, v]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -121,17 +123,27 @@ def adam(w, g, m, v, wb, lr, b1, b2, eps, t, stream=None):
                              lr, b1, b2, eps, bc1, bc2, stream))
 
 
-def features(item_point, item_sample, rates, x0, X, stream=None):
+def features(item_point, item_sample, rates, x0, X, bound=None, stream=None):
+    """X [B][IN] bf16 features; bound [B][S] float32 (optional) the species' bounds."""
     B = item_point.numel()
     _ck(load().rebop_sg_features(B, _p(item_point), _p(item_sample), _p(rates), _p(x0), TMAX, NB,
-                                 _p(X), X.shape[1], stream))
+                                 _p(X), X.shape[1], _p(bound) if bound is not None else None,
+                                 stream))
 
 
-def nb_mixture_nll(head, item_point, item_sample, samples, grad, loss_sum, scale, stream=None):
+def head_offsets(head, bound, stream=None):
+    """Adds ln bound to the head's log means in place (head -> mixture parameters)."""
+    B = head.shape[0]
+    _ck(load().rebop_sg_head_offsets(B, S, K, _p(head), head.shape[1], _p(bound), stream))
+
+
+def nb_mixture_nll(head, item_point, item_sample, samples, grad, loss_sum, scale, bound=None,
+                   stream=None):
     B = item_point.numel()
     _ck(load().rebop_sg_nb_mixture_nll(B, S, K, J, _p(head), head.shape[1], _p(item_point),
                                        _p(item_sample), _p(samples), samples.shape[2], _p(grad),
-                                       _p(loss_sum), float(scale), stream))
+                                       _p(loss_sum), float(scale),
+                                       _p(bound) if bound is not None else None, stream))
 
 
 # ---------------------------------------------------------------------------
@@ -196,17 +208,17 @@ class MLP:
         self.dims = dims
         self.W, self.b, self.Wb = [], [], []
         for l in range(depth):
-            fan_in = 6 if l == 0 else dims[l]
+            fan_in = NF if l == 0 else dims[l]
             w = torch.randn(dims[l], dims[l + 1], generator=g, dtype=torch.float64) * math.sqrt(2.0 / fan_in)
             if l == 0:
-                w[6:, :] = 0.0  # padded inputs
+                w[NF:, :] = 0.0  # padded inputs
             if l == depth - 1:
                 w *= 0.1
                 w[:, S * 3 * K:] = 0.0  # padded head columns
             b = torch.zeros(dims[l + 1], dtype=torch.float64)
             if l == depth - 1:
                 for s in range(S):
-                    b[s * 3 * K + K: s * 3 * K + 2 * K] = math.log(100.0)  # log mean
+                    b[s * 3 * K + K: s * 3 * K + 2 * K] = math.log(0.5)  # log mean / bound
                     b[s * 3 * K + 2 * K: s * 3 * K + 3 * K] = math.log(10.0)  # log dispersion
             self.W.append(w.float().cuda())
             self.b.append(b.float().cuda())
@@ -273,14 +285,19 @@ class MLP:
 
 
 def train(mlp: MLP, data: SweepData, steps: int, batch: int = 4096, lr: float = 1e-3,
-          seed: int = 0, log_every: int = 0, log=None):
-    """Adam on the mean mixture NLL of the SSA samples; returns the loss history."""
+          seed: int = 0, log_every: int = 0, log=None, warmup: int = 200, lr_min: float = 1e-5):
+    """Adam on the mean mixture NLL of the SSA samples; returns the loss history.
+
+    The learning rate rises linearly over ``warmup`` steps, then follows a
+    cosine from ``lr`` down to ``lr_min``.
+    """
     import torch
 
     P = data.rates.shape[0]
     n_items = P * NB
     gen = torch.Generator(device="cuda").manual_seed(seed)
     X = torch.empty((batch, IN), dtype=torch.bfloat16, device="cuda")
+    off = torch.empty((batch, S), dtype=torch.float32, device="cuda")
     grad = torch.zeros((batch, OUT), dtype=torch.float32, device="cuda")
     loss = torch.zeros(1, dtype=torch.float64, device="cuda")
     hist = []
@@ -288,13 +305,17 @@ def train(mlp: MLP, data: SweepData, steps: int, batch: int = 4096, lr: float = 
         q = torch.randint(0, n_items, (batch,), generator=gen, device="cuda")
         ip = (q // NB).to(torch.int32)  # read as uint32 by the kernels
         isamp = (q % NB + 1).to(torch.int32)
-        features(ip, isamp, data.rates, data.x0, X)
+        features(ip, isamp, data.rates, data.x0, X, off)
         head = mlp.forward(X)
         grad.zero_()
         loss.zero_()
-        nb_mixture_nll(head, ip, isamp, data.samples, grad, loss, 1.0 / (batch * J))
+        nb_mixture_nll(head, ip, isamp, data.samples, grad, loss, 1.0 / (batch * J), off)
         dW, db = mlp.backward(grad)
-        lr_t = lr * 0.5 * (1.0 + math.cos(math.pi * it / max(steps, 1)))
+        if it < warmup:
+            lr_t = lr * (it + 1) / warmup
+        else:
+            x = (it - warmup) / max(steps - warmup, 1)
+            lr_t = lr_min + (lr - lr_min) * 0.5 * (1.0 + math.cos(math.pi * x))
         mlp.step(dW, db, lr_t)
         if log_every and (it % log_every == 0 or it == steps - 1):
             hist.append((it, float(loss.item()) / (batch * J)))
@@ -304,13 +325,15 @@ def train(mlp: MLP, data: SweepData, steps: int, batch: int = 4096, lr: float = 
 
 
 def predict(mlp: MLP, rates: np.ndarray, x0: np.ndarray, sample_idx: np.ndarray, chunk: int = 1 << 16):
-    """Mixture parameters for queries (point rows, sample index 1..32): [Q][S][3][K]."""
+    """Mixture parameters for queries (point rows, sample index 1..32): [Q][S][3][K],
+    and the species' bounds [Q][S] (components K/2.. describe bound - x)."""
     import torch
 
     rates_t = torch.as_tensor(np.ascontiguousarray(rates, dtype=np.float64), device="cuda")
     x0_t = torch.as_tensor(np.ascontiguousarray(x0, dtype=np.int64), device="cuda")
     Q = len(sample_idx)
     out = np.zeros((Q, S, 3, K))
+    bounds = np.zeros((Q, S))
     for a in range(0, Q, chunk):
         b = min(Q, a + chunk)
         B = ((b - a + 63) // 64) * 64
@@ -319,27 +342,34 @@ def predict(mlp: MLP, rates: np.ndarray, x0: np.ndarray, sample_idx: np.ndarray,
         ip[: b - a] = torch.arange(a, b, dtype=torch.int32, device="cuda")
         isamp[: b - a] = torch.as_tensor(sample_idx[a:b].astype(np.int32), device="cuda")
         X = torch.empty((B, IN), dtype=torch.bfloat16, device="cuda")
-        features(ip, isamp, rates_t, x0_t, X)
+        off = torch.empty((B, S), dtype=torch.float32, device="cuda")
+        features(ip, isamp, rates_t, x0_t, X, off)
         head = mlp.forward(X, keep=False)
+        head_offsets(head, off)
         out[a:b] = head[: b - a, : S * 3 * K].double().cpu().numpy().reshape(b - a, S, 3, K)
-    return out
+        bounds[a:b] = off[: b - a].double().cpu().numpy()
+    return out, bounds
 
 
 def infer_device(mlp: MLP, d_rates, d_x0, d_point, d_sample, B: int):
-    """Device-only inference for timing: head [B][OUT] for B queries."""
+    """Device-only inference: the mixture parameters [B][OUT] of B queries."""
     import torch
 
     X = torch.empty((B, IN), dtype=torch.bfloat16, device="cuda")
-    features(d_point, d_sample, d_rates, d_x0, X)
-    return mlp.forward(X, keep=False)
+    off = torch.empty((B, S), dtype=torch.float32, device="cuda")
+    features(d_point, d_sample, d_rates, d_x0, X, off)
+    head = mlp.forward(X, keep=False)
+    head_offsets(head, off)
+    return head
 
 
 # ---------------------------------------------------------------------------
 # distributions and the Wasserstein-1 gate
 
 
-def mixture_cdf(params: np.ndarray, xmax: int) -> np.ndarray:
-    """CDF on 0..xmax of one species' mixture (params [3][K])."""
+def mixture_cdf(params: np.ndarray, xmax: int, bound: float | None = None) -> np.ndarray:
+    """CDF on 0..xmax of one species' mixture (params [3][K]); with a bound,
+    components K/2.. are negative binomials of bound - x."""
     from scipy import stats
 
     logits, lmu, lr = params
@@ -350,14 +380,21 @@ def mixture_cdf(params: np.ndarray, xmax: int) -> np.ndarray:
     xs = np.arange(xmax + 1)
     pmf = np.zeros(xmax + 1)
     for c in range(K):
-        pmf += w[c] * stats.nbinom.pmf(xs, r[c], r[c] / (r[c] + mu[c]))
+        if bound is not None and c >= K // 2:
+            y = bound - xs
+            pc = np.where(y >= 0, stats.nbinom.pmf(np.maximum(y, 0), r[c], r[c] / (r[c] + mu[c])), 0.0)
+        else:
+            pc = stats.nbinom.pmf(xs, r[c], r[c] / (r[c] + mu[c]))
+        pmf += w[c] * pc
     return np.minimum(np.cumsum(pmf), 1.0)
 
 
-def w1_to_samples(params: np.ndarray, samples: np.ndarray) -> float:
+def w1_to_samples(params: np.ndarray, samples: np.ndarray, bound: float | None = None) -> float:
     """W1 between the mixture and an empirical distribution of integer samples."""
     xmax = int(max(samples.max(), 1) * 2 + 50)
-    F = mixture_cdf(params, xmax)
+    if bound is not None:
+        xmax = max(xmax, int(bound) + 1)
+    F = mixture_cdf(params, xmax, bound)
     emp = np.searchsorted(np.sort(samples), np.arange(xmax + 1), side="right") / len(samples)
     return float(np.abs(F - emp).sum())
 
@@ -392,11 +429,11 @@ def validate(mlp: MLP, rates: np.ndarray, x0: np.ndarray, n_ref: int, seed: int,
     P = len(rates)
     rel = np.zeros((P, len(times), S))
     for ti, i in enumerate(times):
-        params = predict(mlp, rates, x0, np.full(P, i))
+        params, bounds = predict(mlp, rates, x0, np.full(P, i))
         for p in range(P):
             for s in range(S):
                 xs = samp[i, s, p * n_ref:(p + 1) * n_ref]
-                w1 = w1_to_samples(params[p, s], xs)
+                w1 = w1_to_samples(params[p, s], xs, bounds[p, s])
                 mean = xs.mean()
                 rel[p, ti, s] = w1 / mean if mean > 0 else (0.0 if w1 <= tol else np.inf)
     ok = (rel <= tol).all(axis=(1, 2))

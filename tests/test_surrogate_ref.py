@@ -87,3 +87,71 @@ def test_mlp_backward_matches_torch_autograd():
     for l in range(3):
         assert np.allclose(tW[l].grad.numpy(), dW[l], rtol=1e-10, atol=1e-12)
         assert np.allclose(tb[l].grad.numpy(), db[l], rtol=1e-10, atol=1e-12)
+
+
+def test_nll_with_bounds_matches_scipy():
+    """With bounds: log means relative to ln bound, components K/2.. are
+    negative binomials of bound - x (scipy's pmf, written out)."""
+    rng = np.random.default_rng(4)
+    B, S, J = 3, 4, 12
+    head = _random_head(B, S, rng)
+    for s in range(S):  # log-fractions of the bound
+        head[:, s * 3 * K + K:s * 3 * K + 2 * K] = rng.normal(-1.0, 0.7, size=(B, K))
+    bound = rng.integers(80, 200, size=(B, S)).astype(float)
+    samples = np.minimum(rng.poisson(40, size=(B, S, J)), bound[:, :, None].astype(int))
+    total, _ = ref.nll(head, samples, K, bound=bound)
+    expect = 0.0
+    for b in range(B):
+        for s in range(S):
+            h = head[b, s * 3 * K:(s + 1) * 3 * K]
+            w = np.exp(h[:K] - np.logaddexp.reduce(h[:K]))
+            mu, r = bound[b, s] * np.exp(h[K:2 * K]), np.exp(h[2 * K:])
+            for x in samples[b, s]:
+                p = 0.0
+                for c in range(K):
+                    y = x if c < K // 2 else bound[b, s] - x
+                    p += w[c] * stats.nbinom.pmf(y, r[c], r[c] / (r[c] + mu[c]))
+                expect -= np.log(p)
+    assert total == pytest.approx(expect, rel=1e-10)
+
+
+def test_nll_with_bounds_gradient_matches_finite_differences():
+    rng = np.random.default_rng(6)
+    B, S, J = 2, 4, 8
+    head = _random_head(B, S, rng)
+    for s in range(S):
+        head[:, s * 3 * K + K:s * 3 * K + 2 * K] = rng.normal(-1.0, 0.5, size=(B, K))
+    bound = rng.integers(60, 120, size=(B, S)).astype(float)
+    samples = np.minimum(rng.poisson(30, size=(B, S, J)), bound[:, :, None].astype(int))
+    _, g = ref.nll(head, samples, K, bound=bound)
+    for idx in [(0, K + 1), (1, K + 6), (0, 2 * K + 6), (1, 3 * K + 2 * K + 5), (0, 3 * 3 * K + 7)]:
+        h1, h2 = head.copy(), head.copy()
+        eps = 1e-6
+        h1[idx] += eps
+        h2[idx] -= eps
+        fd = (ref.nll(h1, samples, K, bound=bound)[0] - ref.nll(h2, samples, K, bound=bound)[0]) / (2 * eps)
+        assert g[idx] == pytest.approx(fd, rel=1e-5, abs=1e-7), idx
+
+
+def test_features_are_the_stated_groups():
+    """Each feature against its definition computed from the raw parameters
+    (catches swapped rate constants or species)."""
+    rates = np.array([[0.01, 2.0, 0.3], [5.0, 0.002, 1.5]])
+    x0 = np.array([[20, 800, 0, 0], [150, 60, 0, 0]])
+    sample = np.array([4.0, 32.0])
+    X, bnd = ref.features(rates, x0, sample, 50.0, 32, 64, with_offsets=True)
+    t = 50.0 * sample / 32
+    kf, kb, kc = rates.T
+    e0, s0 = x0[:, 0].astype(float), x0[:, 1].astype(float)
+    assert np.allclose(10 ** (2 * X[:, 0] - 1), kf) and np.allclose(10 ** (2 * X[:, 2] - 1), kc)
+    assert np.allclose(X[:, 3] * 95 + 105, e0) and np.allclose(X[:, 4] * 475 + 525, s0)
+    assert np.allclose((X[:, 5] + 1) / 2 * 50, t)
+    assert np.allclose(10 ** (3 * X[:, 9]), kc * t)
+    assert np.allclose(10 ** (3.4 * X[:, 10] + 1.6), kf * e0 * t)
+    assert np.allclose(10 ** (3.4 * X[:, 11] + 2.3), kf * s0 * t)
+    assert np.allclose(10 ** (4.65 * X[:, 12] - 2.05), (kb + kc) / kf / s0)
+    assert np.allclose(10 ** (4 * X[:, 13]), kb / kc)
+    assert np.allclose(10 ** (1.3 * X[:, 14] - 0.7), e0 / s0)
+    assert np.allclose(10 ** (4.05 * X[:, 15] - 0.75), kc * e0 * t / s0)
+    assert np.all(X[:, 16:] == 0)
+    assert np.array_equal(bnd, np.stack([e0, s0, e0, s0], axis=1))

@@ -398,8 +398,8 @@ def main() -> int:
     F = wl.ops_per_event()
     achieved = ev_per_launch * F / avg_kernel_s / 1e12
     peak = PEAK_FP64_OPS / 1e12
-    traffic = parse_traffic(ROOT / "profiles" / f"ssa_{wl.name}_traffic.txt") if wl.name != "dimer" \
-        else parse_traffic(ROOT / "profiles" / "ssa_dimer_1e6_traffic.txt")
+    stem = {"dimer": "ssa_dimer_1e6", "sir": "ssa_sir_1e4", "syn": "ssa_syn", "het": "ssa_het"}[wl.name]
+    traffic = parse_traffic(ROOT / "profiles" / f"{stem}_traffic.txt")
     line = {
         "metric": "SSA reaction events/sec on 1 MI355X",
         "value": value,

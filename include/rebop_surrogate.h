@@ -45,7 +45,8 @@ int rebop_sg_relu_grad(int M, int N, float *dH, int ldd, const void *H, int ldh,
 
 int rebop_sg_cast_bf16(size_t n, const float *x, void *y, void *stream);
 
-/* out[n] = sum over m of X[m][n]. */
+/* out[n] = sum over m of X[m][n] (partial sums over row blocks are added
+ * atomically, so the FP32 summation order varies from run to run). */
 int rebop_sg_colsum(int M, int N, const float *X, int ldx, float *out, void *stream);
 
 /* One Adam step on n FP32 weights; wb (nullable) receives their BF16 copy.
@@ -65,8 +66,9 @@ int rebop_sg_features(int B, const uint32_t *item_point, const uint32_t *item_sa
                       const double *rates, const int64_t *x0, double tmax, int nb_steps, void *X,
                       int ldx, float *bound, void *stream);
 
-/* head[b][s*3K + K + c] += ln bound[b][s] for every component c: turns the
- * head into the mixtures' parameters (logits, log means, log dispersions). */
+/* head[b][s*3K + K + c] += ln bound[b][s] for the negative-binomial
+ * components c < K/2: turns the head into the mixtures' parameters (logits;
+ * log means and log dispersions, or logit p for the binomial components). */
 int rebop_sg_head_offsets(int B, int S, int K, float *head, int ldh, const float *bound,
                           void *stream);
 
@@ -75,10 +77,11 @@ int rebop_sg_head_offsets(int B, int S, int K, float *head, int ldh, const float
  * and its gradient: grad[b][...] = scale * d NLL / d head[b][...];
  * *loss_sum += NLL.  samples: an SSA output [nb+1][S][n_traj_total] in which
  * item (p, i) owns trajectories p*J .. p*J+J-1 of sample i.  With bound
- * ([B][S], nullable): the head's log means are relative to ln bound, and
- * components K/2 .. K-1 are negative binomials of bound - x instead of x (a
- * count just below its bound has less variance than its mean, which no
- * negative binomial of x has). */
+ * ([B][S], nullable): components 0 .. K/2-1 are negative binomials whose log
+ * means are relative to ln bound, and components K/2 .. K-1 are
+ * binomial(bound, p) with logit p in the log-mean slot and an unused
+ * dispersion slot (a bounded count can vary less than its mean, which no
+ * negative binomial does). */
 int rebop_sg_nb_mixture_nll(int B, int S, int K, int J, const float *head, int ldh,
                             const uint32_t *item_point, const uint32_t *item_sample,
                             const int64_t *samples, uint64_t n_traj_total, float *grad,

@@ -94,41 +94,63 @@ def backward(dhead, H, Ws, emulate_bf16=True):
     return dW, db
 
 
+def _softplus(z):
+    return np.where(z > 0, z + np.log1p(np.exp(-np.abs(z))), np.log1p(np.exp(np.minimum(z, 0.0))))
+
+
 def nll(head: np.ndarray, samples: np.ndarray, K: int, bound: np.ndarray | None = None):
     """head [B][>= S*3K], samples [B][S][J] -> (total NLL, d NLL / d head).
 
-    With bound [B][S]: log means are relative to ln bound, and components
-    K/2.. are negative binomials of bound - x."""
+    Without bound every component is a negative binomial.  With bound [B][S]:
+    components < K/2 are negative binomials with log means relative to
+    ln bound, components >= K/2 are binomial(bound, sigmoid(head log-mean
+    slot)) and their dispersion slot gets no gradient."""
     B, S, J = samples.shape
     grad = np.zeros_like(head, dtype=np.float64)
     total = 0.0
     for s in range(S):
         h = head[:, s * 3 * K:(s + 1) * 3 * K].astype(np.float64)
-        logits, lmu_raw, lr_raw = h[:, :K], h[:, K:2 * K], h[:, 2 * K:]
-        xs = samples[:, s, :].astype(np.float64)[:, :, None]  # [B][J][1]
-        x = np.broadcast_to(xs, (B, J, K)).copy()
+        logits, lmu_raw, lr_raw = h[:, :K], h[:, K:2 * K].copy(), h[:, 2 * K:]
+        x = samples[:, s, :].astype(np.float64)[:, :, None]  # [B][J][1]
+        nbk = K // 2 if bound is not None else K
         if bound is not None:
             bs = np.asarray(bound, dtype=np.float64)[:, s]
-            lmu_raw = lmu_raw + np.log(bs)[:, None]
-            x[:, :, K // 2:] = np.maximum(bs[:, None, None] - xs, 0.0)
-        lmu = np.clip(lmu_raw, -8.0, 9.0)
-        lr = np.clip(lr_raw, -6.0, 12.0)
-        mu, r = np.exp(lmu), np.exp(lr)
+            lmu_raw[:, :nbk] += np.log(bs)[:, None]
         lz = np.logaddexp.reduce(logits, axis=1, keepdims=True)
         lpi = logits - lz  # [B][K]
-        mu3, r3 = mu[:, None, :], r[:, None, :]
+        ll = np.zeros((B, J, K))
+        # negative-binomial components
+        lmu = np.clip(lmu_raw[:, :nbk], -8.0, 9.0)
+        lr = np.clip(lr_raw[:, :nbk], -6.0, 12.0)
+        mu3, r3 = np.exp(lmu)[:, None, :], np.exp(lr)[:, None, :]
         lrm = np.log(r3 + mu3)
-        ll = gammaln(x + r3) - gammaln(r3) - gammaln(x + 1.0) + r3 * (lr[:, None, :] - lrm) + x * (lmu[:, None, :] - lrm)
+        ll[:, :, :nbk] = (gammaln(x + r3) - gammaln(r3) - gammaln(x + 1.0) + r3 * (lr[:, None, :] - lrm)
+                          + x * (lmu[:, None, :] - lrm))
+        if bound is not None:
+            th = np.clip(lmu_raw[:, nbk:], -30.0, 30.0)[:, None, :]
+            bb = bs[:, None, None]
+            lcb = gammaln(bb + 1.0) - gammaln(x + 1.0) - gammaln(np.maximum(bb - x, 0.0) + 1.0)
+            ll[:, :, nbk:] = lcb - x * _softplus(-th) - (bb - x) * _softplus(th)
         lt = lpi[:, None, :] + ll  # [B][J][K]
         lp = np.logaddexp.reduce(lt, axis=2, keepdims=True)
         total -= lp.sum()
         gam = np.exp(lt - lp)
         pi = np.exp(lpi)[:, None, :]
         gl = (pi - gam).sum(axis=1)
-        gm = (-gam * r3 * (x - mu3) / (r3 + mu3)).sum(axis=1)
-        gr = (-gam * r3 * (digamma(x + r3) - digamma(r3) + (lr[:, None, :] - lrm) + (mu3 - x) / (r3 + mu3))).sum(axis=1)
-        gm[lmu_raw != lmu] = 0.0
-        gr[lr_raw != lr] = 0.0
+        gn = gam[:, :, :nbk]
+        gm = np.zeros((B, K))
+        gr = np.zeros((B, K))
+        gm[:, :nbk] = (-gn * r3 * (x - mu3) / (r3 + mu3)).sum(axis=1)
+        gr[:, :nbk] = (-gn * r3 * (digamma(x + r3) - digamma(r3) + (lr[:, None, :] - lrm)
+                                   + (mu3 - x) / (r3 + mu3))).sum(axis=1)
+        clamp_m = np.zeros((B, K), dtype=bool)
+        clamp_m[:, :nbk] = lmu_raw[:, :nbk] != lmu
+        if bound is not None:
+            p = 1.0 / (1.0 + np.exp(-th))
+            gm[:, nbk:] = (-gam[:, :, nbk:] * (x - bb * p)).sum(axis=1)
+            clamp_m[:, nbk:] = lmu_raw[:, nbk:] != th[:, 0, :]
+        gm[clamp_m] = 0.0
+        gr[:, :nbk][lr_raw[:, :nbk] != lr] = 0.0
         grad[:, s * 3 * K:s * 3 * K + K] = gl
         grad[:, s * 3 * K + K:s * 3 * K + 2 * K] = gm
         grad[:, s * 3 * K + 2 * K:(s + 1) * 3 * K] = gr

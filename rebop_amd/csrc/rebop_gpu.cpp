@@ -242,6 +242,7 @@ int upload_net(rebop_gillespie *g) {
   const uint32_t J = (uint32_t)js.size();
   // LDS-kernel reaction records (layout in ssa_layout.h)
   std::vector<uint32_t> rec((size_t)REBOP_REC_WORDS * R, 0u);
+  bool any_slow = false;
   for (uint32_t r = 0; r < R; ++r) {
     uint32_t *w = rec.data() + (size_t)REBOP_REC_WORDS * r;
     uint64_t kb;
@@ -256,7 +257,10 @@ int upload_net(rebop_gillespie *g) {
       if (c >= 4096u || fs[f] >= 65536u) slow_f = true;
       w[2 + q] = (fs[f] & 0xffffu) | ((c & 0xfffu) << 16) | REBOP_REC_PRESENT;
     }
-    if (slow_f) w[3] |= REBOP_REC_SLOW_FACTORS;
+    if (slow_f) {
+      w[3] |= REBOP_REC_SLOW_FACTORS;
+      any_slow = true;
+    }
     const uint32_t nj = jb[r + 1] - jb[r];
     if (nj > 4) {
       w[3] |= REBOP_REC_SLOW_JUMP;
@@ -309,7 +313,7 @@ int upload_net(rebop_gillespie *g) {
   jt.R = R;
   jt.block = R ? (R + MB - 1) / MB : 1;
   jt.nblocks = R ? (R + jt.block - 1) / jt.block : 0;
-  jt.flags = linear ? REBOP_JOB_LINEAR : 0u;
+  jt.flags = (linear ? REBOP_JOB_LINEAR : 0u) | (any_slow ? REBOP_JOB_SLOW : 0u);
   jt.k = g->view.k;
   jt.fac_begin = g->view.fac_begin;
   jt.fac_s = g->view.fac_s;

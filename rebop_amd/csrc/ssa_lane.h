@@ -14,7 +14,7 @@
 //                       (choose_cumrate_sum, gillespie.rs:402-407), apply
 //                       reaction k (Jump::affect, :142-152); the propensities
 //                       are recomputed in the reference's order (make_cumrates
-//                       :357-364 over Rate::rate :71-90) at the next iteration
+//                       :357-364 over Rate::rate :71-90) right after the jump
 //
 // The network enters through a policy object `net`.  The lane's register
 // state is two local arrays of lane_loop, double x[Net::MS] (counts, for
@@ -116,6 +116,19 @@ __device__ __forceinline__ uint64_t next_trajectory(const rebop_dev_run &run, bo
   return queue_offset + base + (uint64_t)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
+// make_cumrates after a change of the counts.  An infinite total would stop
+// time (E / tot == 0) for good: it flags the run and is returned as 0, so the
+// trajectory runs out as an absorbing state and every lane finishes.
+template <class Net, class X, class C>
+__device__ __forceinline__ double props_checked(Net &net, X &x, C &cum, LaneTotals &acc) {
+  double tot = net.props(x, cum);
+  if (tot == __builtin_inf()) {
+    acc.flags |= REBOP_FLAG_RANGE;
+    tot = 0.0;
+  }
+  return tot;
+}
+
 // Runs trajectories on this lane until none is left.  `first` is the lane's
 // first local trajectory index (>= run.n_traj for none); a persistent launch
 // then takes indices queue_offset + atomicAdd(queue) from the device queue.
@@ -137,7 +150,7 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
   int64_t *orec = nullptr;
   uint32_t i = 0, ov = 0;
   LaneTotals acc;
-  bool done = true, started = false, need_props = false;
+  bool done = true, started = false;
 
   for (;;) {
     if (done) {
@@ -163,19 +176,8 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
       i = 0;
       T = grid_time(run, 0);
       orec = run.out ? run.out + jl : nullptr;
-      need_props = true;
       done = false;
-    }
-
-    if (need_props) {
-      tot = net.props(x, cum);
-      need_props = false;
-      // an infinite total would stop time (E / tot == 0) for good: flag the
-      // run and let the trajectory run out as an absorbing state
-      if (tot == __builtin_inf()) {
-        acc.flags |= REBOP_FLAG_RANGE;
-        tot = 0.0;
-      }
+      tot = props_checked(net, x, cum, acc);
     }
 
     bool rec = true; // absorbing state: sample without drawing
@@ -216,7 +218,7 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
     } else {
       t = tn;
       net.affect(x, net.choose(x, cum, tot * u));
-      need_props = true;
+      tot = props_checked(net, x, cum, acc);
     }
   }
   return acc;
@@ -293,7 +295,8 @@ __device__ __forceinline__ LaneTotals lane_loop_group(Net &net, const rebop_dev_
   uint32_t i = 0, ov = 0;
   double T = grid_time(run, 0), tot = 0.0;
   int64_t *orec = run.out && lead ? run.out + jl : nullptr;
-  bool need_props = true, done = false;
+  bool done = false;
+  tot = props_checked(net, x, cum, acc);
 
   while (!done) {
     const rebop_u32x4 blk = rebop_draw_block(run.seed, run.traj_begin + jl, draw + sub);
@@ -303,14 +306,6 @@ __device__ __forceinline__ LaneTotals lane_loop_group(Net &net, const rebop_dev_
       const double e = group_read<K>(e_own, q);
       const double u = group_read<K>(u_own, q);
       if (!done) {
-        if (need_props) {
-          tot = net.props(x, cum);
-          need_props = false;
-          if (tot == __builtin_inf()) {
-            acc.flags |= REBOP_FLAG_RANGE;
-            tot = 0.0;
-          }
-        }
         bool rec = true;
         double tn = 0.0;
         if (0.0 < tot) {
@@ -348,7 +343,7 @@ __device__ __forceinline__ LaneTotals lane_loop_group(Net &net, const rebop_dev_
         } else {
           t = tn;
           net.affect(x, net.choose(x, cum, tot * u));
-          need_props = true;
+          tot = props_checked(net, x, cum, acc);
         }
       }
     }

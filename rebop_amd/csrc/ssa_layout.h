@@ -91,6 +91,7 @@ typedef struct {
  * `block` for the checkpointed cumulative sum.  The jump is sparse.
  */
 #define REBOP_JOB_LINEAR 1u /* cumulative sum may be non-monotone: full linear search */
+#define REBOP_JOB_SLOW 2u   /* some reaction's factors are not all in its record */
 #define REBOP_LDS_LIMIT 1073741824 /* |count| <= 2^30 in the int32 LDS kernel */
 
 /*

@@ -47,6 +47,7 @@ struct LdsNet {
   // job fields (wave-uniform)
   uint32_t nS, R, B, nblk;
   bool job_linear;
+  bool job_slow; // some reaction needs the global factor lists
   const double *k;
   const uint32_t *fb;
   const uint32_t *fs;
@@ -119,15 +120,45 @@ struct LdsNet {
     return a;
   }
 
+  // Rate::rate of reaction r when its factors are all in the record
+  __device__ __forceinline__ double rate_fast(uint32_t r) const {
+    const uint4 q = *(const uint4 *)(rec + (size_t)r * REBOP_REC_WORDS);
+    double a = rebop_f64_from_bits(((uint64_t)q.y << 32) | q.x);
+    a = a * factor(q.z);
+    a = a * factor(q.w);
+    return a;
+  }
+
   // make_cumrates with a checkpoint at the end of every block
   __device__ __forceinline__ double props(double (&)[MS], double (&cum)[MR]) const {
     double acc = 0.0;
     uint32_t r = 0;
+    if (!job_slow) {
+      // every reaction's factors are in its record: four reactions per step,
+      // whose record and count reads are independent of each other (the sum
+      // still adds them one by one, in order)
 #pragma unroll
-    for (int b = 0; b < MB; ++b) { // static: cum[] stays in VGPRs
+      for (int b = 0; b < MB; ++b) { // static: cum[] stays in VGPRs
+        if ((uint32_t)b < nblk) {
+          const uint32_t re = (r + B < R) ? r + B : R;
+          for (; r + 4 <= re; r += 4) {
+            const double a0 = rate_fast(r), a1 = rate_fast(r + 1);
+            const double a2 = rate_fast(r + 2), a3 = rate_fast(r + 3);
+            acc = acc + a0;
+            acc = acc + a1;
+            acc = acc + a2;
+            acc = acc + a3;
+          }
+          for (; r < re; ++r) acc = acc + rate_fast(r);
+          cum[b] = acc;
+        }
+      }
+      return acc;
+    }
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
       if ((uint32_t)b < nblk) {
         const uint32_t re = (r + B < R) ? r + B : R;
-#pragma unroll 4
         for (; r < re; ++r) acc = acc + rate(r);
         cum[b] = acc;
       }
@@ -230,6 +261,7 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
     net.B = job.block;
     net.nblk = job.nblocks;
     net.job_linear = (job.flags & REBOP_JOB_LINEAR) != 0u;
+    net.job_slow = (job.flags & REBOP_JOB_SLOW) != 0u;
     net.k = job.k;
     net.fb = job.fac_begin;
     net.fs = job.fac_s;

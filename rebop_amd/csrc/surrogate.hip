@@ -1,8 +1,9 @@
 // surrogate.hip -- kernels of the learned surrogate of the SSA.
 //
 // A multilayer perceptron maps (log rate constants, initial counts, t) of a
-// Michaelis-Menten network to a K-component negative-binomial mixture per
-// species.  Its dense layers run on the matrix cores: BF16 operands, FP32
+// Michaelis-Menten network to a K-component mixture per species (negative
+// binomials, and binomials on the species' bound).  Its dense layers run on
+// the matrix cores: BF16 operands, FP32
 // accumulation (v_mfma_f32_16x16x32_bf16).  The loss head (mixture negative
 // log-likelihood of the SSA samples and its gradient) runs in FP64.  Adam
 // keeps FP32 master weights and writes the BF16 copy the GEMMs read.
@@ -131,13 +132,26 @@ __global__ void cast_bf16_kernel(size_t n, const float *__restrict__ x, __bf16 *
 }
 
 // out[n] = sum_m X[m][n]: one thread per column, rows strided (coalesced).
-__global__ void colsum_kernel(int M, int N, const float *__restrict__ X, int ldx,
-                              float *__restrict__ out) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+// Column sums over a 2-D grid: block (x, y) sums columns 64x .. 64x+63 over
+// rows 256y .. 256y+255 (four row groups of 64 in parallel, combined in LDS)
+// and adds its partial sums into out (zeroed first) atomically.
+__global__ void __launch_bounds__(256) colsum_kernel(int M, int N, const float *__restrict__ X,
+                                                     int ldx, float *__restrict__ out) {
+  __shared__ float part[4][64];
+  const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int n = blockIdx.x * 64 + c;
+  const int m0 = blockIdx.y * 256 + g * 64;
   float s = 0.f;
-  for (int m = 0; m < M; ++m) s += X[(size_t)m * ldx + n];
-  out[n] = s;
+  if (n < N)
+    for (int m = m0; m < m0 + 64 && m < M; ++m) s += X[(size_t)m * ldx + n];
+  part[g][c] = s;
+  __syncthreads();
+  if (g == 0 && n < N) atomicAdd(&out[n], part[0][c] + part[1][c] + part[2][c] + part[3][c]);
+}
+
+__global__ void fill_kernel(int n, float *__restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0.f;
 }
 
 // Adam with FP32 master weights; writes the BF16 copy read by the GEMMs.
@@ -203,12 +217,14 @@ __global__ void features_kernel(int B, const uint32_t *__restrict__ item_point,
   }
 }
 
-// head[b][s*3K + K + c] += ln bound[b][s]: the log means of the mixtures.
+// head[b][s*3K + K + c] += ln bound[b][s] for the negative-binomial
+// components c < K/2: their log means (the binomial components' logits stay).
 __global__ void head_offsets_kernel(int B, int S, int K, float *__restrict__ head, int ldh,
                                     const float *__restrict__ bound) {
+  const int half = K / 2;
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= B * S * K) return;
-  const int b = idx / (S * K), s = (idx / K) % S, c = idx % K;
+  if (idx >= B * S * half) return;
+  const int b = idx / (S * half), s = (idx / half) % S, c = idx % half;
   head[(size_t)b * ldh + (size_t)s * 3 * K + K + c] += logf(bound[(size_t)b * S + s]);
 }
 
@@ -216,11 +232,14 @@ __global__ void head_offsets_kernel(int B, int S, int K, float *__restrict__ hea
 // series of ln Gamma (Stirling) and of the digamma function, with the
 // Bernoulli-number coefficients B_2n / (2n (2n-1)) and B_2n / (2n).
 __device__ __forceinline__ double lgamma_pos(double z) {
-  double shift = 0.0;
+  // ln Gamma(z) = ln Gamma(z + n) - ln(z (z+1) ... (z+n-1)); one log of the
+  // product (at most 10 factors below 10: no overflow)
+  double prod = 1.0;
   while (z < 10.0) {
-    shift += log(z);
+    prod *= z;
     z += 1.0;
   }
+  const double shift = log(prod);
   const double iz = 1.0 / z, iz2 = iz * iz;
   const double series =
       iz * (1.0 / 12.0 - iz2 * (1.0 / 360.0 - iz2 * (1.0 / 1260.0 - iz2 * (1.0 / 1680.0 - iz2 * (1.0 / 1188.0)))));
@@ -244,9 +263,10 @@ __device__ __forceinline__ double digamma_pos(double z) {
 // per lane (64 samples).  Head layout per species s (K components):
 // [s*3K + c] mixture logit, [s*3K + K + c] log mean, [s*3K + 2K + c] log
 // dispersion.  With bounds, components c < K/2 are negative binomials of the
-// count x, components c >= K/2 negative binomials of bound - x (a count
-// just below its bound is narrower than any negative binomial of x can be),
-// and every log mean is relative to ln bound.  samples: the SSA output
+// count x with log means relative to ln bound, and components c >= K/2 are
+// binomial(bound, p) with logit p in the log-mean slot (a bounded count can
+// vary less than its mean, which no negative binomial does; their dispersion
+// slot is unused).  samples: the SSA output
 // [nb+1][S][P * J], item (p, i) reads samples[(i*S + s) * P*J + p*J + j].
 // grad = d(mean NLL over items and samples)/d head (FP32, [B][ldh]);
 // loss_sum += total NLL (FP64).
@@ -276,22 +296,30 @@ __global__ void __launch_bounds__(64) nb_mixture_nll_kernel(
     for (int c = 0; c < K; ++c) zs += exp((double)h[c] - lmax);
     const double lz = lmax + log(zs);
     double lterm[K], mu[K], r[K], lr[K], lmu[K];
-    // the variable of component c: x, or (bounded species) bound - x
-    const double xr = bound ? fmax(bnd - xs, 0.0) : xs;
     const double lgx1 = lgamma_pos(xs + 1.0);
-    const double lgr1 = bound ? lgamma_pos(xr + 1.0) : lgx1;
+    // ln C(bound, x) for the binomial components of a bounded species
+    const double lcb = bound ? lgamma_pos(bnd + 1.0) - lgx1 - lgamma_pos(fmax(bnd - xs, 0.0) + 1.0) : 0.0;
     double best = -1e300;
 #pragma unroll
     for (int c = 0; c < K; ++c) {
-      const bool refl = bound && c >= K / 2;
-      const double x = refl ? xr : xs;
-      lmu[c] = fmin(fmax((double)h[K + c] + off, -8.0), 9.0);
-      lr[c] = fmin(fmax((double)h[2 * K + c], -6.0), 12.0);
-      mu[c] = exp(lmu[c]);
-      r[c] = exp(lr[c]);
-      const double lrm = log(r[c] + mu[c]);
-      const double ll = lgamma_pos(x + r[c]) - lgamma_pos(r[c]) - (refl ? lgr1 : lgx1) +
-                        r[c] * (lr[c] - lrm) + x * (lmu[c] - lrm);
+      double ll;
+      if (bound && c >= K / 2) {
+        // binomial(bound, p), p = sigmoid(theta): ln p = -softplus(-theta)
+        lmu[c] = fmin(fmax((double)h[K + c], -30.0), 30.0);
+        const double th = lmu[c];
+        const double sp_neg = th < 0.0 ? log1p(exp(th)) - th : log1p(exp(-th)); // softplus(-th)
+        const double sp_pos = sp_neg + th;                                          // softplus(th)
+        mu[c] = exp(-sp_neg); // p
+        ll = lcb - xs * sp_neg - (bnd - xs) * sp_pos;
+      } else {
+        lmu[c] = fmin(fmax((double)h[K + c] + off, -8.0), 9.0);
+        lr[c] = fmin(fmax((double)h[2 * K + c], -6.0), 12.0);
+        mu[c] = exp(lmu[c]);
+        r[c] = exp(lr[c]);
+        const double lrm = log(r[c] + mu[c]);
+        ll = lgamma_pos(xs + r[c]) - lgamma_pos(r[c]) - lgx1 + r[c] * (lr[c] - lrm) +
+             xs * (lmu[c] - lrm);
+      }
       lterm[c] = ((double)h[c] - lz) + ll;
       best = fmax(best, lterm[c]);
     }
@@ -302,22 +330,28 @@ __global__ void __launch_bounds__(64) nb_mixture_nll_kernel(
     if (active) lane_loss -= lp;
 #pragma unroll
     for (int c = 0; c < K; ++c) {
-      const double x = (bound && c >= K / 2) ? xr : xs;
       const double gam = active ? exp(lterm[c] - lp) : 0.0;
       const double pi = exp((double)h[c] - lz);
-      const double rm = r[c] + mu[c];
       double gl = active ? (pi - gam) : 0.0;
-      double gm = -gam * r[c] * (x - mu[c]) / rm;
-      double gr = -gam * r[c] *
-                  (digamma_pos(x + r[c]) - digamma_pos(r[c]) + (lr[c] - log(rm)) + (mu[c] - x) / rm);
-      if ((double)h[K + c] + off != lmu[c]) gm = 0.0; // clamped
-      if ((double)h[2 * K + c] != lr[c]) gr = 0.0;
+      double gm, gr;
+      if (bound && c >= K / 2) {
+        gm = -gam * (xs - bnd * mu[c]); // d/dtheta of -ln binomial
+        gr = 0.0;
+        if ((double)h[K + c] != lmu[c]) gm = 0.0; // clamped
+      } else {
+        const double rm = r[c] + mu[c];
+        gm = -gam * r[c] * (xs - mu[c]) / rm;
+        gr = -gam * r[c] *
+             (digamma_pos(xs + r[c]) - digamma_pos(r[c]) + (lr[c] - log(rm)) + (mu[c] - xs) / rm);
+        if ((double)h[K + c] + off != lmu[c]) gm = 0.0; // clamped
+        if ((double)h[2 * K + c] != lr[c]) gr = 0.0;
+      }
       // sum over the wave's samples
 #pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        gl += __shfl_xor(gl, off);
-        gm += __shfl_xor(gm, off);
-        gr += __shfl_xor(gr, off);
+      for (int o = 32; o >= 1; o >>= 1) {
+        gl += __shfl_xor(gl, o);
+        gm += __shfl_xor(gm, o);
+        gr += __shfl_xor(gr, o);
       }
       if (j == 0) {
         float *g = grad + (size_t)b * ldh + (size_t)s * 3 * K;
@@ -390,8 +424,10 @@ int rebop_sg_cast_bf16(size_t n, const float *x, void *y, void *stream) {
 }
 
 int rebop_sg_colsum(int M, int N, const float *X, int ldx, float *out, void *stream) {
-  hipLaunchKernelGGL(sg::colsum_kernel, dim3(nblk((size_t)N, 256)), dim3(256), 0,
-                     (hipStream_t)stream, M, N, X, ldx, out);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(sg::fill_kernel, dim3(nblk((size_t)N, 256)), dim3(256), 0, st, N, out);
+  hipLaunchKernelGGL(sg::colsum_kernel, dim3(nblk((size_t)N, 64), nblk((size_t)M, 256)), dim3(256),
+                     0, st, M, N, X, ldx, out);
   return ret(hipGetLastError());
 }
 

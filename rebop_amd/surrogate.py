@@ -218,7 +218,8 @@ class MLP:
             b = torch.zeros(dims[l + 1], dtype=torch.float64)
             if l == depth - 1:
                 for s in range(S):
-                    b[s * 3 * K + K: s * 3 * K + 2 * K] = math.log(0.5)  # log mean / bound
+                    b[s * 3 * K + K: s * 3 * K + K + K // 2] = math.log(0.5)  # log mean / bound
+                    b[s * 3 * K + K + K // 2: s * 3 * K + 2 * K] = 0.0  # binomial logit
                     b[s * 3 * K + 2 * K: s * 3 * K + 3 * K] = math.log(10.0)  # log dispersion
             self.W.append(w.float().cuda())
             self.b.append(b.float().cuda())
@@ -326,7 +327,7 @@ def train(mlp: MLP, data: SweepData, steps: int, batch: int = 4096, lr: float = 
 
 def predict(mlp: MLP, rates: np.ndarray, x0: np.ndarray, sample_idx: np.ndarray, chunk: int = 1 << 16):
     """Mixture parameters for queries (point rows, sample index 1..32): [Q][S][3][K],
-    and the species' bounds [Q][S] (components K/2.. describe bound - x)."""
+    and the species' bounds [Q][S] (components K/2.. are binomials on the bound)."""
     import torch
 
     rates_t = torch.as_tensor(np.ascontiguousarray(rates, dtype=np.float64), device="cuda")
@@ -369,22 +370,22 @@ def infer_device(mlp: MLP, d_rates, d_x0, d_point, d_sample, B: int):
 
 def mixture_cdf(params: np.ndarray, xmax: int, bound: float | None = None) -> np.ndarray:
     """CDF on 0..xmax of one species' mixture (params [3][K]); with a bound,
-    components K/2.. are negative binomials of bound - x."""
+    components K/2.. are binomial(bound, sigmoid(params[1][c]))."""
     from scipy import stats
 
     logits, lmu, lr = params
     w = np.exp(logits - logits.max())
     w /= w.sum()
-    mu = np.exp(np.clip(lmu, -8, 9))
-    r = np.exp(np.clip(lr, -6, 12))
     xs = np.arange(xmax + 1)
     pmf = np.zeros(xmax + 1)
     for c in range(K):
         if bound is not None and c >= K // 2:
-            y = bound - xs
-            pc = np.where(y >= 0, stats.nbinom.pmf(np.maximum(y, 0), r[c], r[c] / (r[c] + mu[c])), 0.0)
+            p = 1.0 / (1.0 + np.exp(-np.clip(lmu[c], -30, 30)))
+            pc = stats.binom.pmf(xs, int(round(bound)), p)
         else:
-            pc = stats.nbinom.pmf(xs, r[c], r[c] / (r[c] + mu[c]))
+            mu = np.exp(np.clip(lmu[c], -8, 9))
+            r = np.exp(np.clip(lr[c], -6, 12))
+            pc = stats.nbinom.pmf(xs, r, r / (r + mu))
         pmf += w[c] * pc
     return np.minimum(np.cumsum(pmf), 1.0)
 

@@ -90,8 +90,8 @@ def test_mlp_backward_matches_torch_autograd():
 
 
 def test_nll_with_bounds_matches_scipy():
-    """With bounds: log means relative to ln bound, components K/2.. are
-    negative binomials of bound - x (scipy's pmf, written out)."""
+    """With bounds: negative-binomial log means relative to ln bound,
+    components K/2.. binomial(bound, sigmoid(slot)) (scipy's pmfs, written out)."""
     rng = np.random.default_rng(4)
     B, S, J = 3, 4, 12
     head = _random_head(B, S, rng)
@@ -106,11 +106,14 @@ def test_nll_with_bounds_matches_scipy():
             h = head[b, s * 3 * K:(s + 1) * 3 * K]
             w = np.exp(h[:K] - np.logaddexp.reduce(h[:K]))
             mu, r = bound[b, s] * np.exp(h[K:2 * K]), np.exp(h[2 * K:])
+            pbin = 1.0 / (1.0 + np.exp(-h[K:2 * K]))
             for x in samples[b, s]:
                 p = 0.0
                 for c in range(K):
-                    y = x if c < K // 2 else bound[b, s] - x
-                    p += w[c] * stats.nbinom.pmf(y, r[c], r[c] / (r[c] + mu[c]))
+                    if c < K // 2:
+                        p += w[c] * stats.nbinom.pmf(x, r[c], r[c] / (r[c] + mu[c]))
+                    else:
+                        p += w[c] * stats.binom.pmf(x, int(bound[b, s]), pbin[c])
                 expect -= np.log(p)
     assert total == pytest.approx(expect, rel=1e-10)
 
@@ -124,7 +127,8 @@ def test_nll_with_bounds_gradient_matches_finite_differences():
     bound = rng.integers(60, 120, size=(B, S)).astype(float)
     samples = np.minimum(rng.poisson(30, size=(B, S, J)), bound[:, :, None].astype(int))
     _, g = ref.nll(head, samples, K, bound=bound)
-    for idx in [(0, K + 1), (1, K + 6), (0, 2 * K + 6), (1, 3 * K + 2 * K + 5), (0, 3 * 3 * K + 7)]:
+    for idx in [(0, K + 1), (1, K + 6), (0, 2 * K + 2), (1, 3 * K + 2 * K + 1), (0, 3 * 3 * K + 7),
+                (1, 3 * K + K + 5), (0, 3)]:
         h1, h2 = head.copy(), head.copy()
         eps = 1e-6
         h1[idx] += eps

@@ -26,63 +26,74 @@ namespace sg {
 
 // ---------------------------------------------------------------------------
 // C = op(A) op(B), M x N x K, FP32 out.  !TA: A[m][k] (lda); TA: A stored
-// [k][m].  !TB: B[k][n] (ldb); TB: B stored [n][k].  Tiles 64 x 64 x 32, four
-// waves, each 32 x 32 (2 x 2 MFMA tiles).  M, N multiples of 64, K of 32.
-// With gridDim.z > 1 (split K), block z covers k in [z*K/gz, (z+1)*K/gz) and
-// adds its partial product into C (zeroed beforehand) atomically.
-template <bool TA, bool TB>
+// [k][m].  !TB: B[k][n] (ldb); TB: B stored [n][k].  Tiles TM x TN x 32 (TM,
+// TN = 64 or 128), four waves in 2 x 2, each (TM/2) x (TN/2) of 16 x 16 MFMA
+// tiles: a 128 x 128 tile issues 16 MFMAs per wave per 8 fragment loads.
+// M, N multiples of the tile, K of 32.  With gridDim.z > 1 (split K), block z
+// covers k in [z*K/gz, (z+1)*K/gz) and adds its partial product into C
+// (zeroed beforehand) atomically.
+template <bool TA, bool TB, int TM, int TN>
 __global__ void __launch_bounds__(256) gemm_kernel(int M, int N, int K, const __bf16 *__restrict__ A,
                                                    int lda, const __bf16 *__restrict__ B, int ldb,
                                                    float *__restrict__ C, int ldc) {
-  __shared__ __attribute__((aligned(16))) __bf16 As[64][40]; // [m][k], rows padded to 80 B
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[64][40]; // [n][k]
+  constexpr int FM = TM / 32, FN = TN / 32; // 16-row fragments per wave
+  __shared__ __attribute__((aligned(16))) __bf16 As[TM][40]; // [m][k], rows padded to 80 B
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[TN][40]; // [n][k]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  floatx4 acc[2][2];
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const int wm = (wave >> 1) * (TM / 2), wn = (wave & 1) * (TN / 2);
+  floatx4 acc[FM][FN];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < FN; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
   const int ksplit = K / (int)gridDim.z; // a multiple of 32 (checked by the host)
   const int kbeg = (int)blockIdx.z * ksplit, kend = kbeg + ksplit;
   for (int k0 = kbeg; k0 < kend; k0 += 32) {
-    if (!TA) {
-      const int r = tid >> 2, c = (tid & 3) * 8;
-      *(bf16x8 *)&As[r][c] = *(const bf16x8 *)&A[(size_t)(m0 + r) * lda + k0 + c];
-    } else {
-      const int kk = tid >> 3, c = (tid & 7) * 8;
-      const bf16x8 v = *(const bf16x8 *)&A[(size_t)(k0 + kk) * lda + m0 + c];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) As[c + q][kk] = v[q];
+    for (int p = 0; p < TM / 64; ++p) {
+      const int idx = tid + 256 * p;
+      if (!TA) {
+        const int r = idx >> 2, c = (idx & 3) * 8;
+        *(bf16x8 *)&As[r][c] = *(const bf16x8 *)&A[(size_t)(m0 + r) * lda + k0 + c];
+      } else {
+        const int kk = idx / (TM / 8), c = (idx % (TM / 8)) * 8;
+        const bf16x8 v = *(const bf16x8 *)&A[(size_t)(k0 + kk) * lda + m0 + c];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) As[c + q][kk] = v[q];
+      }
     }
-    if (TB) {
-      const int r = tid >> 2, c = (tid & 3) * 8;
-      *(bf16x8 *)&Bs[r][c] = *(const bf16x8 *)&B[(size_t)(n0 + r) * ldb + k0 + c];
-    } else {
-      const int kk = tid >> 3, c = (tid & 7) * 8;
-      const bf16x8 v = *(const bf16x8 *)&B[(size_t)(k0 + kk) * ldb + n0 + c];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) Bs[c + q][kk] = v[q];
+    for (int p = 0; p < TN / 64; ++p) {
+      const int idx = tid + 256 * p;
+      if (TB) {
+        const int r = idx >> 2, c = (idx & 3) * 8;
+        *(bf16x8 *)&Bs[r][c] = *(const bf16x8 *)&B[(size_t)(n0 + r) * ldb + k0 + c];
+      } else {
+        const int kk = idx / (TN / 8), c = (idx % (TN / 8)) * 8;
+        const bf16x8 v = *(const bf16x8 *)&B[(size_t)(k0 + kk) * ldb + n0 + c];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) Bs[c + q][kk] = v[q];
+      }
     }
     __syncthreads();
-    bf16x8 a[2], b[2];
+    bf16x8 a[FM], b[FN];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) a[i] = *(const bf16x8 *)&As[wm + i * 16 + (lane & 15)][(lane >> 4) * 8];
+    for (int i = 0; i < FM; ++i) a[i] = *(const bf16x8 *)&As[wm + i * 16 + (lane & 15)][(lane >> 4) * 8];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) b[j] = *(const bf16x8 *)&Bs[wn + j * 16 + (lane & 15)][(lane >> 4) * 8];
+    for (int j = 0; j < FN; ++j) b[j] = *(const bf16x8 *)&Bs[wn + j * 16 + (lane & 15)][(lane >> 4) * 8];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < FN; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int row = m0 + wm + i * 16 + (lane >> 4) * 4 + q;
@@ -378,6 +389,18 @@ int ret(hipError_t e) {
   return e == hipSuccess ? 0 : 4;
 }
 unsigned nblk(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+template <bool TA, bool TB>
+void launch_gemm(bool big, dim3 grid, hipStream_t s, int M, int N, int K, const __bf16 *a,
+                        int lda, const __bf16 *b, int ldb, float *C, int ldc) {
+  if (big)
+    hipLaunchKernelGGL((sg::gemm_kernel<TA, TB, 128, 128>), grid, dim3(256), 0, s, M, N, K, a, lda, b,
+                       ldb, C, ldc);
+  else
+    hipLaunchKernelGGL((sg::gemm_kernel<TA, TB, 64, 64>), grid, dim3(256), 0, s, M, N, K, a, lda, b,
+                       ldb, C, ldc);
+}
+
 } // namespace
 
 extern "C" {
@@ -387,19 +410,23 @@ const char *rebop_sg_last_error(void) { return hipGetErrorString(g_err); }
 int rebop_sg_gemm_bf16(int ta, int tb, int M, int N, int K, const void *A, int lda, const void *B,
                        int ldb, float *C, int ldc, void *stream) {
   if (M % 64 || N % 64 || K % 32 || M <= 0 || N <= 0 || K <= 0) return 3;
-  // split K when the output has too few 64 x 64 tiles to fill the chip
+  // 128 x 128 tiles when there are enough of them to fill the chip
+  const bool big = M % 128 == 0 && N % 128 == 0 && (M / 128) * (N / 128) >= 512;
+  const int tile = big ? 128 : 64;
+  // split K when the output has too few tiles to fill the chip
   int splits = 1;
-  while ((M / 64) * (N / 64) * splits < 512 && K % (64 * splits) == 0 && K / (32 * splits) >= 8)
+  while (!big && (M / 64) * (N / 64) * splits < 512 && K % (64 * splits) == 0 &&
+         K / (32 * splits) >= 8)
     splits *= 2;
-  const dim3 grid(N / 64, M / 64, splits);
+  const dim3 grid(N / tile, M / tile, splits);
   const __bf16 *a = (const __bf16 *)A, *b = (const __bf16 *)B;
   hipStream_t s = (hipStream_t)stream;
   if (splits > 1)
     hipLaunchKernelGGL(sg::zero_kernel, dim3(nblk((size_t)M * N, 256)), dim3(256), 0, s, M, N, C, ldc);
-  if (!ta && !tb) hipLaunchKernelGGL((sg::gemm_kernel<false, false>), grid, dim3(256), 0, s, M, N, K, a, lda, b, ldb, C, ldc);
-  else if (!ta && tb) hipLaunchKernelGGL((sg::gemm_kernel<false, true>), grid, dim3(256), 0, s, M, N, K, a, lda, b, ldb, C, ldc);
-  else if (ta && !tb) hipLaunchKernelGGL((sg::gemm_kernel<true, false>), grid, dim3(256), 0, s, M, N, K, a, lda, b, ldb, C, ldc);
-  else hipLaunchKernelGGL((sg::gemm_kernel<true, true>), grid, dim3(256), 0, s, M, N, K, a, lda, b, ldb, C, ldc);
+  if (!ta && !tb) launch_gemm<false, false>(big, grid, s, M, N, K, a, lda, b, ldb, C, ldc);
+  else if (!ta && tb) launch_gemm<false, true>(big, grid, s, M, N, K, a, lda, b, ldb, C, ldc);
+  else if (ta && !tb) launch_gemm<true, false>(big, grid, s, M, N, K, a, lda, b, ldb, C, ldc);
+  else launch_gemm<true, true>(big, grid, s, M, N, K, a, lda, b, ldb, C, ldc);
   return ret(hipGetLastError());
 }
 

@@ -137,3 +137,40 @@ def test_fused_heterogeneous_jobs_bit_exact(nthreads):
         assert np.array_equal(outs[i].cpu().numpy(), oout), i
         assert int(ev[i]) == oev, i
     assert int(ev.sum()) == st.events
+
+
+def test_lds_records_that_do_not_fit(nthreads):
+    """Reactions whose factors (third order: three factors) or jump (five
+    species) do not fit the 32-byte LDS record take the global lists; the
+    result stays bit-exact, in a network that mixes both kinds."""
+    S = 6
+    rx = [
+        (0.002, [3, 0, 0, 0, 0, 0], [-3, 1, 0, 0, 0, 0]),  # 3A -> B: three factors
+        (0.5, [0, 1, 0, 0, 0, 0], [1, -1, 1, 1, 1, 1]),    # B -> A + C + D + E + F: six jump entries
+        (0.3, [0, 0, 1, 0, 0, 0], [0, 0, -1, 0, 0, 0]),
+        (0.3, [0, 0, 0, 1, 0, 0], [0, 0, 0, -1, 0, 0]),
+        (0.2, [0, 0, 0, 0, 1, 1], [2, 0, 0, 0, -1, -1]),  # E + F -> 2A
+        (5.0, [0, 0, 0, 0, 0, 0], [1, 0, 0, 0, 0, 0]),
+    ]
+    spec = (S, rx, [40, 5, 0, 0, 0, 0])
+    g = build_device(spec, 21)
+    out, st = _dev(g, 4.0, 20, 300, LDS)
+    o = build_oracle(spec)
+    _, oout, _, oev, _ = o.run_grid(21, spec[2], 4.0, 20, n_traj=300, nthreads=nthreads)
+    assert np.array_equal(out, oout)
+    assert st.events == oev and oev > 300 * 50
+
+
+def test_autotune_picks_a_variant_that_stays_bit_exact(nthreads):
+    """rebop_autotune times the applicable variants; the AUTO runs that follow
+    use the winner and still equal the oracle bit for bit."""
+    spec = dimers()
+    g = build_device(spec, 31)
+    res = g.autotune(0.5, 50, n_pilot=4096)
+    assert res.n_tried >= 8 and res.ms > 0
+    assert res.threads in (64, 128, 256) and 0 < res.occupancy <= 1.0
+    out, st = _dev(g, 0.5, 50, 4096, _native.REBOP_KERNEL_AUTO)
+    assert st.kernel == res.kernel
+    o = build_oracle(spec)
+    _, oout, _, oev, _ = o.run_grid(31, spec[2], 0.5, 50, n_traj=4096, nthreads=nthreads)
+    assert np.array_equal(out, oout) and st.events == oev

@@ -23,13 +23,13 @@ def _bf(x):
     return torch.as_tensor(x, dtype=torch.float32, device="cuda").to(torch.bfloat16)
 
 
-@pytest.mark.parametrize("M,N,Kd", [(128, 192, 96), (64, 128, 2048)])
+@pytest.mark.parametrize("M,N,Kd", [(128, 192, 96), (64, 128, 2048), (4096, 2048, 64)])
 @pytest.mark.parametrize("ta,tb", [(False, False), (False, True), (True, False), (True, True)])
 def test_gemm_bf16_matches_fp64(sg, ta, tb, M, N, Kd):
     """op(A) op(B) on the matrix cores equals the FP64 product of the same
     BF16 values to FP32-accumulation accuracy (the lane layout of
     v_mfma_f32_16x16x32_bf16 is checked here; the second shape has too few
-    output tiles and runs split over K)."""
+    output tiles and runs split over K, the third runs on 128 x 128 tiles)."""
     import torch
 
     from oracle.surrogate_ref import bf16

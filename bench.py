@@ -331,7 +331,7 @@ def main() -> int:
                     help="dimer (the headline configuration), sir, syn (128x512), het (4096 "
                          "networks), surrogate (Michaelis-Menten sweep -> MLP -> W1 gate)")
     ap.add_argument("--surrogate-points", type=int, default=100_000)
-    ap.add_argument("--surrogate-train-steps", type=int, default=1500)
+    ap.add_argument("--surrogate-train-steps", type=int, default=12_000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one device copy of 1 GiB (read 1 GiB, write 1 GiB) and exit; "
                          "scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")

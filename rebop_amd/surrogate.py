@@ -449,8 +449,9 @@ def validate(mlp: MLP, rates: np.ndarray, x0: np.ndarray, n_ref: int, seed: int,
     return rel, TrustMap(bins, passed), data.stats
 
 
-def benchmark(n_train: int = 100_000, steps: int = 1500, batch: int = 4096, n_queries: int = 1_000_000,
-              n_valid: int = 128, n_ref: int = 10_000, seed: int = 2026, log=print) -> dict:
+def benchmark(n_train: int = 100_000, steps: int = 12_000, batch: int = 8192, n_queries: int = 1_000_000,
+              n_valid: int = 128, n_ref: int = 10_000, seed: int = 2026, lr: float = 1e-3,
+              log=print) -> dict:
     """End-to-end: SSA training data -> training -> W1 validation -> query timing.
 
     Returns a dict with the surrogate's and the SSA's queries/s (a query is
@@ -470,7 +471,7 @@ def benchmark(n_train: int = 100_000, steps: int = 1500, batch: int = 4096, n_qu
     log(f"training data: {n_train} points x {J} trajectories, {data.stats.events} events, {gen_s:.2f} s")
     mlp = MLP(seed=seed)
     t0 = time.perf_counter()
-    hist = train(mlp, data, steps=steps, batch=batch, lr=2e-3, seed=seed, log_every=max(1, steps // 10),
+    hist = train(mlp, data, steps=steps, batch=batch, lr=lr, seed=seed, log_every=max(1, steps // 10),
                  log=log)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0

@@ -242,6 +242,7 @@ int upload_net(rebop_gillespie *g) {
   const uint32_t J = (uint32_t)js.size();
   // LDS-kernel reaction records (layout in ssa_layout.h)
   std::vector<uint32_t> rec((size_t)REBOP_REC_WORDS * R, 0u);
+  std::vector<uint32_t> jrec((size_t)REBOP_JREC_WORDS * R, 0u);
   bool any_slow = false;
   for (uint32_t r = 0; r < R; ++r) {
     uint32_t *w = rec.data() + (size_t)REBOP_REC_WORDS * r;
@@ -265,9 +266,10 @@ int upload_net(rebop_gillespie *g) {
     if (nj > 4) {
       w[3] |= REBOP_REC_SLOW_JUMP;
     } else {
+      uint32_t *jw = jrec.data() + (size_t)REBOP_JREC_WORDS * r;
       for (uint32_t q = 0; q < nj; ++q) {
         const uint32_t e = jb[r] + q;
-        w[4 + q] = (js[e] & 0xffu) | ((uint32_t)jd[e] << 8);
+        jw[q] = (js[e] & 0xffu) | ((uint32_t)jd[e] << 8);
       }
     }
   }
@@ -281,12 +283,14 @@ int upload_net(rebop_gillespie *g) {
   const size_t ojs = ojb + al(sizeof(uint32_t) * (R + 1));
   const size_t ojd = ojs + al(sizeof(uint32_t) * J);
   const size_t orc = ojd + al(sizeof(int32_t) * J);
-  const size_t total = orc + al(sizeof(uint32_t) * rec.size());
+  const size_t ojr = orc + al(sizeof(uint32_t) * rec.size());
+  const size_t total = ojr + al(sizeof(uint32_t) * jrec.size());
   std::vector<unsigned char> host(total, 0);
   std::memcpy(host.data() + ojb, jb.data(), sizeof(uint32_t) * (R + 1));
   if (J) std::memcpy(host.data() + ojs, js.data(), sizeof(uint32_t) * J);
   if (J) std::memcpy(host.data() + ojd, jd.data(), sizeof(int32_t) * J);
   if (R) std::memcpy(host.data() + orc, rec.data(), sizeof(uint32_t) * rec.size());
+  if (R) std::memcpy(host.data() + ojr, jrec.data(), sizeof(uint32_t) * jrec.size());
   if (R) std::memcpy(host.data() + ok, k.data(), sizeof(double) * R);
   std::memcpy(host.data() + ofb, fb.data(), sizeof(uint32_t) * (R + 1));
   if (F) std::memcpy(host.data() + ofs, fs.data(), sizeof(uint32_t) * F);
@@ -322,6 +326,7 @@ int upload_net(rebop_gillespie *g) {
   jt.jump_s = (const uint32_t *)(base + ojs);
   jt.jump_d = (const int32_t *)(base + ojd);
   jt.rec = (const uint32_t *)(base + orc);
+  jt.jrec = (const uint32_t *)(base + ojr);
   g->job = jt;
   g->dirty = false;
   g->dev_id = cur;

@@ -95,19 +95,21 @@ typedef struct {
 #define REBOP_LDS_LIMIT 1073741824 /* |count| <= 2^30 in the int32 LDS kernel */
 
 /*
- * Reaction records of the LDS kernel: 8 words per reaction, copied into LDS
- * when a wave starts an item of the job.
+ * Reaction records of the LDS kernel, 4 words per reaction, copied into LDS
+ * when a wave starts an item of the job (read R times per event):
  *   w0, w1  rate constant k (f64 bits, low word first)
  *   w2, w3  the first two propensity factors (x[s] - c) in Rate::rate order:
  *           s in bits 0-15, c in bits 16-27, bit 31 = present
  *   w3      bit 30: the factors are not all here (more than two, or c >=
  *           4096): use fac_begin/fac_s/fac_c for this reaction
- *           bit 29: the jump is not all here (more than four entries): use
- *           jump_begin/jump_s/jump_d
- *   w4..w7  up to four jump entries: species in bits 0-7, signed difference
- *           in bits 8-31 (zero difference = unused)
+ *           bit 29: the jump is not all in the jump record (more than four
+ *           entries): use jump_begin/jump_s/jump_d
+ * and jump records, 4 words per reaction, read from global memory once per
+ * event: up to four entries, species in bits 0-7, signed difference in bits
+ * 8-31 (zero difference = unused).
  */
-#define REBOP_REC_WORDS 8u
+#define REBOP_REC_WORDS 4u
+#define REBOP_JREC_WORDS 4u
 #define REBOP_REC_PRESENT 0x80000000u
 #define REBOP_REC_SLOW_FACTORS 0x40000000u
 #define REBOP_REC_SLOW_JUMP 0x20000000u
@@ -122,7 +124,8 @@ typedef struct {
   const uint32_t *jump_begin; /* [R+1] */
   const uint32_t *jump_s;
   const int32_t *jump_d;
-  const uint32_t *rec; /* [R][REBOP_REC_WORDS] reaction records */
+  const uint32_t *rec;  /* [R][REBOP_REC_WORDS] reaction records (copied to LDS) */
+  const uint32_t *jrec; /* [R][REBOP_JREC_WORDS] jump records */
   /* the run of this job (fields as rebop_dev_run) */
   uint64_t seed;
   uint64_t traj_begin;

@@ -19,9 +19,9 @@
 // uses the full linear count instead, which is the reference's definition.
 //
 // The network is read from LDS too: when a wave starts an item it copies the
-// job's reaction records (rate constant, the first two propensity factors,
-// up to four jump entries; 32 bytes each, see ssa_layout.h) next to the
-// counts.  A make_cumrates pass then costs one uniform 16-byte LDS read and
+// job's reaction records (rate constant and the first two propensity factors,
+// 16 bytes each, see ssa_layout.h) next to the counts; the jump records (up
+// to four entries, 16 bytes) stay in global memory, read once per event.  A make_cumrates pass then costs one uniform 16-byte LDS read and
 // two count reads per reaction, independent from one reaction to the next
 // (the loop is unrolled so the reads of later reactions overlap the
 // arithmetic of earlier ones); a missing factor multiplies by exactly 1.0.
@@ -58,8 +58,9 @@ struct LdsNet {
   // this lane's counts
   int32_t *xl;
   uint32_t BL;
-  // the job's reaction records in LDS
+  // the job's reaction records in LDS, jump records in global memory
   const uint32_t *rec;
+  const uint32_t *jrec;
   // per-lane flags
   bool linear;
   bool bad;
@@ -207,13 +208,12 @@ struct LdsNet {
 
   // Jump::affect (src/gillespie.rs:142-152), sparse form
   __device__ __forceinline__ void affect(double (&)[MS], uint32_t kk) {
-    const uint32_t *w = rec + (size_t)kk * REBOP_REC_WORDS;
-    if (w[3] & REBOP_REC_SLOW_JUMP) {
+    const uint4 j = *(const uint4 *)(jrec + (size_t)kk * REBOP_JREC_WORDS);
+    if (rec[(size_t)kk * REBOP_REC_WORDS + 3] & REBOP_REC_SLOW_JUMP) {
       const uint32_t e1 = jb[kk + 1];
       for (uint32_t e = jb[kk]; e < e1; ++e) add_count(js[e], jd[e]);
       return;
     }
-    const uint4 j = *(const uint4 *)(w + 4);
     const uint32_t e[4] = {j.x, j.y, j.z, j.w};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -272,6 +272,7 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
     net.xl = lds_x + threadIdx.x;
     net.BL = blockDim.x;
     net.rec = lrec;
+    net.jrec = job.jrec;
     net.linear = net.job_linear;
     net.bad = false;
 

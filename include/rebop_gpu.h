@@ -135,6 +135,35 @@ int rebop_gillespie_add_reaction_expr(rebop_gillespie *g, const uint32_t *code, 
                                       const double *consts, uint32_t n_consts,
                                       const int64_t *differences, uint32_t n_differences);
 
+/*
+ * define_system! networks -- src/gillespie_macro.rs:49-170.  The macro DSL's
+ * arithmetic differs from Gillespie's: a reaction's propensity is
+ * `rate * lma(m_1 r_1) * lma(m_2 r_2) * ...` over its reactant terms in written
+ * order, where lma(m x) = x (x-1) ... (x-m+1) is an integer product (isize,
+ * wrapping; m = 0 gives x, like the macro's loop) converted to f64 once
+ * (_rate_lma!, :128-142), and the choice is the first reaction r with
+ * `chosen < cum_r` (_choice!, :146-170; none: no reaction, time still
+ * advances).  Draws and the time update are those of advance_until.
+ *
+ * rebop_system_new makes such a handle (rebop_gillespie_add_reaction_* then
+ * return REBOP_ERR_UNSUPPORTED); every run entry point, batch and parameter
+ * sweep accepts it and uses the network-specialised kernels (S <= 32,
+ * R <= 64); the LDS kernel and fused job batches return REBOP_ERR_UNSUPPORTED.
+ */
+#define REBOP_SYSTEM_MAX_COUNT 64u /* largest stoichiometric coefficient */
+int rebop_system_new(const int64_t *species, uint32_t nb_species, uint64_t seed,
+                     rebop_gillespie **out);
+/* One reaction `m_1 r_1 + m_2 r_2 + ... => p_1 q_1 + ... @ rate`: reactants[i]
+ * with coefficient reactant_counts[i] (repeats allowed: `A + A` is lma(1 A)^2,
+ * not lma(2 A)), products likewise.  The rate is the value of the macro's
+ * rate expression (finite, or NaN for a parameter never set: the system then
+ * never reacts).  REBOP_ERR_INDEX for a species >= nb_species, REBOP_ERR_VALUE
+ * for a coefficient above REBOP_SYSTEM_MAX_COUNT or a net change beyond 2^20. */
+int rebop_system_add_reaction(rebop_gillespie *g, double rate, const uint32_t *reactants,
+                              const uint32_t *reactant_counts, uint32_t n_reactants,
+                              const uint32_t *products, const uint32_t *product_counts,
+                              uint32_t n_products);
+
 /* Compiles and caches the kernel `kernel` (REBOP_KERNEL_*) will use for this
  * network, so that the first run does not pay for it.  Without a device it
  * only compiles, which checks the specialised kernel for gfx950.  No
@@ -275,6 +304,12 @@ int rebop_batch_set_species(rebop_batch *b, const int64_t *species);
 /* Gillespie::get_time / set_time (src/gillespie.rs:246-252) for every trajectory. */
 int rebop_batch_get_time(const rebop_batch *b, double *t);
 int rebop_batch_set_time(rebop_batch *b, const double *t);
+/* The position of every trajectory's random stream (draws made so far; the
+ * counter of its Philox stream).  With set_time and set_species it moves a
+ * trajectory to another batch unchanged, as cloning a Gillespie (rng
+ * included) does in the reference; no separate counterpart there. */
+int rebop_batch_get_draws(const rebop_batch *b, uint64_t *draws);
+int rebop_batch_set_draws(rebop_batch *b, const uint64_t *draws);
 
 /* Description of the last error on this thread ("" if none). */
 const char *rebop_last_error(void);

@@ -28,8 +28,11 @@
 struct orc_net {
   uint32_t S, R, cap;
   int sparse;         /* Gillespie.sparse (gillespie.rs:162) */
+  int system;         /* define_system! semantics (gillespie_macro.rs:49-125) */
   /* Rate: LMA (dense, rate_sparse=0) holds S orders at rate_off[r]..+S;
-   *       LMASparse holds (idx, order) pairs rate_off[r]..rate_off[r+1]. */
+   *       LMASparse holds (idx, order) pairs rate_off[r]..rate_off[r+1];
+   *       a define_system! reaction (rate_sparse=3) holds its reactant terms
+   *       (species, multiplicity) in written order at the same place. */
   int *rate_sparse;
   double *k;
   uint32_t *rate_off; /* R+1 */
@@ -61,6 +64,12 @@ orc_net *orc_net_new(uint32_t nb_species, int sparse) {
     orc_net_free(n);
     return NULL;
   }
+  return n;
+}
+
+orc_net *orc_net_new_system(uint32_t nb_species) {
+  orc_net *n = orc_net_new(nb_species, 1);
+  if (n) n->system = 1;
   return n;
 }
 
@@ -151,6 +160,7 @@ static int push_jump(orc_net *n, uint32_t r, const int64_t *diffs, uint32_t ndif
 int orc_net_add_reaction(orc_net *n, int rate_sparse, double k, const uint32_t *idx,
                          const uint32_t *orders, uint32_t cnt, const int64_t *diffs,
                          uint32_t ndiffs) {
+  if (n->system) return ORC_ERR_VALUE;
   /* gillespie.rs:227-235: the rate must operate on existing species */
   if (!rate_sparse) {
     if (cnt != n->S) return ORC_ERR_SHAPE;
@@ -199,6 +209,7 @@ int orc_net_add_reaction(orc_net *n, int rate_sparse, double k, const uint32_t *
 int orc_net_add_reaction_expr(orc_net *n, const uint32_t *code, uint32_t n_code,
                               const double *consts, uint32_t n_consts, const int64_t *diffs,
                               uint32_t ndiffs) {
+  if (n->system) return ORC_ERR_VALUE;
   if (ndiffs != n->S) return ORC_ERR_SHAPE;
   if (n_code == 0 || n_code > REBOP_EXPR_MAX_CODE) return ORC_ERR_VALUE;
   uint32_t depth = 0;
@@ -246,6 +257,52 @@ int orc_net_add_reaction_expr(orc_net *n, const uint32_t *code, uint32_t n_code,
   return ORC_OK;
 }
 
+/* One define_system! reaction (gillespie_macro.rs:55-58, 106-112, 146-170):
+ * `rate * lma(n_1 r_1) * lma(n_2 r_2) ...` over the reactant terms in written
+ * order, then `r_i -= n_i` for every reactant term and `p_j += m_j` for every
+ * product term; the jump is stored as the net difference per species (the
+ * reference's separate integer decrements and increments give the same
+ * counts).  Multiplicity 0 is accepted as the macro accepts the literal 0. */
+int orc_net_add_reaction_system(orc_net *n, double k, const uint32_t *r_idx,
+                                const uint32_t *r_mult, uint32_t n_r, const uint32_t *p_idx,
+                                const uint32_t *p_mult, uint32_t n_p) {
+  if (!n->system) return ORC_ERR_VALUE;
+  for (uint32_t i = 0; i < n_r; ++i)
+    if (r_idx[i] >= n->S) return ORC_ERR_INDEX;
+  for (uint32_t i = 0; i < n_p; ++i)
+    if (p_idx[i] >= n->S) return ORC_ERR_INDEX;
+  const uint32_t r = n->R;
+  if (r + 2 > n->cap && reserve_reaction(n, r)) return ORC_ERR_ALLOC;
+  int64_t *d = (int64_t *)calloc(n->S ? n->S : 1, sizeof(int64_t));
+  if (!d) return ORC_ERR_ALLOC;
+  for (uint32_t i = 0; i < n_r; ++i) d[r_idx[i]] -= (int64_t)r_mult[i];
+  for (uint32_t i = 0; i < n_p; ++i) d[p_idx[i]] += (int64_t)p_mult[i];
+  n->k[r] = k;
+  n->rate_sparse[r] = 3;
+  for (uint32_t i = 0; i < n_r; ++i)
+    if (push_rate_entry(n, r_idx[i], r_mult[i])) {
+      free(d);
+      return ORC_ERR_ALLOC;
+    }
+  n->rate_off[r + 1] = n->rate_len;
+  n->expr_off[r + 1] = n->expr_len;
+  int rc = push_jump(n, r, d, n->S);
+  free(d);
+  if (rc) return rc;
+  n->R = r + 1;
+  return ORC_OK;
+}
+
+/* _rate_lma! (gillespie_macro.rs:128-142): x (x-1) ... (x-m+1) in isize --
+ * the loop `for i in 1..m { rate *= x - i }` starts from x, so m = 0 and
+ * m = 1 both give x -- with release-build wrapping on overflow, then one
+ * conversion to f64. */
+static double system_lma(int64_t x, uint32_t m) {
+  uint64_t p = (uint64_t)x;
+  for (uint32_t i = 1; i < m; ++i) p = p * (uint64_t)(x - (int64_t)i);
+  return (double)(int64_t)p;
+}
+
 /* Expr::eval (expr.rs:24-38) over the postfix program: counts enter as f64. */
 static double eval_expr(const orc_net *net, uint32_t r, const int64_t *x) {
   double st[REBOP_EXPR_MAX_DEPTH];
@@ -284,6 +341,13 @@ static double eval_expr(const orc_net *net, uint32_t r, const int64_t *x) {
  * the dense form stores every species in ascending order. */
 double orc_rate(const orc_net *net, uint32_t r, const int64_t *x) {
   if (net->rate_sparse[r] == 2) return eval_expr(net, r, x);
+  if (net->rate_sparse[r] == 3) {
+    /* gillespie_macro.rs:108: `$rate * lma(..) * lma(..)`, left to right */
+    double a = net->k[r];
+    for (uint32_t i = net->rate_off[r]; i < net->rate_off[r + 1]; ++i)
+      a = a * system_lma(x[net->rate_idx[i]], net->rate_ord[i]);
+    return a;
+  }
   double acc = net->k[r];
   for (uint32_t i = net->rate_off[r]; i < net->rate_off[r + 1]; ++i) {
     const int64_t n = x[net->rate_idx[i]];
@@ -293,8 +357,10 @@ double orc_rate(const orc_net *net, uint32_t r, const int64_t *x) {
   return acc;
 }
 
-/* Jump::affect (gillespie.rs:142-152) */
+/* Jump::affect (gillespie.rs:142-152); r == R (a define_system! choice that
+ * matched no reaction) changes nothing */
 void orc_affect(const orc_net *net, uint32_t r, int64_t *x) {
+  if (r >= net->R) return;
   for (uint32_t i = net->jump_off[r]; i < net->jump_off[r + 1]; ++i)
     x[net->jump_idx[i]] += net->jump_d[i];
 }
@@ -314,6 +380,20 @@ uint64_t orc_choose_cumrate_sum(double chosen, const double *cum, uint64_t n) {
   uint64_t k = 0;
   for (uint64_t r = 0; r < n; ++r) k += (cum[r] < chosen) ? 1u : 0u;
   return k;
+}
+
+/* _choice! (gillespie_macro.rs:146-170): the first r with chosen < carry +
+ * rate_r, where carry is `0. + rate_0 + ... + rate_{r-1}` -- the same sum as
+ * cum[r] -- and n (no reaction applied) when no r qualifies. */
+uint64_t orc_choose_first_above(double chosen, const double *cum, uint64_t n) {
+  for (uint64_t r = 0; r < n; ++r)
+    if (chosen < cum[r]) return r;
+  return n;
+}
+
+static inline uint64_t choose(const orc_net *net, double chosen, const double *cum) {
+  return net->system ? orc_choose_first_above(chosen, cum, net->R)
+                     : orc_choose_cumrate_sum(chosen, cum, net->R);
 }
 
 /* ------------------------------------------------------------------ */
@@ -367,7 +447,9 @@ static inline double draw_exp1(orc_state *st, double *u_out) {
   return -log((double)((a >> 11) + 1) * 0x1p-53);
 }
 
-/* Gillespie::advance_until (gillespie.rs:315-344) */
+/* Gillespie::advance_until (gillespie.rs:315-344); for a define_system! net
+ * the macro's advance_until (gillespie_macro.rs:99-124), which differs only in
+ * the propensity products (orc_rate) and the choice (choose) */
 void orc_advance_until(const orc_net *net, orc_state *st, double tmax, double *cum) {
   for (;;) {
     const double total = orc_make_cumrates(net, st->x, cum);
@@ -383,7 +465,7 @@ void orc_advance_until(const orc_net *net, orc_state *st, double tmax, double *c
       return;
     }
     const double chosen = total * u; /* :332 */
-    const uint64_t k = orc_choose_cumrate_sum(chosen, cum, net->R);
+    const uint64_t k = choose(net, chosen, cum);
     orc_affect(net, (uint32_t)k, st->x); /* :342 */
     st->events++;
   }
@@ -400,7 +482,7 @@ void orc_advance_one_reaction(const orc_net *net, orc_state *st, double *cum) {
   const double e = draw_exp1(st, &u);
   st->t = st->t + e / total;
   const double chosen = total * u;
-  const uint64_t k = orc_choose_cumrate_sum(chosen, cum, net->R);
+  const uint64_t k = choose(net, chosen, cum);
   orc_affect(net, (uint32_t)k, st->x);
   st->events++;
 }

@@ -19,6 +19,8 @@
  *   _advance_one_reaction src/gillespie.rs:275-297  orc_advance_one_reaction()
  *   Gillespie.run grid    src/pyo3_gillespie.rs:200-208  orc_run_grid()
  *   Gillespie.run events  src/pyo3_gillespie.rs:209-223  orc_run_all_events()
+ *   define_system!        src/gillespie_macro.rs:49-170 orc_net_new_system(),
+ *                         orc_net_add_reaction_system(), orc_choose_first_above()
  * with the random source swapped for the Philox stream of rebop_rng.h
  * (rng kind ORC_RNG_PHILOX) or for an independent xoshiro256++ + libm log
  * stream (ORC_RNG_XOSHIRO) used for two-sample statistical tests and as the
@@ -40,6 +42,9 @@ typedef struct orc_net orc_net;
 
 /* Gillespie::new(species, sparse) without the species vector (x0 is given per run). */
 orc_net *orc_net_new(uint32_t nb_species, int sparse);
+/* A define_system! network (gillespie_macro.rs:49-125): integer mass-action
+ * products and the first-match choice; takes orc_net_add_reaction_system only. */
+orc_net *orc_net_new_system(uint32_t nb_species);
 void orc_net_free(orc_net *net);
 uint32_t orc_net_nb_species(const orc_net *net);
 uint32_t orc_net_nb_reactions(const orc_net *net);
@@ -66,13 +71,25 @@ int orc_net_add_reaction_expr(orc_net *net, const uint32_t *code, uint32_t n_cod
                               const double *consts, uint32_t n_consts, const int64_t *diffs,
                               uint32_t ndiffs);
 
+/*
+ * One define_system! reaction: `rate * lma(m_1 r_1) * ...` over the reactant
+ * terms (r_idx, r_mult) in written order (repeats allowed), and the jump
+ * r_i -= m_i, p_j += m_j.  ORC_ERR_VALUE on a net not made by
+ * orc_net_new_system (whose other add_reaction calls return ORC_ERR_VALUE too).
+ */
+int orc_net_add_reaction_system(orc_net *net, double k, const uint32_t *r_idx,
+                                const uint32_t *r_mult, uint32_t n_r, const uint32_t *p_idx,
+                                const uint32_t *p_mult, uint32_t n_p);
+
 /* Rate::rate of reaction r at counts x (for the known-answer tables). */
 double orc_rate(const orc_net *net, uint32_t r, const int64_t *x);
 /* make_cumrates: fills cum[R], returns the total. */
 double orc_make_cumrates(const orc_net *net, const int64_t *x, double *cum);
 /* choose_cumrate_sum over cum[0..n). */
 uint64_t orc_choose_cumrate_sum(double chosen, const double *cum, uint64_t n);
-/* Jump::affect of reaction r on x. */
+/* _choice!: the first r < n with chosen < cum[r], else n. */
+uint64_t orc_choose_first_above(double chosen, const double *cum, uint64_t n);
+/* Jump::affect of reaction r on x (r >= R: no change). */
 void orc_affect(const orc_net *net, uint32_t r, int64_t *x);
 
 /* One trajectory's resumable state (Gillespie's species, t and rng). */

@@ -40,6 +40,12 @@ def load() -> ctypes.CDLL:
     P = POINTER
     sig = {
         "orc_net_new": (c_void_p, [c_uint32, c_int]),
+        "orc_net_new_system": (c_void_p, [c_uint32]),
+        "orc_net_add_reaction_system": (
+            c_int,
+            [c_void_p, c_double, P(c_uint32), P(c_uint32), c_uint32, P(c_uint32), P(c_uint32), c_uint32],
+        ),
+        "orc_choose_first_above": (c_uint64, [c_double, P(c_double), c_uint64]),
         "orc_net_free": (None, [c_void_p]),
         "orc_net_nb_species": (c_uint32, [c_void_p]),
         "orc_net_nb_reactions": (c_uint32, [c_void_p]),
@@ -113,10 +119,29 @@ class OracleError(RuntimeError):
 class OracleNet:
     """A reaction network in the oracle (Gillespie's reactions + sparse flag)."""
 
-    def __init__(self, nb_species: int, sparse: bool = False):
+    def __init__(self, nb_species: int, sparse: bool = False, system: bool = False):
         self.lib = load()
-        self.h = self.lib.orc_net_new(int(nb_species), int(bool(sparse)))
+        if system:
+            self.h = self.lib.orc_net_new_system(int(nb_species))
+        else:
+            self.h = self.lib.orc_net_new(int(nb_species), int(bool(sparse)))
         self.S = int(nb_species)
+
+    @classmethod
+    def system(cls, nb_species: int) -> "OracleNet":
+        """A define_system! network (integer mass-action products, first-match choice)."""
+        return cls(nb_species, system=True)
+
+    def add_system(self, k: float, reactants, products) -> None:
+        """define_system! reaction: reactants/products as [(species, multiplicity)] in written order."""
+        ri = np.ascontiguousarray([p[0] for p in reactants], dtype=np.uint32)
+        rm = np.ascontiguousarray([p[1] for p in reactants], dtype=np.uint32)
+        pi = np.ascontiguousarray([p[0] for p in products], dtype=np.uint32)
+        pm = np.ascontiguousarray([p[1] for p in products], dtype=np.uint32)
+        rc = self.lib.orc_net_add_reaction_system(self.h, float(k), _p(ri, c_uint32), _p(rm, c_uint32), len(ri),
+                                                  _p(pi, c_uint32), _p(pm, c_uint32), len(pi))
+        if rc:
+            raise OracleError(rc)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -262,6 +287,11 @@ def fmin64(a: float, b: float) -> float:
 def choose_cumrate_sum(chosen: float, cum) -> int:
     c = np.ascontiguousarray(cum, dtype=np.float64)
     return int(load().orc_choose_cumrate_sum(float(chosen), _p(c, c_double), len(c)))
+
+
+def choose_first_above(chosen: float, cum) -> int:
+    c = np.ascontiguousarray(cum, dtype=np.float64)
+    return int(load().orc_choose_first_above(float(chosen), _p(c, c_double), len(c)))
 
 
 def hw_threads() -> int:

@@ -44,6 +44,8 @@ REBOP_EXPR_MAX = 9
 REBOP_EXPR_MIN = 10
 REBOP_EXPR_EXP = 11
 
+REBOP_SYSTEM_MAX_COUNT = 64  # largest define_system stoichiometric coefficient
+
 # every entry point of include/rebop_gpu.h: name -> (restype, argtypes)
 SIGNATURES: dict[str, tuple] = {
     "rebop_gillespie_new": (c_int, [POINTER(c_int64), c_uint32, c_int, POINTER(c_void_p)]),
@@ -98,6 +100,14 @@ SIGNATURES: dict[str, tuple] = {
     "rebop_batch_set_species": (c_int, [c_void_p, POINTER(c_int64)]),
     "rebop_batch_get_time": (c_int, [c_void_p, POINTER(c_double)]),
     "rebop_batch_set_time": (c_int, [c_void_p, POINTER(c_double)]),
+    "rebop_batch_get_draws": (c_int, [c_void_p, POINTER(c_uint64)]),
+    "rebop_batch_set_draws": (c_int, [c_void_p, POINTER(c_uint64)]),
+    "rebop_system_new": (c_int, [POINTER(c_int64), c_uint32, c_uint64, POINTER(c_void_p)]),
+    "rebop_system_add_reaction": (
+        c_int,
+        [c_void_p, c_double, POINTER(c_uint32), POINTER(c_uint32), c_uint32,
+         POINTER(c_uint32), POINTER(c_uint32), c_uint32],
+    ),
     "rebop_last_error": (c_char_p, []),
     "rebop_device_info": (c_int, [c_char_p, c_size_t, POINTER(c_int)]),
 }

@@ -22,6 +22,8 @@ struct JitReaction {
   // Rate::Expr: postfix tokens (rebop_expr.h) and their constants; empty for LMA
   std::vector<uint32_t> expr;
   std::vector<double> consts;
+  // define_system reaction: reactant terms (species, multiplicity) in written order
+  std::vector<std::pair<uint32_t, uint32_t>> terms;
 };
 
 struct JitKernel {
@@ -32,8 +34,10 @@ struct JitKernel {
   hipFunction_t fn_g4 = nullptr; // 4 lanes per trajectory
 };
 
-// Kernel source for a network of S species and the given reactions.
-std::string jit_source(uint32_t S, const std::vector<JitReaction> &rx);
+// Kernel source for a network of S species and the given reactions; with
+// `system`, the define_system arithmetic (integer mass-action products over
+// `terms`, first-match choice).
+std::string jit_source(uint32_t S, const std::vector<JitReaction> &rx, bool system = false);
 // Compiles (or finds in the cache) and loads the kernel on the current device.
 int jit_get(const std::string &src, JitKernel *out, std::string *err);
 // Compiles for gfx950 without loading (no device needed).

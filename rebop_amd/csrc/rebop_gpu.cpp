@@ -103,7 +103,9 @@ int ensure_device() {
 
 struct Reaction {
   double k;
-  std::vector<std::pair<uint32_t, uint32_t>> entries; // (species, order) in evaluation order
+  // (species, order) in evaluation order; for a define_system network the
+  // reactant terms (species, multiplicity) in written order
+  std::vector<std::pair<uint32_t, uint32_t>> entries;
   std::vector<int64_t> diff;                          // dense [S]
   std::vector<uint32_t> expr;                         // Rate::Expr postfix tokens (else empty)
   std::vector<double> consts;                         // the expression's constants
@@ -140,6 +142,7 @@ struct rebop_gillespie {
   std::vector<int64_t> x0;
   std::vector<Reaction> rx;
   uint32_t n_expr = 0; // reactions with a rate expression (specialised kernels only)
+  bool system = false; // define_system semantics (specialised kernels only)
   // packed device copy of the network, rebuilt when reactions change
   bool dirty = true;
   int dev_id = -1;
@@ -263,7 +266,10 @@ int upload_net(rebop_gillespie *g) {
       any_slow = true;
     }
     const uint32_t nj = jb[r + 1] - jb[r];
-    if (nj > 4) {
+    bool slow_j = nj > 4;
+    for (uint32_t e = jb[r]; e < jb[r + 1]; ++e) // species in 8 bits, difference in 24
+      if (js[e] >= 256u || jd[e] < -(1 << 23) || jd[e] >= (1 << 23) || jd[e] == 0) slow_j = true;
+    if (slow_j) {
       w[3] |= REBOP_REC_SLOW_JUMP;
     } else {
       uint32_t *jw = jrec.data() + (size_t)REBOP_JREC_WORDS * r;
@@ -344,11 +350,14 @@ bool jit_eligible(const rebop_gillespie *g) {
   if (g->S > 32 || g->rx.size() > 64) return false;
   size_t F = 0;
   for (auto &r : g->rx) {
-    for (auto &e : r.entries) F += e.second;
+    for (auto &e : r.entries) F += e.second + (g->system ? 1u : 0u);
     F += r.expr.size();
   }
   return F <= 4096;
 }
+
+// Networks whose arithmetic exists only in the specialised kernels.
+bool specialised_only(const rebop_gillespie *g) { return g->n_expr || g->system; }
 
 bool jit_disabled() {
   const char *v = std::getenv("REBOP_NO_SPECIALIZE");
@@ -361,14 +370,17 @@ std::string jit_source_of(const rebop_gillespie *g) {
   for (auto &r : g->rx) {
     rebop::JitReaction j;
     j.k = r.k;
-    for (auto &e : r.entries)
-      for (uint32_t q = 0; q < e.second; ++q) j.factors.emplace_back(e.first, e.second - 1u - q);
+    if (g->system)
+      j.terms = r.entries;
+    else
+      for (auto &e : r.entries)
+        for (uint32_t q = 0; q < e.second; ++q) j.factors.emplace_back(e.first, e.second - 1u - q);
     j.diff = r.diff;
     j.expr = r.expr;
     j.consts = r.consts;
     jr.push_back(std::move(j));
   }
-  std::string src = rebop::jit_source(g->S, jr);
+  std::string src = rebop::jit_source(g->S, jr, g->system);
   // REBOP_DUMP_SPECIALIZED=<file>: write the generated kernel source there
   if (const char *path = std::getenv("REBOP_DUMP_SPECIALIZED")) {
     if (FILE *f = std::fopen(path, "w")) {
@@ -503,11 +515,13 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
   // when the network is small enough; else the data-driven register kernel
   // when it fits, else the LDS kernel
   const int variant = kernel & 0xF;
-  if (g->n_expr) {
-    // rate expressions are compiled into the specialised kernel only
+  if (specialised_only(g)) {
+    // rate expressions and define_system arithmetic are compiled into the
+    // specialised kernel only
     if (variant == REBOP_KERNEL_LDS)
       return fail(REBOP_ERR_UNSUPPORTED,
-                  "networks with rate expressions run on the specialised kernels only");
+                  g->system ? "define_system networks run on the specialised kernels only"
+                            : "networks with rate expressions run on the specialised kernels only");
     kernel |= REBOP_KERNEL_SPECIALIZED;
   }
   const bool fits_regs = pick_reg_shape(S, R) >= 0;
@@ -515,9 +529,10 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
   if (!use_lds && (kernel & REBOP_KERNEL_SPECIALIZED)) {
     if (!jit_eligible(g))
       return fail(REBOP_ERR_UNSUPPORTED,
-                  g->n_expr ? "network with rate expressions too large for a specialised kernel "
-                              "(S <= 32, R <= 64)"
-                            : "network too large for a specialised kernel");
+                  specialised_only(g)
+                      ? "network with rate expressions or define_system semantics too large for "
+                        "a specialised kernel (S <= 32, R <= 64)"
+                      : "network too large for a specialised kernel");
     rc = prepare_jit(g);
     if (rc) return rc;
     spec = true;
@@ -676,6 +691,54 @@ int rebop_gillespie_new(const int64_t *species, uint32_t nb_species, int sparse,
   return rebop_gillespie_new_with_seed(species, nb_species, sparse, seed, out);
 }
 
+int rebop_system_new(const int64_t *species, uint32_t nb_species, uint64_t seed,
+                     rebop_gillespie **out) {
+  const int rc = rebop_gillespie_new_with_seed(species, nb_species, 1, seed, out);
+  if (rc == REBOP_OK) (*out)->system = true;
+  return rc;
+}
+
+int rebop_system_add_reaction(rebop_gillespie *g, double rate, const uint32_t *reactants,
+                              const uint32_t *reactant_counts, uint32_t n_reactants,
+                              const uint32_t *products, const uint32_t *product_counts,
+                              uint32_t n_products) {
+  if (int rc = check_handle(g, "gillespie")) return rc;
+  if (!g->system)
+    return fail(REBOP_ERR_UNSUPPORTED, "rebop_system_add_reaction needs a rebop_system_new handle");
+  if (n_reactants && (!reactants || !reactant_counts))
+    return fail(REBOP_ERR_VALUE, "reactants is NULL");
+  if (n_products && (!products || !product_counts))
+    return fail(REBOP_ERR_VALUE, "products is NULL");
+  if (std::isinf(rate)) return fail(REBOP_ERR_VALUE, "rate constant must be finite (or NaN)");
+  std::vector<int64_t> d(g->S, 0);
+  for (int side = 0; side < 2; ++side) {
+    const uint32_t n = side ? n_products : n_reactants;
+    const uint32_t *sp = side ? products : reactants;
+    const uint32_t *m = side ? product_counts : reactant_counts;
+    for (uint32_t i = 0; i < n; ++i) {
+      if (sp[i] >= g->S)
+        return fail(REBOP_ERR_INDEX, "species index " + std::to_string(sp[i]) +
+                                         " out of range for " + std::to_string(g->S) + " species");
+      if (m[i] > REBOP_SYSTEM_MAX_COUNT)
+        return fail(REBOP_ERR_VALUE, "stoichiometric coefficient " + std::to_string(m[i]) +
+                                         " is beyond " + std::to_string(REBOP_SYSTEM_MAX_COUNT));
+      d[sp[i]] += side ? (int64_t)m[i] : -(int64_t)m[i];
+    }
+  }
+  for (uint32_t s = 0; s < g->S; ++s)
+    if (d[s] > REBOP_DIFF_LIMIT || d[s] < -REBOP_DIFF_LIMIT)
+      return fail(REBOP_ERR_VALUE, "stoichiometric difference " + std::to_string(d[s]) +
+                                       " is beyond |d| <= 2^20");
+  Reaction r;
+  r.k = rate;
+  for (uint32_t i = 0; i < n_reactants; ++i) r.entries.emplace_back(reactants[i], reactant_counts[i]);
+  r.diff = std::move(d);
+  g->rx.push_back(std::move(r));
+  g->dirty = true;
+  g->jit_dirty = true;
+  return REBOP_OK;
+}
+
 void rebop_gillespie_free(rebop_gillespie *g) { delete g; }
 
 int rebop_gillespie_seed(rebop_gillespie *g, uint64_t seed) {
@@ -693,6 +756,8 @@ static int add_reaction_common(rebop_gillespie *g, double rate, bool rate_sparse
                                const uint32_t *idx, const uint32_t *orders, uint32_t n,
                                const int64_t *diffs, uint32_t nd) {
   if (int rc = check_handle(g, "gillespie")) return rc;
+  if (g->system)
+    return fail(REBOP_ERR_UNSUPPORTED, "a define_system network takes rebop_system_add_reaction");
   if (n && !orders) return fail(REBOP_ERR_VALUE, "orders is NULL");
   if (rate_sparse && n && !idx) return fail(REBOP_ERR_VALUE, "species_idx is NULL");
   if (nd && !diffs) return fail(REBOP_ERR_VALUE, "differences is NULL");
@@ -751,6 +816,8 @@ int rebop_gillespie_add_reaction_expr(rebop_gillespie *g, const uint32_t *code, 
                                       const double *consts, uint32_t n_consts,
                                       const int64_t *differences, uint32_t n_differences) {
   if (int rc = check_handle(g, "gillespie")) return rc;
+  if (g->system)
+    return fail(REBOP_ERR_UNSUPPORTED, "a define_system network takes rebop_system_add_reaction");
   if (n_code && !code) return fail(REBOP_ERR_VALUE, "code is NULL");
   if (n_consts && !consts) return fail(REBOP_ERR_VALUE, "consts is NULL");
   if (n_differences && !differences) return fail(REBOP_ERR_VALUE, "differences is NULL");
@@ -1050,10 +1117,11 @@ int rebop_run_jobs_device(const rebop_job *jobs, uint32_t n_jobs, void *stream_,
     if (int e = check_handle(J.net, "job net")) return e;
     if (J.nb_steps == 0) return fail(REBOP_ERR_VALUE, "job " + std::to_string(j) + ": nb_steps must be > 0");
     rebop_gillespie *g = const_cast<rebop_gillespie *>(J.net);
-    if (g->n_expr)
+    if (specialised_only(g))
       return fail(REBOP_ERR_UNSUPPORTED, "job " + std::to_string(j) +
-                                             ": networks with rate expressions run on the "
-                                             "specialised kernels, not in fused batches");
+                                             ": networks with rate expressions or define_system "
+                                             "semantics run on the specialised kernels, not in "
+                                             "fused batches");
     if (!J.d_x0) {
       if (int e = check_x0(g->x0.data(), g->x0.size())) return e;
     }
@@ -1259,6 +1327,18 @@ int rebop_batch_get_time(const rebop_batch *b, double *t) {
 int rebop_batch_set_time(rebop_batch *b, const double *t) {
   if (int rc = check_handle(b, "batch")) return rc;
   if (b->n) HIP_TRY(hipMemcpy(b->t.p, t, sizeof(double) * b->n, hipMemcpyHostToDevice));
+  return REBOP_OK;
+}
+
+int rebop_batch_get_draws(const rebop_batch *b, uint64_t *draws) {
+  if (int rc = check_handle(b, "batch")) return rc;
+  if (b->n) HIP_TRY(hipMemcpy(draws, b->draw.p, sizeof(uint64_t) * b->n, hipMemcpyDeviceToHost));
+  return REBOP_OK;
+}
+
+int rebop_batch_set_draws(rebop_batch *b, const uint64_t *draws) {
+  if (int rc = check_handle(b, "batch")) return rc;
+  if (b->n) HIP_TRY(hipMemcpy(b->draw.p, draws, sizeof(uint64_t) * b->n, hipMemcpyHostToDevice));
   return REBOP_OK;
 }
 

@@ -30,8 +30,8 @@
 //   void write_counts(const X &x, int64_t *dst, uint64_t stride);  dst[s * stride]
 //   bool in_range(const X &x);          counts inside the exactly simulated range
 //   double props(X &x, C &cum);         make_cumrates -> total
-//   uint32_t choose(X &x, C &cum, double chosen);
-//   void affect(X &x, uint32_t k);
+//   uint32_t choose(X &x, C &cum, double chosen);   a reaction, or >= R for none
+//   void affect(X &x, uint32_t k);                   (then no change)
 // so that one loop serves counts in VGPRs (reg_* helpers below) and in LDS,
 // for a data-driven network or one compiled into the kernel.
 //

@@ -147,6 +147,31 @@ class GillespieNet:
         """``Gillespie::new_with_seed`` -- src/gillespie.rs:179-187."""
         return cls(species, sparse, seed)
 
+    @classmethod
+    def system(cls, species: Sequence[int], seed: int) -> "GillespieNet":
+        """A network with ``define_system!``'s arithmetic (src/gillespie_macro.rs:49-170):
+        integer mass-action products and the first-match choice.  Reactions
+        are added with ``add_system_reaction``."""
+        self = cls.__new__(cls)
+        lib = load()
+        x = np.ascontiguousarray(np.asarray(species, dtype=np.int64).reshape(-1))
+        h = c_void_p()
+        check(lib.rebop_system_new(_i64(x), len(x), int(seed) & (2**64 - 1), ctypes.byref(h)))
+        self._h = h
+        self._lib = lib
+        return self
+
+    def add_system_reaction(self, rate: float, reactants: Sequence[tuple[int, int]],
+                            products: Sequence[tuple[int, int]]) -> None:
+        """One ``define_system!`` reaction ``m1 r1 + ... => p1 q1 + ... @ rate``
+        (src/gillespie_macro.rs:55-58): (species, coefficient) terms in written order."""
+        ri = np.ascontiguousarray([t[0] for t in reactants], dtype=np.uint32)
+        rm = np.ascontiguousarray([t[1] for t in reactants], dtype=np.uint32)
+        pi = np.ascontiguousarray([t[0] for t in products], dtype=np.uint32)
+        pm = np.ascontiguousarray([t[1] for t in products], dtype=np.uint32)
+        check(self._lib.rebop_system_add_reaction(self._h, float(rate), _u32(ri), _u32(rm), len(ri),
+                                                  _u32(pi), _u32(pm), len(pi)))
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
@@ -367,6 +392,16 @@ class Batch:
     def set_time(self, t) -> None:
         a = np.ascontiguousarray(np.broadcast_to(np.asarray(t, dtype=np.float64), (self.n,)))
         check(self._lib.rebop_batch_set_time(self._h, _f64(a)))
+
+    def get_draws(self) -> np.ndarray:
+        """Draws made so far by every trajectory (its random stream's position)."""
+        out = np.zeros(self.n, dtype=np.uint64)
+        check(self._lib.rebop_batch_get_draws(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+        return out
+
+    def set_draws(self, draws) -> None:
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(draws, dtype=np.uint64), (self.n,)))
+        check(self._lib.rebop_batch_set_draws(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
 
 
 def run_jobs(jobs, stream: int | None = None, per_job_events: bool = False):

@@ -141,7 +141,7 @@ def test_fused_heterogeneous_jobs_bit_exact(nthreads):
 
 def test_lds_records_that_do_not_fit(nthreads):
     """Reactions whose factors (third order: three factors) or jump (five
-    species) do not fit the 32-byte LDS record take the global lists; the
+    species) do not fit the LDS record take the global lists; the
     result stays bit-exact, in a network that mixes both kinds."""
     S = 6
     rx = [
@@ -159,6 +159,35 @@ def test_lds_records_that_do_not_fit(nthreads):
     _, oout, _, oev, _ = o.run_grid(21, spec[2], 4.0, 20, n_traj=300, nthreads=nthreads)
     assert np.array_equal(out, oout)
     assert st.events == oev and oev > 300 * 50
+
+
+def test_lds_jumps_on_species_beyond_255(nthreads):
+    """A jump entry keeps 8 bits of species index in its record; jumps that
+    touch species 256 and above take the global lists and stay bit-exact
+    (300 species: molecules hop i -> i + 7 mod 300, with sources at 0 and 280)."""
+    S = 300
+    rx = []
+    for i in range(S):
+        o = [0] * S
+        o[i] = 1
+        d = [0] * S
+        d[i] = -1
+        d[(i + 7) % S] += 1
+        rx.append((0.5, o, d))
+    for s in (0, 280):
+        d = [0] * S
+        d[s] = 1
+        rx.append((3.0, [0] * S, d))
+    x0 = [0] * S
+    for i in range(250, S):
+        x0[i] = 2
+    spec = (S, rx, x0)
+    g = build_device(spec, 17)
+    out, st = _dev(g, 2.0, 10, 128, LDS)
+    o = build_oracle(spec)
+    _, oout, _, oev, _ = o.run_grid(17, x0, 2.0, 10, n_traj=128, nthreads=nthreads)
+    assert np.array_equal(out, oout)
+    assert st.events == oev and oev > 128 * 50
 
 
 def test_autotune_picks_a_variant_that_stays_bit_exact(nthreads):

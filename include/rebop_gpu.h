@@ -49,14 +49,16 @@ typedef struct rebop_stats {
   uint64_t draws;        /* Philox blocks drawn (events + discarded overshoot draws) */
   uint64_t trajectories; /* trajectories completed */
   uint64_t kernel;       /* which kernel ran: REBOP_KERNEL_* */
-  double kernel_ms;      /* device time of the SSA kernel (0 if not timed) */
+  double kernel_ms;      /* device time of the SSA kernel(s) (0 if not timed) */
+  uint64_t tail;         /* trajectories finished by the grouped tail launch */
 } rebop_stats;
 
 enum rebop_kernel_kind {
   REBOP_KERNEL_AUTO = 0,
   REBOP_KERNEL_PLAIN = 1,      /* one trajectory per lane, lanes retire when done */
   REBOP_KERNEL_PERSISTENT = 2, /* lanes refill from an atomic trajectory queue */
-  REBOP_KERNEL_LDS = 3,        /* counts in LDS (int32), any network up to 256 species */
+  REBOP_KERNEL_LDS = 3,        /* counts in LDS (int32): 256 bytes per species and 16 per
+                                  reaction must fit one workgroup's LDS (160 KB) */
   /* flag: the network compiled into the kernel (run-time compilation,
    * cached); combine with PLAIN or PERSISTENT */
   REBOP_KERNEL_SPECIALIZED = 16,
@@ -64,7 +66,13 @@ enum rebop_kernel_kind {
    * the next draws (PLAIN | SPECIALIZED grid runs; AUTO picks them when the
    * run has too few trajectories for more than one wave per SIMD) */
   REBOP_KERNEL_GROUP2 = 32,
-  REBOP_KERNEL_GROUP4 = 64
+  REBOP_KERNEL_GROUP4 = 64,
+  /* flag: a persistent specialised grid or target run finishes every
+   * trajectory in its one launch.  Without it, once the trajectory queue is
+   * empty, waves with at most 16 lanes still running hand those trajectories
+   * (state written to device memory) to a second launch that runs them with
+   * 4 lanes each, which shortens the end of the run; results are identical */
+  REBOP_KERNEL_WHOLE = 128
 };
 
 /* Opaque network + initial state: rebop's `gillespie::Gillespie` (src/gillespie.rs:157-163). */

@@ -30,6 +30,7 @@ REBOP_KERNEL_LDS = 3
 REBOP_KERNEL_SPECIALIZED = 16
 REBOP_KERNEL_GROUP2 = 32
 REBOP_KERNEL_GROUP4 = 64
+REBOP_KERNEL_WHOLE = 128
 
 # rate-expression bytecode operations (token = op | arg << 8)
 REBOP_EXPR_CONST = 1
@@ -122,6 +123,7 @@ class Stats(ctypes.Structure):
         ("trajectories", c_uint64),
         ("kernel", c_uint64),
         ("kernel_ms", c_double),
+        ("tail", c_uint64),
     ]
 
 

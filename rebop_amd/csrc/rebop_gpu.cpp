@@ -164,6 +164,8 @@ struct rebop_gillespie {
   int jit_dev = -1;
   rebop::JitKernel jit{};
   std::string jit_err;
+  // written-out state of the trajectories a persistent launch hands to its tail
+  DevBuf spill;
 };
 
 struct rebop_batch {
@@ -321,7 +323,9 @@ int upload_net(rebop_gillespie *g) {
   rebop_dev_job jt{};
   jt.S = S;
   jt.R = R;
-  jt.block = R ? (R + MB - 1) / MB : 1;
+  // checkpoint blocks of a multiple of 4 reactions (the LDS kernel's pass
+  // handles reactions in groups of 4); any block size gives the same sums
+  jt.block = R ? ((R + MB - 1) / MB + 3u) & ~3u : 4;
   jt.nblocks = R ? (R + jt.block - 1) / jt.block : 0;
   jt.flags = (linear ? REBOP_JOB_LINEAR : 0u) | (any_slow ? REBOP_JOB_SLOW : 0u);
   jt.k = g->view.k;
@@ -358,6 +362,22 @@ bool jit_eligible(const rebop_gillespie *g) {
 
 // Networks whose arithmetic exists only in the specialised kernels.
 bool specialised_only(const rebop_gillespie *g) { return g->n_expr || g->system; }
+
+// Lanes at or below which a wave of a persistent specialised launch hands
+// its trajectories to the grouped tail launch once the queue is empty
+// (REBOP_TAIL_LANES overrides 16; 0 turns the hand-off off), and the lanes
+// per trajectory of that launch (REBOP_TAIL_GROUP: 2 or 4).
+uint32_t tail_lanes() {
+  const char *v = std::getenv("REBOP_TAIL_LANES");
+  if (!v || !*v) return 16u;
+  const long n = std::strtol(v, nullptr, 10);
+  // below 64: a wave is thin only once some lane left, i.e. the queue is empty
+  return n <= 0 ? 0u : n >= 63 ? 63u : (uint32_t)n;
+}
+int tail_group() {
+  const char *v = std::getenv("REBOP_TAIL_GROUP");
+  return v && std::strcmp(v, "2") == 0 ? 2 : 4;
+}
 
 bool jit_disabled() {
   const char *v = std::getenv("REBOP_NO_SPECIALIZE");
@@ -482,6 +502,9 @@ int finish_stats(rebop_stats *stats, unsigned long long *counters, hipEvent_t ev
   stats->trajectories = ctr[REBOP_CTR_TRAJ];
   stats->kernel = (uint64_t)kind;
   stats->kernel_ms = ms;
+  stats->tail = ctr[REBOP_CTR_SPILL];
+  if (ctr[REBOP_CTR_FLAGS] & REBOP_FLAG_INTERNAL)
+    return fail(REBOP_ERR_DEVICE, "internal error: a kernel launch invariant did not hold");
   if (ctr[REBOP_CTR_FLAGS] & REBOP_FLAG_RANGE)
     return fail(REBOP_ERR_RANGE, "a count left the exactly simulated range (|x| <= 2^52), or a "
                                  "total propensity was infinite");
@@ -628,6 +651,32 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
   const int kind = (persistent ? REBOP_KERNEL_PERSISTENT : REBOP_KERNEL_PLAIN) |
                    (spec ? REBOP_KERNEL_SPECIALIZED : 0) |
                    (group == 4 ? REBOP_KERNEL_GROUP4 : group == 2 ? REBOP_KERNEL_GROUP2 : 0);
+  // tail hand-off (see rebop_dev_run): a wave spills at most spill_below
+  // trajectories, once, so the arrays need (waves x spill_below) entries
+  const uint32_t below = tail_lanes();
+  const bool tail = persistent && spec && !run.rates && run.mode != REBOP_MODE_EVENTS &&
+                    !(kernel & REBOP_KERNEL_WHOLE) && below > 0 && run.n_traj > 0;
+  const int tail_k = tail_group();
+  uint64_t tail_blocks = 0;
+  if (tail) {
+    const uint64_t cap = blocks * (uint64_t)threads / 64u * below;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_draw = al(sizeof(uint64_t) * cap), o_mark = o_draw + al(sizeof(uint64_t) * cap);
+    const size_t o_t = o_mark + al(sizeof(uint64_t) * cap), o_i = o_t + al(sizeof(double) * cap);
+    const size_t o_x = o_i + al(sizeof(uint32_t) * cap);
+    const size_t bytes = o_x + sizeof(double) * cap * (S ? S : 1);
+    HIP_TRY(g->spill.ensure(bytes));
+    char *b = (char *)g->spill.p;
+    run.spill_below = below;
+    run.spill_cap = cap;
+    run.spill_j = (uint64_t *)b;
+    run.spill_draw = (uint64_t *)(b + o_draw);
+    run.spill_mark = (uint64_t *)(b + o_mark);
+    run.spill_t = (double *)(b + o_t);
+    run.spill_i = (uint32_t *)(b + o_i);
+    run.spill_x = (double *)(b + o_x);
+    tail_blocks = (cap * (uint64_t)tail_k + threads - 1) / threads;
+  }
 
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (stats) {
@@ -641,6 +690,14 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
       HIP_TRY(rebop::jit_launch(g->jit, run, (int)blocks, threads, stream, group));
     else
       HIP_TRY(rebop::launch_reg(shape, g->view, run, (int)blocks, threads, lds, stream));
+  }
+  if (tail) {
+    // the handed-off trajectories, tail_k lanes each (entry e on group e)
+    rebop_dev_run r2 = run;
+    r2.persistent = 0;
+    r2.spill_below = 0;
+    r2.resume = 1;
+    HIP_TRY(rebop::jit_launch(g->jit, r2, (int)tail_blocks, threads, stream, tail_k));
   }
   if (stats) HIP_TRY(hipEventRecord(ev1, stream));
 

@@ -150,35 +150,49 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
   int64_t *orec = nullptr;
   uint32_t i = 0, ov = 0;
   LaneTotals acc;
-  bool done = true, started = false;
+  bool done = true, started = false, spill = false;
 
   for (;;) {
+    bool leave = false;
     if (done) {
       // next trajectory: the lane's own first index, then (persistent
       // launches) indices from the device-wide queue
       const uint64_t nj = next_trajectory(run, started, first, queue_offset);
-      if (nj >= n) break;
-      jl = nj;
-      net.start(run, jl);
-      if (run.st_x) {
-        net.load_state(x, run.st_x + jl, n);
-        t = run.st_t[jl];
-        draw = run.st_draw[jl];
+      if (nj >= n) {
+        leave = true;
       } else {
-        net.load_x0(x, run.x0_div ? run.x0 + (jl / run.x0_div) * net.S() : run.x0);
-        t = 0.0;
-        draw = 0;
+        jl = nj;
+        net.start(run, jl);
+        if (run.st_x) {
+          net.load_state(x, run.st_x + jl, n);
+          t = run.st_t[jl];
+          draw = run.st_draw[jl];
+        } else {
+          net.load_x0(x, run.x0_div ? run.x0 + (jl / run.x0_div) * net.S() : run.x0);
+          t = 0.0;
+          draw = 0;
+        }
+        if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
+        draw0 = draw;
+        mark = draw;
+        ov = 0;
+        i = 0;
+        T = grid_time(run, 0);
+        orec = run.out ? run.out + jl : nullptr;
+        done = false;
+        tot = props_checked(net, x, cum, acc);
       }
-      if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
-      draw0 = draw;
-      mark = draw;
-      ov = 0;
-      i = 0;
-      T = grid_time(run, 0);
-      orec = run.out ? run.out + jl : nullptr;
-      done = false;
-      tot = props_checked(net, x, cum, acc);
     }
+    // Tail hand-off.  In a persistent launch a lane leaves the loop only when
+    // the queue is empty, so a thin wave means no work is left to take: its
+    // remaining lanes leave too and write their state out below, and a
+    // grouped launch finishes those trajectories (lane_loop_group).  Both
+    // reasons share one exit, which keeps the loop's scalar registers free.
+    if (!leave && run.spill_below && (uint32_t)__popcll(__ballot(1)) <= run.spill_below) {
+      spill = true;
+      leave = true;
+    }
+    if (leave) break;
 
     bool rec = true; // absorbing state: sample without drawing
     double tn = 0.0, u = 0.0;
@@ -221,6 +235,29 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
       tot = props_checked(net, x, cum, acc);
     }
   }
+  if (spill) {
+    // one entry per handed-off trajectory, one atomic per wave
+    const uint64_t act = __ballot(1);
+    const uint32_t lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    unsigned long long base = 0ull;
+    if ((int)lane == leader)
+      base = atomicAdd(&run.counters[REBOP_CTR_SPILL], (unsigned long long)__popcll(act));
+    base = __shfl(base, leader);
+    const uint64_t e = base + (uint64_t)__popcll(act & ((1ull << lane) - 1ull));
+    if (e < run.spill_cap) {
+      run.spill_j[e] = jl;
+      run.spill_draw[e] = draw;
+      run.spill_mark[e] = mark;
+      run.spill_t[e] = t;
+      run.spill_i[e] = i;
+      net.store_state(x, run.spill_x + e, run.spill_cap);
+    } else {
+      acc.flags |= REBOP_FLAG_INTERNAL; // cannot happen: one entry per lane
+    }
+    acc.events += (draw - draw0) - ov;
+    acc.draws += draw - draw0;
+  }
   return acc;
 }
 
@@ -258,14 +295,24 @@ __device__ __forceinline__ double group_read(double v, int q) {
 // draw, and a trajectory in an absorbing state never draws again (its counts
 // no longer change), so the blocks of a round are used in order or not at
 // all.  All lanes of a group hold the same state; lane 0 of the group writes
-// and counts.  Plain launches only: trajectory j is group j.
+// and counts.  Plain launches only: trajectory j is group j, or with
+// run.resume the trajectory of spill entry j (the tail of a persistent
+// launch, continued from its written-out state).
 template <class Net, int K>
 __device__ __forceinline__ LaneTotals lane_loop_group(Net &net, const rebop_dev_run &run,
                                                       uint64_t gtid) {
   LaneTotals acc;
   const uint64_t n = run.n_traj;
-  const uint64_t jl = gtid / K;
-  if (jl >= n) return acc;
+  const uint64_t e = gtid / K; // group index
+  uint64_t jl = e;
+  if (run.resume) {
+    const uint64_t cnt = __hip_atomic_load(&run.counters[REBOP_CTR_SPILL], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+    if (e >= cnt || e >= run.spill_cap) return acc;
+    jl = run.spill_j[e];
+  } else if (jl >= n) {
+    return acc;
+  }
   const uint32_t sub = __lane_id() % K;
   const bool lead = sub == 0;
   const uint32_t nb = run.mode == REBOP_MODE_TARGET ? 0u : (uint32_t)run.nb_steps;
@@ -279,22 +326,29 @@ __device__ __forceinline__ LaneTotals lane_loop_group(Net &net, const rebop_dev_
   for (int r = 0; r < Net::MR; ++r) cum[r] = 0.0;
   net.start(run, jl);
   double t;
-  uint64_t draw;
-  if (run.st_x) {
+  uint64_t draw, mark;
+  uint32_t i = 0, ov = 0;
+  if (run.resume) {
+    net.load_state(x, run.spill_x + e, run.spill_cap);
+    t = run.spill_t[e];
+    draw = run.spill_draw[e];
+    mark = run.spill_mark[e];
+    i = run.spill_i[e];
+  } else if (run.st_x) {
     net.load_state(x, run.st_x + jl, n);
     t = run.st_t[jl];
     draw = run.st_draw[jl];
+    mark = draw;
   } else {
     net.load_x0(x, run.x0_div ? run.x0 + (jl / run.x0_div) * net.S() : run.x0);
     t = 0.0;
     draw = 0;
+    mark = 0;
   }
   if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
   const uint64_t draw0 = draw;
-  uint64_t mark = draw;
-  uint32_t i = 0, ov = 0;
-  double T = grid_time(run, 0), tot = 0.0;
-  int64_t *orec = run.out && lead ? run.out + jl : nullptr;
+  double T = grid_time(run, i), tot = 0.0;
+  int64_t *orec = run.out && lead ? run.out + jl + (uint64_t)i * Sn : nullptr;
   bool done = false;
   tot = props_checked(net, x, cum, acc);
 
@@ -430,6 +484,27 @@ __device__ __forceinline__ void flush_block_totals(unsigned long long *counters,
     atomicAdd(&counters[REBOP_CTR_DRAWS], blk_ctr[1]);
     atomicAdd(&counters[REBOP_CTR_TRAJ], blk_ctr[2]);
     if (blk_ctr[3]) atomicOr(&counters[REBOP_CTR_FLAGS], blk_ctr[3]);
+  }
+}
+
+// One global update per wave, reduced across the wave's lanes (no LDS): for
+// kernels whose LDS is all spoken for.
+__device__ __forceinline__ void flush_wave_totals(const LaneTotals &acc,
+                                                  unsigned long long *counters) {
+  unsigned long long ev = acc.events, dr = acc.draws, tr = acc.trajectories;
+  uint32_t fl = acc.flags;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    ev += __shfl_xor(ev, off);
+    dr += __shfl_xor(dr, off);
+    tr += __shfl_xor(tr, off);
+    fl |= __shfl_xor(fl, off);
+  }
+  if (__lane_id() == 0) {
+    atomicAdd(&counters[REBOP_CTR_EVENTS], ev);
+    atomicAdd(&counters[REBOP_CTR_DRAWS], dr);
+    atomicAdd(&counters[REBOP_CTR_TRAJ], tr);
+    if (fl) atomicOr(&counters[REBOP_CTR_FLAGS], (unsigned long long)fl);
   }
 }
 

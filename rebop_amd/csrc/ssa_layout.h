@@ -36,6 +36,7 @@
 
 /* error flags raised by kernels */
 #define REBOP_FLAG_RANGE 1u
+#define REBOP_FLAG_INTERNAL 2u /* a launch invariant broke (reported as a device error) */
 
 /* counters[] slots */
 #define REBOP_CTR_EVENTS 0
@@ -43,6 +44,7 @@
 #define REBOP_CTR_TRAJ 2
 #define REBOP_CTR_FLAGS 3
 #define REBOP_CTR_QUEUE 4
+#define REBOP_CTR_SPILL 5 /* trajectories handed from a persistent launch to its tail */
 #define REBOP_CTR_COUNT 8
 
 typedef struct {
@@ -83,6 +85,22 @@ typedef struct {
   const uint64_t *rec_offsets; /* [n] or NULL */
   double *rec_times;           /* [total] */
   int64_t *rec_states;         /* [total][S] */
+  /* Tail hand-off of a persistent launch.  Once the trajectory queue is
+   * empty, a wave with at most spill_below lanes still running writes their
+   * trajectories' state (index, grid position, time, stream position, range
+   * mark, counts) to entry counters[REBOP_CTR_SPILL]++ of the spill arrays and
+   * stops; a grouped launch with resume = 1 then continues entry e on the
+   * lanes of group e, so the longest trajectories finish on several lanes
+   * each instead of holding a mostly idle wave. */
+  uint32_t spill_below; /* 0: no hand-off */
+  uint32_t resume;      /* trajectories are spill entries */
+  uint64_t spill_cap;   /* entries (and the row stride of spill_x) */
+  uint64_t *spill_j;    /* [cap] local trajectory index */
+  uint64_t *spill_draw; /* [cap] draws made */
+  uint64_t *spill_mark; /* [cap] draw count at the last sample (range check) */
+  double *spill_t;      /* [cap] */
+  uint32_t *spill_i;    /* [cap] next grid point */
+  double *spill_x;      /* [S][cap] counts */
 } rebop_dev_run;
 
 /*

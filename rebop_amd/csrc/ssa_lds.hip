@@ -105,10 +105,18 @@ struct LdsNet {
     return a;
   }
 
-  // factor (x[s] - c) of a record word, or exactly 1.0 when absent
+  // factor (x[s] - c) of a record word, or exactly 1.0 when absent.  The
+  // difference is taken in int32 (|x| <= 2^30, c < 4096) and converted once:
+  // the integer Rate::rate converts, so the value is the same
+  __device__ __forceinline__ int32_t factor_int(uint32_t w, int32_t x) const {
+    const int32_t v = x - (int32_t)((w >> 16) & 0xfffu);
+    return (w & REBOP_REC_PRESENT) ? v : 1;
+  }
   __device__ __forceinline__ double factor(uint32_t w) const {
-    const double v = fcnt(w & 0xffffu) - (double)((w >> 16) & 0xfffu);
-    return (w & REBOP_REC_PRESENT) ? v : 1.0;
+    return (double)factor_int(w, cnt(w & 0xffffu));
+  }
+  __device__ __forceinline__ uint4 record(uint32_t r) const {
+    return *(const uint4 *)(rec + (size_t)r * REBOP_REC_WORDS);
   }
 
   // Rate::rate of reaction r, factors in stored order: k * f1 * f2
@@ -130,21 +138,47 @@ struct LdsNet {
     return a;
   }
 
+  // k * f1 * f2 of one record whose factor counts are already read
+  __device__ __forceinline__ double rate_of(const uint4 &q, int32_t x1, int32_t x2) const {
+    double a = rebop_f64_from_bits(((uint64_t)q.y << 32) | q.x);
+    a = a * (double)factor_int(q.z, x1);
+    a = a * (double)factor_int(q.w, x2);
+    return a;
+  }
+
   // make_cumrates with a checkpoint at the end of every block
   __device__ __forceinline__ double props(double (&)[MS], double (&cum)[MR]) const {
     double acc = 0.0;
     uint32_t r = 0;
+    if (R == 0) return 0.0; // no record to prefetch
     if (!job_slow) {
-      // every reaction's factors are in its record: four reactions per step,
-      // whose record and count reads are independent of each other (the sum
-      // still adds them one by one, in order)
+      // every reaction's factors are in its record: groups of four reactions
+      // (blocks hold a multiple of 4), software-pipelined: the records of the
+      // next group are read while this group's counts are in flight, so a
+      // group costs one LDS round trip (the sum still adds them one by one)
+      const uint32_t last = R - 1u;
+      uint4 q0 = record(0), q1 = record(1u < last ? 1u : last);
+      uint4 q2 = record(2u < last ? 2u : last), q3 = record(3u < last ? 3u : last);
 #pragma unroll
       for (int b = 0; b < MB; ++b) { // static: cum[] stays in VGPRs
         if ((uint32_t)b < nblk) {
           const uint32_t re = (r + B < R) ? r + B : R;
           for (; r + 4 <= re; r += 4) {
-            const double a0 = rate_fast(r), a1 = rate_fast(r + 1);
-            const double a2 = rate_fast(r + 2), a3 = rate_fast(r + 3);
+            const uint4 c0 = q0, c1 = q1, c2 = q2, c3 = q3;
+            const uint32_t n0 = r + 4u < last ? r + 4u : last, n1 = r + 5u < last ? r + 5u : last;
+            const uint32_t n2 = r + 6u < last ? r + 6u : last, n3 = r + 7u < last ? r + 7u : last;
+            q0 = record(n0);
+            q1 = record(n1);
+            q2 = record(n2);
+            q3 = record(n3);
+            const int32_t x0a = cnt(c0.z & 0xffffu), x0b = cnt(c0.w & 0xffffu);
+            const int32_t x1a = cnt(c1.z & 0xffffu), x1b = cnt(c1.w & 0xffffu);
+            const int32_t x2a = cnt(c2.z & 0xffffu), x2b = cnt(c2.w & 0xffffu);
+            const int32_t x3a = cnt(c3.z & 0xffffu), x3b = cnt(c3.w & 0xffffu);
+            // keep the next records' reads ahead of the wait for these counts
+            __builtin_amdgcn_sched_barrier(0);
+            const double a0 = rate_of(c0, x0a, x0b), a1 = rate_of(c1, x1a, x1b);
+            const double a2 = rate_of(c2, x2a, x2b), a3 = rate_of(c3, x3a, x3b);
             acc = acc + a0;
             acc = acc + a1;
             acc = acc + a2;
@@ -230,10 +264,9 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
                                                       unsigned long long *counters,
                                                       unsigned long long *job_events,
                                                       uint32_t rec_word_off) {
+  // no static LDS: the dynamic size (counts + records) alone decides how
+  // many workgroups share a CU (e.g. 4 x 40 KB for 128 species, 512 reactions)
   extern __shared__ int32_t lds_x[]; // [S_max][blockDim.x], then the records
-  __shared__ unsigned long long blk_ctr[4];
-  if (threadIdx.x < 4) blk_ctr[threadIdx.x] = 0ull;
-  __syncthreads();
 
   const uint32_t lane = __lane_id();
   LaneTotals total;
@@ -312,9 +345,7 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
     total.trajectories += acc.trajectories;
     total.flags |= acc.flags;
   }
-  add_totals(total, blk_ctr);
-  __syncthreads();
-  flush_block_totals(counters, blk_ctr);
+  flush_wave_totals(total, counters);
 }
 
 // ---------------------------------------------------------------------------

@@ -28,6 +28,8 @@ int reg_shape_ms(int c);
 int reg_shape_mr(int c);
 hipError_t reg_occupancy(int c, int threads, size_t lds, int *blocks_per_cu);
 hipError_t launch_reset_counters(unsigned long long *counters, hipStream_t stream);
+hipError_t launch_grid(double *grid, const rebop_dev_run &run, hipStream_t stream);
+hipError_t launch_raw_counts(int64_t *out, uint64_t n, int cus, hipStream_t stream);
 hipError_t launch_reg(int c, const rebop_dev_net &net, const rebop_dev_run &run, int blocks,
                       int threads, size_t lds, hipStream_t stream);
 
@@ -166,6 +168,8 @@ struct rebop_gillespie {
   std::string jit_err;
   // written-out state of the trajectories a persistent launch hands to its tail
   DevBuf spill;
+  // sample times of the current launch (register and specialised kernels)
+  DevBuf grid;
 };
 
 struct rebop_batch {
@@ -684,6 +688,11 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
     HIP_TRY(hipEventCreate(&ev1));
   }
   HIP_TRY(rebop::launch_reset_counters(run.counters, stream));
+  // the sample times, read by the kernels at every sample
+  HIP_TRY(g->grid.ensure(sizeof(double) *
+                         (run.mode == REBOP_MODE_GRID ? (size_t)run.nb_steps + 1u : 1u)));
+  run.grid = (const double *)g->grid.p;
+  HIP_TRY(rebop::launch_grid((double *)g->grid.p, run, stream));
   if (stats) HIP_TRY(hipEventRecord(ev0, stream));
   if (run.n_traj > 0) {
     if (spec)
@@ -699,6 +708,10 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
     r2.resume = 1;
     HIP_TRY(rebop::jit_launch(g->jit, r2, (int)tail_blocks, threads, stream, tail_k));
   }
+  // the samples were written as f64 bits: convert them to int64 in place
+  if (run.out && run.mode == REBOP_MODE_GRID && run.n_traj > 0)
+    HIP_TRY(rebop::launch_raw_counts(run.out, (run.nb_steps + 1u) * (uint64_t)S * run.n_traj,
+                                     g_dev.cus, stream));
   if (stats) HIP_TRY(hipEventRecord(ev1, stream));
 
   if (stats) return finish_stats(stats, run.counters, ev0, ev1, kind, stream);

@@ -24,6 +24,10 @@ struct GenericNet {
   const double *lds_diff; // [R][S]
   double kr[MR];          // this trajectory's rate constants
 
+  __device__ __forceinline__ double grid_time(const rebop_dev_run &run, uint64_t i) const {
+    return table_time(run, i);
+  }
+
   // the network's constants, or (parameter sweeps) this trajectory's row
   __device__ __forceinline__ void start(const rebop_dev_run &run, uint64_t j) {
     const double *src = run.rates ? run.rates + (j / run.rates_div) * net.R : net.k;
@@ -39,6 +43,9 @@ struct GenericNet {
   }
   __device__ __forceinline__ void store_state(const double (&x)[MS], double *st, uint64_t stride) const {
     reg_store_state(x, st, stride, net.S);
+  }
+  __device__ __forceinline__ void write_raw(const double (&x)[MS], int64_t *dst, uint64_t stride) const {
+    reg_write_raw(x, dst, stride, net.S);
   }
   __device__ __forceinline__ void write_counts(const double (&x)[MS], int64_t *dst, uint64_t stride) const {
     reg_write_counts(x, dst, stride, net.S);
@@ -99,6 +106,13 @@ __global__ void __launch_bounds__(256) ssa_reg_kernel(rebop_dev_net net, rebop_d
 
 // Clears the launch counters with a kernel (not hipMemsetAsync), so that the
 // sequence stays valid inside a captured hipGraph.
+// The sample times tmax * i / nb_steps (src/pyo3_gillespie.rs:202: product,
+// then quotient, as grid_time computes them), or tmax for a target run.
+__global__ void grid_kernel(double *grid, double tmax, uint64_t nb, uint64_t n, int is_grid) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) grid[i] = is_grid ? grid_time(tmax, i, nb) : tmax;
+}
+
 __global__ void reset_counters_kernel(unsigned long long *counters) {
   if (threadIdx.x < REBOP_CTR_COUNT) counters[threadIdx.x] = 0ull;
 }
@@ -136,6 +150,42 @@ hipError_t reg_occupancy(int c, int threads, size_t lds, int *blocks_per_cu) {
 
 hipError_t launch_reset_counters(unsigned long long *counters, hipStream_t stream) {
   hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, stream, counters);
+  return hipGetLastError();
+}
+
+// out[k] holds the f64 bits of an exact integer count (|x| <= 2^52 unless
+// the run reports a range error): replace it by the int64 value.
+__global__ void raw_counts_kernel(int64_t *out, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * 2u;
+  for (uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2u; k < n; k += stride) {
+    if (k + 1u < n) {
+      int2 *p = (int2 *)(out + k); // 16-byte aligned: k is even
+      const longlong2 v = *(const longlong2 *)p;
+      longlong2 r;
+      r.x = (long long)rebop_f64_from_bits((uint64_t)v.x);
+      r.y = (long long)rebop_f64_from_bits((uint64_t)v.y);
+      *(longlong2 *)p = r;
+    } else {
+      out[k] = (int64_t)rebop_f64_from_bits((uint64_t)out[k]);
+    }
+  }
+}
+
+hipError_t launch_raw_counts(int64_t *out, uint64_t n, int cus, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  uint64_t blocks = (n / 2u + 255u) / 256u;
+  const uint64_t cap = (uint64_t)(cus > 0 ? cus : 256) * 16u;
+  if (blocks > cap) blocks = cap;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(raw_counts_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_grid(double *grid, const rebop_dev_run &run, hipStream_t stream) {
+  const uint64_t n = run.mode == REBOP_MODE_GRID ? run.nb_steps + 1u : 1u;
+  const unsigned blocks = (unsigned)((n + 255u) / 256u);
+  hipLaunchKernelGGL(grid_kernel, dim3(blocks), dim3(256), 0, stream, grid, run.tmax, run.nb_steps,
+                     n, run.mode == REBOP_MODE_GRID ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -28,7 +28,11 @@
 //   void load_state(X &x, const double *st, uint64_t stride);  st[s * stride]
 //   void store_state(const X &x, double *st, uint64_t stride);
 //   void write_counts(const X &x, int64_t *dst, uint64_t stride);  dst[s * stride]
+//   void write_raw(const X &x, int64_t *dst, uint64_t stride);     the samples: as
+//        write_counts, or (counts in f64) their f64 bits, which a pass after
+//        the launch converts in place (raw_counts_to_int64)
 //   bool in_range(const X &x);          counts inside the exactly simulated range
+//   double grid_time(const rebop_dev_run &run, uint64_t i);   sample time i
 //   double props(X &x, C &cum);         make_cumrates -> total
 //   uint32_t choose(X &x, C &cum, double chosen);   a reaction, or >= R for none
 //   void affect(X &x, uint32_t k);                   (then no change)
@@ -50,10 +54,19 @@
 
 namespace rebop {
 
+__device__ __forceinline__ double grid_time(double tmax, uint64_t i, uint64_t nb) {
+  // src/pyo3_gillespie.rs:202: tmax * i as f64 / nb_steps as f64
+  return tmax * (double)i / (double)nb;
+}
+
 __device__ __forceinline__ double grid_time(const rebop_dev_run &run, uint64_t i) {
   if (run.mode == REBOP_MODE_TARGET) return run.tmax;
-  // src/pyo3_gillespie.rs:202: tmax * i as f64 / nb_steps as f64
-  return run.tmax * (double)i / (double)run.nb_steps;
+  return grid_time(run.tmax, i, run.nb_steps);
+}
+
+// The sample time from the table written before the launch (same value).
+__device__ __forceinline__ double table_time(const rebop_dev_run &run, uint64_t i) {
+  return run.grid[i];
 }
 
 struct LaneTotals {
@@ -87,6 +100,17 @@ __device__ __forceinline__ void reg_write_counts(const double (&x)[MS], int64_t 
   for (int s = 0; s < MS; ++s)
     if (s < (int)S) dst[(uint64_t)s * stride] = (int64_t)x[s];
 }
+// The sample's counts as their f64 bits (8 bytes each, in the int64 slots):
+// the conversion, five FP64 operations per count, runs after the launch in
+// a streaming pass instead of in the event loop.
+template <int MS>
+__device__ __forceinline__ void reg_write_raw(const double (&x)[MS], int64_t *dst, uint64_t stride,
+                                              uint32_t S) {
+#pragma unroll
+  for (int s = 0; s < MS; ++s)
+    if (s < (int)S) dst[(uint64_t)s * stride] = (int64_t)rebop_f64_bits(x[s]);
+}
+
 template <int MS>
 __device__ __forceinline__ bool reg_in_range(const double (&x)[MS], uint32_t S) {
   bool ok = true;
@@ -177,7 +201,7 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
         mark = draw;
         ov = 0;
         i = 0;
-        T = grid_time(run, 0);
+        T = net.grid_time(run, 0);
         orec = run.out ? run.out + jl : nullptr;
         done = false;
         tot = props_checked(net, x, cum, acc);
@@ -207,7 +231,7 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
 
     if (rec) {
       if (orec) {
-        net.write_counts(x, orec, n);
+        net.write_raw(x, orec, n);
         orec += Sn;
       }
       if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
@@ -227,7 +251,7 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
         acc.trajectories += 1;
         done = true;
       } else {
-        T = grid_time(run, i);
+        T = net.grid_time(run, i);
       }
     } else {
       t = tn;
@@ -347,6 +371,7 @@ __device__ __forceinline__ LaneTotals lane_loop_group(Net &net, const rebop_dev_
   }
   if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
   const uint64_t draw0 = draw;
+  // latency-bound launches: computing a sample time beats loading it
   double T = grid_time(run, i), tot = 0.0;
   int64_t *orec = run.out && lead ? run.out + jl + (uint64_t)i * Sn : nullptr;
   bool done = false;
@@ -370,7 +395,7 @@ __device__ __forceinline__ LaneTotals lane_loop_group(Net &net, const rebop_dev_
         }
         if (rec) {
           if (orec) {
-            net.write_counts(x, orec, n);
+            net.write_raw(x, orec, n);
             orec += Sn;
           }
           if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;

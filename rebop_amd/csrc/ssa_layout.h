@@ -79,6 +79,9 @@ typedef struct {
   double *st_t;        /* [n] */
   uint64_t *st_draw;   /* [n] */
   unsigned long long *counters; /* [REBOP_CTR_COUNT] */
+  const double *grid;  /* [nb_steps + 1] sample times tmax * i / nb_steps ([0] = tmax
+                          in target mode), written before the launch; the LDS
+                          kernel computes them instead */
   /* REBOP_MODE_EVENTS: records of trajectory j go to rows rec_offsets[j] ..
    * (row = time + counts); with rec_offsets == NULL only n_records[j] is set */
   uint64_t *n_records;         /* [n] */

@@ -70,6 +70,10 @@ struct LdsNet {
 
   __device__ __forceinline__ uint32_t S() const { return nS; }
   __device__ __forceinline__ void start(const rebop_dev_run &, uint64_t) {}
+  // each job has its own horizon: computed, not tabled
+  __device__ __forceinline__ double grid_time(const rebop_dev_run &run, uint64_t i) const {
+    return rebop::grid_time(run, i);
+  }
 
   __device__ __forceinline__ void set_count(uint32_t s, double v) {
     if (!(__builtin_fabs(v) <= (double)REBOP_LDS_LIMIT)) {
@@ -91,6 +95,10 @@ struct LdsNet {
   }
   __device__ __forceinline__ void store_state(const double (&)[MS], double *st, uint64_t stride) const {
     for (uint32_t s = 0; s < nS; ++s) st[(uint64_t)s * stride] = fcnt(s);
+  }
+  // int32 counts: written as int64 directly (no conversion pass)
+  __device__ __forceinline__ void write_raw(const double (&x)[MS], int64_t *dst, uint64_t stride) const {
+    write_counts(x, dst, stride);
   }
   __device__ __forceinline__ void write_counts(const double (&)[MS], int64_t *dst, uint64_t stride) const {
     for (uint32_t s = 0; s < nS; ++s) dst[(uint64_t)s * stride] = (int64_t)cnt(s);

@@ -7,9 +7,11 @@ nb_steps=100 samples (101 grid points), FP64 times/propensities, int64
 output [101][4][10^6] written to HBM.  The network and the output buffer are
 resident on the device before timing starts; each step uses a new seed.
 
-Prints ONE JSON line (see the keys below).  ``roofline`` uses the SSA
-kernel's own device time (HIP events on its launch stream) and the
-algorithmic FP64 operation count per event; ``cpu_baseline`` times the CPU
+Prints ONE JSON line (see the keys below).  ``roofline`` uses the device
+time of the SSA launch sequence (the main kernel, the grouped launch that
+finishes its last trajectories, and the pass that converts the f64 samples to
+int64; HIP events on their launch stream) and the algorithmic FP64 operation
+count per event; ``cpu_baseline`` times the CPU
 oracle (the C restatement of the reference algorithm) on a bounded sample of
 the same workload over the host's cores.
 """
@@ -432,6 +434,7 @@ def main() -> int:
             "traffic": traffic,
             "ops_per_event": F,
             "kernel_ms": avg_kernel_s * 1e3,
+            "timed": "SSA launch sequence: main kernel + tail launch + int64 sample conversion (HIP events)",
         },
     }
     if not args.no_cpu_baseline:

@@ -155,16 +155,22 @@ hipError_t launch_reset_counters(unsigned long long *counters, hipStream_t strea
 
 // out[k] holds the f64 bits of an exact integer count (|x| <= 2^52 unless
 // the run reports a range error): replace it by the int64 value.
-__global__ void raw_counts_kernel(int64_t *out, uint64_t n) {
+__global__ void raw_counts_kernel(int64_t *out, uint64_t n, int pairs) {
+  if (!pairs) { // an output that is only 8-byte aligned
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+      out[k] = (int64_t)rebop_f64_from_bits((uint64_t)out[k]);
+    return;
+  }
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * 2u;
   for (uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2u; k < n; k += stride) {
     if (k + 1u < n) {
-      int2 *p = (int2 *)(out + k); // 16-byte aligned: k is even
-      const longlong2 v = *(const longlong2 *)p;
+      longlong2 *p = (longlong2 *)(out + k); // 16-byte aligned: out is, and k is even
+      const longlong2 v = *p;
       longlong2 r;
       r.x = (long long)rebop_f64_from_bits((uint64_t)v.x);
       r.y = (long long)rebop_f64_from_bits((uint64_t)v.y);
-      *(longlong2 *)p = r;
+      *p = r;
     } else {
       out[k] = (int64_t)rebop_f64_from_bits((uint64_t)out[k]);
     }
@@ -177,7 +183,8 @@ hipError_t launch_raw_counts(int64_t *out, uint64_t n, int cus, hipStream_t stre
   const uint64_t cap = (uint64_t)(cus > 0 ? cus : 256) * 16u;
   if (blocks > cap) blocks = cap;
   if (blocks == 0) blocks = 1;
-  hipLaunchKernelGGL(raw_counts_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, out, n);
+  const int pairs = ((uintptr_t)out & 15u) == 0u ? 1 : 0;
+  hipLaunchKernelGGL(raw_counts_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, out, n, pairs);
   return hipGetLastError();
 }
 

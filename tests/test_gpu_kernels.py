@@ -282,3 +282,24 @@ def test_grouped_lanes_refuse_persistent_and_generic():
         with pytest.raises(K.RebopError) as e:
             g.run_device(10.0, 4, 64, out.data_ptr(), kernel=kern)
         assert e.value.code == K.REBOP_ERR_UNSUPPORTED
+
+
+def test_output_only_8_byte_aligned(nthreads):
+    """The sample conversion pass reads pairs of int64 when the output is
+    16-byte aligned; an output at an odd int64 offset takes the one-by-one
+    path and gives the same samples."""
+    import torch
+
+    spec = dimers()
+    g = build_device(spec, 14)
+    n, nb, tmax = 999, 20, 0.3
+    total = (nb + 1) * 4 * n
+    base = torch.zeros(total + 1, dtype=torch.int64, device="cuda")
+    assert base.data_ptr() % 16 == 0
+    st = g.run_device(tmax, nb, n, base.data_ptr() + 8)  # 8-byte aligned only
+    torch.cuda.synchronize()
+    got = base[1:].cpu().numpy().reshape(nb + 1, 4, n)
+    assert int(base[0]) == 0  # nothing written before the output
+    o = build_oracle(spec)
+    _, oout, _, oev, _ = o.run_grid(14, spec[2], tmax, nb, n_traj=n, nthreads=nthreads)
+    assert np.array_equal(got, oout) and st.events == oev

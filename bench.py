@@ -299,6 +299,8 @@ def surrogate_main(args) -> int:
             "speedup_vs_gpu_ssa": res["speedup"],
             "w1_pass_fraction": res["w1_pass_fraction"],
             "point_pass_fraction": res["point_pass_fraction"],
+            "w1_pass_fraction_reproducible": res["w1_pass_fraction_reproducible"],
+            "ssa_self_w1_over_2pct": res["ssa_self_w1_over_2pct"],
             "trusted_cells": res["trusted_cells"],
             "rel_w1_median": res["rel_w1_median"],
             "train_s": res["train_s"],

@@ -35,6 +35,18 @@ const char *rebop_sg_last_error(void);
 int rebop_sg_gemm_bf16(int ta, int tb, int M, int N, int K, const void *A, int lda, const void *B,
                        int ldb, float *C, int ldc, void *stream);
 
+/* H = bf16(act(A B + bias)) and/or F = act(A B + bias) (FP32) in one launch:
+ * the product of rebop_sg_gemm_bf16 (A [M][K], B [K][N], no transposes)
+ * with rebop_sg_bias_act's epilogue applied to its FP32 accumulators, without
+ * the FP32 product going through memory.  Never splits K, so where
+ * rebop_sg_gemm_bf16 does not split it either (outputs of at least 512
+ * tiles) the values are the same bits.  M, N multiples of 64, K of 32;
+ * bias [N]; H or F may be NULL (not both).  Returns 3 on a bad shape or
+ * argument. */
+int rebop_sg_gemm_bias_act_bf16(int M, int N, int K, const void *A, int lda, const void *B,
+                                int ldb, const float *bias, int relu, void *H, int ldh, float *F,
+                                int ldf, void *stream);
+
 /* H = bf16(act(C + bias)) and/or F = act(C + bias) (either may be NULL). */
 int rebop_sg_bias_act(int M, int N, const float *C, int ldc, const float *bias, int relu, void *H,
                       int ldh, float *F, int ldf, void *stream);

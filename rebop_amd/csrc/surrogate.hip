@@ -31,11 +31,23 @@ namespace sg {
 // tiles: a 128 x 128 tile issues 16 MFMAs per wave per 8 fragment loads.
 // M, N multiples of the tile, K of 32.  With gridDim.z > 1 (split K), block z
 // covers k in [z*K/gz, (z+1)*K/gz) and adds its partial product into C
-// (zeroed beforehand) atomically.
-template <bool TA, bool TB, int TM, int TN>
+// (zeroed beforehand) atomically.  FUSED (gridDim.z == 1): instead of C, the
+// epilogue writes act(C + bias) as BF16 to epi.H and/or FP32 to epi.F -- the
+// arithmetic of bias_act_kernel on the same FP32 values, so the same bits,
+// without the FP32 product's round trip through memory.
+struct Epilogue {
+  const float *bias;
+  int relu;
+  __bf16 *H;
+  int ldh;
+  float *F;
+  int ldf;
+};
+
+template <bool TA, bool TB, int TM, int TN, bool FUSED>
 __global__ void __launch_bounds__(256) gemm_kernel(int M, int N, int K, const __bf16 *__restrict__ A,
                                                    int lda, const __bf16 *__restrict__ B, int ldb,
-                                                   float *__restrict__ C, int ldc) {
+                                                   float *__restrict__ C, int ldc, Epilogue epi) {
   constexpr int FM = TM / 32, FN = TN / 32; // 16-row fragments per wave
   __shared__ __attribute__((aligned(16))) __bf16 As[TM][40]; // [m][k], rows padded to 80 B
   __shared__ __attribute__((aligned(16))) __bf16 Bs[TN][40]; // [n][k]
@@ -98,10 +110,16 @@ __global__ void __launch_bounds__(256) gemm_kernel(int M, int N, int K, const __
       for (int q = 0; q < 4; ++q) {
         const int row = m0 + wm + i * 16 + (lane >> 4) * 4 + q;
         const int col = n0 + wn + j * 16 + (lane & 15);
-        if (gridDim.z == 1)
+        if (FUSED) {
+          float v = acc[i][j][q] + epi.bias[col];
+          if (epi.relu) v = v > 0.f ? v : 0.f;
+          if (epi.H) epi.H[(size_t)row * epi.ldh + col] = (__bf16)v;
+          if (epi.F) epi.F[(size_t)row * epi.ldf + col] = v;
+        } else if (gridDim.z == 1) {
           C[(size_t)row * ldc + col] = acc[i][j][q];
-        else
+        } else {
           atomicAdd(&C[(size_t)row * ldc + col], acc[i][j][q]);
+        }
       }
 }
 
@@ -392,13 +410,21 @@ unsigned nblk(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
 template <bool TA, bool TB>
 void launch_gemm(bool big, dim3 grid, hipStream_t s, int M, int N, int K, const __bf16 *a,
-                        int lda, const __bf16 *b, int ldb, float *C, int ldc) {
-  if (big)
-    hipLaunchKernelGGL((sg::gemm_kernel<TA, TB, 128, 128>), grid, dim3(256), 0, s, M, N, K, a, lda, b,
-                       ldb, C, ldc);
+                        int lda, const __bf16 *b, int ldb, float *C, int ldc,
+                        const sg::Epilogue *epi = nullptr) {
+  const sg::Epilogue none{nullptr, 0, nullptr, 0, nullptr, 0};
+  if (epi && big)
+    hipLaunchKernelGGL((sg::gemm_kernel<TA, TB, 128, 128, true>), grid, dim3(256), 0, s, M, N, K, a,
+                       lda, b, ldb, C, ldc, *epi);
+  else if (epi)
+    hipLaunchKernelGGL((sg::gemm_kernel<TA, TB, 64, 64, true>), grid, dim3(256), 0, s, M, N, K, a,
+                       lda, b, ldb, C, ldc, *epi);
+  else if (big)
+    hipLaunchKernelGGL((sg::gemm_kernel<TA, TB, 128, 128, false>), grid, dim3(256), 0, s, M, N, K, a,
+                       lda, b, ldb, C, ldc, none);
   else
-    hipLaunchKernelGGL((sg::gemm_kernel<TA, TB, 64, 64>), grid, dim3(256), 0, s, M, N, K, a, lda, b,
-                       ldb, C, ldc);
+    hipLaunchKernelGGL((sg::gemm_kernel<TA, TB, 64, 64, false>), grid, dim3(256), 0, s, M, N, K, a,
+                       lda, b, ldb, C, ldc, none);
 }
 
 } // namespace
@@ -427,6 +453,20 @@ int rebop_sg_gemm_bf16(int ta, int tb, int M, int N, int K, const void *A, int l
   else if (!ta && tb) launch_gemm<false, true>(big, grid, s, M, N, K, a, lda, b, ldb, C, ldc);
   else if (ta && !tb) launch_gemm<true, false>(big, grid, s, M, N, K, a, lda, b, ldb, C, ldc);
   else launch_gemm<true, true>(big, grid, s, M, N, K, a, lda, b, ldb, C, ldc);
+  return ret(hipGetLastError());
+}
+
+int rebop_sg_gemm_bias_act_bf16(int M, int N, int K, const void *A, int lda, const void *B,
+                                int ldb, const float *bias, int relu, void *H, int ldh, float *F,
+                                int ldf, void *stream) {
+  if (M % 64 || N % 64 || K % 32 || M <= 0 || N <= 0 || K <= 0) return 3;
+  if (!bias || (!H && !F)) return 3;
+  const bool big = M % 128 == 0 && N % 128 == 0 && (M / 128) * (N / 128) >= 512;
+  const int tile = big ? 128 : 64;
+  const dim3 grid(N / tile, M / tile, 1); // no split K: the epilogue needs the whole sum
+  const sg::Epilogue epi{bias, relu, (__bf16 *)H, ldh, F, ldf};
+  launch_gemm<false, false>(big, grid, (hipStream_t)stream, M, N, K, (const __bf16 *)A, lda,
+                            (const __bf16 *)B, ldb, nullptr, 0, &epi);
   return ret(hipGetLastError());
 }
 

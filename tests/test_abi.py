@@ -38,6 +38,21 @@ def test_library_exports_every_declared_symbol():
     assert declared == set(_native.SIGNATURES)
 
 
+def test_surrogate_library_exports_every_declared_symbol():
+    """include/rebop_surrogate.h: every declared entry point is exported by
+    librebop_surrogate.so and has a signature in the Python binding."""
+    from rebop_amd import surrogate
+
+    text = (ROOT / "include" / "rebop_surrogate.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    declared = set(re.findall(r"\b(rebop_sg_[a-z0-9_]+)\s*\(", text))
+    assert len(declared) >= 10
+    lib = surrogate.load()
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert declared == set(surrogate.SIGNATURES)
+
+
 def _has_gpu() -> bool:
     try:
         import torch

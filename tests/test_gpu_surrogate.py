@@ -48,6 +48,42 @@ def test_gemm_bf16_matches_fp64(sg, ta, tb, M, N, Kd):
     assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("M,N,Kd", [(128, 192, 64), (4096, 512, 512), (8192, 128, 512)])
+@pytest.mark.parametrize("relu", [True, False])
+def test_fused_gemm_epilogue_equals_gemm_then_bias_act(sg, M, N, Kd, relu):
+    """The fused launch writes the same bits as gemm followed by bias_act
+    (BF16 and FP32 outputs; 64 x 64 and 128 x 128 tiles) where gemm does not
+    split K; the third shape has too few output tiles, gemm splits K there
+    and adds partial sums atomically, so only FP32-rounding agreement holds."""
+    import torch
+
+    rng = np.random.default_rng(5)
+    A = _bf(rng.normal(size=(M, Kd)))
+    B = _bf(rng.normal(size=(Kd, N)) / np.sqrt(Kd))
+    bias = torch.as_tensor(rng.normal(size=N), dtype=torch.float32, device="cuda")
+    C = torch.empty((M, N), dtype=torch.float32, device="cuda")
+    H0 = torch.empty((M, N), dtype=torch.bfloat16, device="cuda")
+    F0 = torch.empty((M, N), dtype=torch.float32, device="cuda")
+    sg.gemm(A, B, C)
+    sg.bias_act(C, bias, relu, H=H0, F=F0)
+    H1 = torch.full((M, N), float("nan"), dtype=torch.bfloat16, device="cuda")
+    F1 = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
+    sg.gemm_bias_act(A, B, bias, relu, H=H1, F=F1)
+    torch.cuda.synchronize()
+    F2 = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
+    sg.gemm_bias_act(A, B, bias, relu, F=F2)  # one output only
+    torch.cuda.synchronize()
+    assert torch.equal(F1.view(torch.int32), F2.view(torch.int32))
+    split = (M // 64) * (N // 64) < 512 and not (M % 128 == 0 and N % 128 == 0 and (M // 128) * (N // 128) >= 512)
+    if not split:
+        assert torch.equal(H0.view(torch.int16), H1.view(torch.int16))
+        assert torch.equal(F0.view(torch.int32), F1.view(torch.int32))
+    else:
+        scale = float(F0.abs().max())
+        assert float((F0 - F1).abs().max()) <= 1e-5 * scale
+        assert float((H0.float() - H1.float()).abs().max()) <= 2.0 ** -7 * scale
+
+
 def test_nll_kernel_matches_fp64_reference(sg):
     import torch
 

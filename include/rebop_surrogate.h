@@ -36,14 +36,15 @@ int rebop_sg_gemm_bf16(int ta, int tb, int M, int N, int K, const void *A, int l
                        int ldb, float *C, int ldc, void *stream);
 
 /* H = bf16(act(A B + bias)) and/or F = act(A B + bias) (FP32) in one launch:
- * the product of rebop_sg_gemm_bf16 (A [M][K], B [K][N], no transposes)
+ * the product of rebop_sg_gemm_bf16 (A [M][K]; B [K][N], or with tb the
+ * [N][K] layout, which loads faster)
  * with rebop_sg_bias_act's epilogue applied to its FP32 accumulators, without
  * the FP32 product going through memory.  Never splits K, so where
  * rebop_sg_gemm_bf16 does not split it either (outputs of at least 512
  * tiles) the values are the same bits.  M, N multiples of 64, K of 32;
  * bias [N]; H or F may be NULL (not both).  Returns 3 on a bad shape or
  * argument. */
-int rebop_sg_gemm_bias_act_bf16(int M, int N, int K, const void *A, int lda, const void *B,
+int rebop_sg_gemm_bias_act_bf16(int tb, int M, int N, int K, const void *A, int lda, const void *B,
                                 int ldb, const float *bias, int relu, void *H, int ldh, float *F,
                                 int ldf, void *stream);
 
@@ -56,6 +57,10 @@ int rebop_sg_relu_grad(int M, int N, float *dH, int ldd, const void *H, int ldh,
                        void *stream);
 
 int rebop_sg_cast_bf16(size_t n, const float *x, void *y, void *stream);
+
+/* y[c][r] = bf16(x[r][c]): the transposed BF16 copy of an FP32 matrix
+ * (rows x cols, row-major). */
+int rebop_sg_transpose_bf16(int rows, int cols, const float *x, void *y, void *stream);
 
 /* out[n] = sum over m of X[m][n] (partial sums over row blocks are added
  * atomically, so the FP32 summation order varies from run to run). */

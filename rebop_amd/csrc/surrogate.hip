@@ -155,6 +155,14 @@ __global__ void relu_grad_kernel(int M, int N, float *__restrict__ dH, int ldd,
   dZ[(size_t)m * ldz + n] = (__bf16)g;
 }
 
+// y[c][r] = bf16(x[r][c]) for an R x C matrix
+__global__ void transpose_bf16_kernel(int R, int C, const float *__restrict__ x, __bf16 *__restrict__ y) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)R * C) return;
+  const int r = (int)(i / C), c = (int)(i % C);
+  y[(size_t)c * R + r] = (__bf16)x[i];
+}
+
 __global__ void cast_bf16_kernel(size_t n, const float *__restrict__ x, __bf16 *__restrict__ y) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = (__bf16)x[i];
@@ -456,7 +464,7 @@ int rebop_sg_gemm_bf16(int ta, int tb, int M, int N, int K, const void *A, int l
   return ret(hipGetLastError());
 }
 
-int rebop_sg_gemm_bias_act_bf16(int M, int N, int K, const void *A, int lda, const void *B,
+int rebop_sg_gemm_bias_act_bf16(int tb, int M, int N, int K, const void *A, int lda, const void *B,
                                 int ldb, const float *bias, int relu, void *H, int ldh, float *F,
                                 int ldf, void *stream) {
   if (M % 64 || N % 64 || K % 32 || M <= 0 || N <= 0 || K <= 0) return 3;
@@ -465,8 +473,12 @@ int rebop_sg_gemm_bias_act_bf16(int M, int N, int K, const void *A, int lda, con
   const int tile = big ? 128 : 64;
   const dim3 grid(N / tile, M / tile, 1); // no split K: the epilogue needs the whole sum
   const sg::Epilogue epi{bias, relu, (__bf16 *)H, ldh, F, ldf};
-  launch_gemm<false, false>(big, grid, (hipStream_t)stream, M, N, K, (const __bf16 *)A, lda,
-                            (const __bf16 *)B, ldb, nullptr, 0, &epi);
+  if (tb)
+    launch_gemm<false, true>(big, grid, (hipStream_t)stream, M, N, K, (const __bf16 *)A, lda,
+                             (const __bf16 *)B, ldb, nullptr, 0, &epi);
+  else
+    launch_gemm<false, false>(big, grid, (hipStream_t)stream, M, N, K, (const __bf16 *)A, lda,
+                              (const __bf16 *)B, ldb, nullptr, 0, &epi);
   return ret(hipGetLastError());
 }
 
@@ -481,6 +493,13 @@ int rebop_sg_relu_grad(int M, int N, float *dH, int ldd, const void *H, int ldh,
                        void *stream) {
   hipLaunchKernelGGL(sg::relu_grad_kernel, dim3(nblk((size_t)M * N, 256)), dim3(256), 0,
                      (hipStream_t)stream, M, N, dH, ldd, (const __bf16 *)H, ldh, (__bf16 *)dZ, ldz);
+  return ret(hipGetLastError());
+}
+
+int rebop_sg_transpose_bf16(int rows, int cols, const float *x, void *y, void *stream) {
+  if (rows <= 0 || cols <= 0) return 3;
+  hipLaunchKernelGGL(sg::transpose_bf16_kernel, dim3(nblk((size_t)rows * cols, 256)), dim3(256), 0,
+                     (hipStream_t)stream, rows, cols, x, (__bf16 *)y);
   return ret(hipGetLastError());
 }
 

@@ -48,7 +48,8 @@ _i, _v, _d, _f, _z = c_int, c_void_p, c_double, c_float, c_size_t
 SIGNATURES: dict[str, tuple] = {
     "rebop_sg_last_error": (ctypes.c_char_p, []),
     "rebop_sg_gemm_bf16": (_i, [_i, _i, _i, _i, _i, _v, _i, _v, _i, _v, _i, _v]),
-    "rebop_sg_gemm_bias_act_bf16": (_i, [_i, _i, _i, _v, _i, _v, _i, _v, _i, _v, _i, _v, _i, _v]),
+    "rebop_sg_gemm_bias_act_bf16": (_i, [_i, _i, _i, _i, _v, _i, _v, _i, _v, _i, _v, _i, _v, _i, _v]),
+    "rebop_sg_transpose_bf16": (_i, [_i, _i, _v, _v, _v]),
     "rebop_sg_bias_act": (_i, [_i, _i, _v, _i, _v, _i, _v, _i, _v, _i, _v]),
     "rebop_sg_relu_grad": (_i, [_i, _i, _v, _i, _v, _i, _v, _i, _v]),
     "rebop_sg_cast_bf16": (_i, [_z, _v, _v, _v]),
@@ -97,12 +98,13 @@ def gemm(A, B, C, ta=False, tb=False, stream=None):
                                   _p(C), C.shape[1], stream))
 
 
-def gemm_bias_act(A, B, bias, relu, H=None, F=None, stream=None):
-    """H = bf16(act(A B + bias)) and/or F = act(A B + bias) in one launch
-    (the values of gemm followed by bias_act, without the FP32 product)."""
+def gemm_bias_act(A, B, bias, relu, H=None, F=None, tb=False, stream=None):
+    """H = bf16(act(A op(B) + bias)) and/or F = act(A op(B) + bias) in one
+    launch (the values of gemm followed by bias_act, without the FP32
+    product); tb: B is stored [N][K]."""
     out = H if H is not None else F
     M, N = out.shape
-    _ck(load().rebop_sg_gemm_bias_act_bf16(M, N, A.shape[1], _p(A), A.shape[1], _p(B), B.shape[1],
+    _ck(load().rebop_sg_gemm_bias_act_bf16(int(tb), M, N, A.shape[1], _p(A), A.shape[1], _p(B), B.shape[1],
                                            _p(bias), int(relu),
                                            _p(H) if H is not None else None, H.shape[1] if H is not None else 0,
                                            _p(F) if F is not None else None, F.shape[1] if F is not None else 0,
@@ -129,6 +131,12 @@ def cast_bf16(x, y, stream=None):
 def colsum(X, out, stream=None):
     M, N = X.shape
     _ck(load().rebop_sg_colsum(M, N, _p(X), X.shape[1], _p(out), stream))
+
+
+def transpose_bf16(x, y, stream=None):
+    """y = bf16(x) transposed (x float32 [R][C], y bfloat16 [C][R])."""
+    R, C = x.shape
+    _ck(load().rebop_sg_transpose_bf16(R, C, _p(x), _p(y), stream))
 
 
 def adam(w, g, m, v, wb, lr, b1, b2, eps, t, stream=None):
@@ -221,7 +229,10 @@ class MLP:
         g = torch.Generator(device="cpu").manual_seed(seed)
         dims = [IN] + [width] * (depth - 1) + [OUT]
         self.dims = dims
-        self.W, self.b, self.Wb = [], [], []
+        # FP32 master weights W [in][out], their BF16 copy Wb (the backward's
+        # operand) and its transpose Wt [out][in] (the forward's: the layout
+        # the GEMM loads fastest)
+        self.W, self.b, self.Wb, self.Wt = [], [], [], []
         for l in range(depth):
             fan_in = NF if l == 0 else dims[l]
             w = torch.randn(dims[l], dims[l + 1], generator=g, dtype=torch.float64) * math.sqrt(2.0 / fan_in)
@@ -239,6 +250,7 @@ class MLP:
             self.W.append(w.float().cuda())
             self.b.append(b.float().cuda())
             self.Wb.append(self.W[-1].to(torch.bfloat16))
+            self.Wt.append(self.W[-1].t().contiguous().to(torch.bfloat16))
         self.mW = [torch.zeros_like(w) for w in self.W]
         self.vW = [torch.zeros_like(w) for w in self.W]
         self.mb = [torch.zeros_like(b) for b in self.b]
@@ -260,11 +272,11 @@ class MLP:
             # gemm + bias_act)
             if l < self.depth - 1:
                 h = torch.empty((B, self.dims[l + 1]), dtype=torch.bfloat16, device="cuda")
-                gemm_bias_act(H[-1], self.Wb[l], self.b[l], True, H=h)
+                gemm_bias_act(H[-1], self.Wt[l], self.b[l], True, H=h, tb=True)
                 H.append(h)
             else:
                 head = torch.empty((B, self.dims[l + 1]), dtype=torch.float32, device="cuda")
-                gemm_bias_act(H[-1], self.Wb[l], self.b[l], False, F=head)
+                gemm_bias_act(H[-1], self.Wt[l], self.b[l], False, F=head, tb=True)
         if keep:
             self._H = H
         return head
@@ -297,6 +309,7 @@ class MLP:
         self.t += 1
         for l in range(self.depth):
             adam(self.W[l], dW[l], self.mW[l], self.vW[l], self.Wb[l], lr, b1, b2, eps, self.t)
+            transpose_bf16(self.W[l], self.Wt[l])
             adam(self.b[l], db[l], self.mb[l], self.vb[l], None, lr, b1, b2, eps, self.t)
 
 

@@ -72,8 +72,13 @@ def test_fused_gemm_epilogue_equals_gemm_then_bias_act(sg, M, N, Kd, relu):
     torch.cuda.synchronize()
     F2 = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
     sg.gemm_bias_act(A, B, bias, relu, F=F2)  # one output only
+    H3 = torch.full((M, N), float("nan"), dtype=torch.bfloat16, device="cuda")
+    F3 = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
+    sg.gemm_bias_act(A, B.t().contiguous(), bias, relu, H=H3, F=F3, tb=True)  # B stored [N][K]
     torch.cuda.synchronize()
     assert torch.equal(F1.view(torch.int32), F2.view(torch.int32))
+    assert torch.equal(F1.view(torch.int32), F3.view(torch.int32))
+    assert torch.equal(H1.view(torch.int16), H3.view(torch.int16))
     split = (M // 64) * (N // 64) < 512 and not (M % 128 == 0 and N % 128 == 0 and (M // 128) * (N // 128) >= 512)
     if not split:
         assert torch.equal(H0.view(torch.int16), H1.view(torch.int16))
@@ -82,6 +87,16 @@ def test_fused_gemm_epilogue_equals_gemm_then_bias_act(sg, M, N, Kd, relu):
         scale = float(F0.abs().max())
         assert float((F0 - F1).abs().max()) <= 1e-5 * scale
         assert float((H0.float() - H1.float()).abs().max()) <= 2.0 ** -7 * scale
+
+
+def test_transposed_bf16_copy(sg):
+    import torch
+
+    x = torch.randn(192, 320, dtype=torch.float32, device="cuda")
+    y = torch.empty(320, 192, dtype=torch.bfloat16, device="cuda")
+    sg.transpose_bf16(x, y)
+    torch.cuda.synchronize()
+    assert torch.equal(y.view(torch.int16), x.t().contiguous().to(torch.bfloat16).view(torch.int16))
 
 
 def test_nll_kernel_matches_fp64_reference(sg):

@@ -123,6 +123,99 @@ __global__ void __launch_bounds__(256) gemm_kernel(int M, int N, int K, const __
       }
 }
 
+// The same product for A [M][K] and B stored [N][K] (both k-contiguous, as
+// the forward pass and the backward's dX use them), software-pipelined: the
+// next 32-wide k slice is loaded from global memory into registers while the
+// matrix cores work on the current one, which sits in one of two LDS
+// buffers, so one barrier per slice separates the stores into the other
+// buffer from its reads.  Same tiles, fragments and summation order per
+// output as gemm_kernel<false, true>.
+template <int TM, int TN, bool FUSED>
+__global__ void __launch_bounds__(256) gemm_nt_kernel(int M, int N, int K, const __bf16 *__restrict__ A,
+                                                      int lda, const __bf16 *__restrict__ B, int ldb,
+                                                      float *__restrict__ C, int ldc, Epilogue epi) {
+  constexpr int FM = TM / 32, FN = TN / 32, PA = TM / 64, PB = TN / 64;
+  __shared__ __attribute__((aligned(16))) __bf16 As[2][TM][40];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][TN][40];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const int wm = (wave >> 1) * (TM / 2), wn = (wave & 1) * (TN / 2);
+  floatx4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  const int ksplit = K / (int)gridDim.z;
+  const int kbeg = (int)blockIdx.z * ksplit, kend = kbeg + ksplit;
+  // thread tid moves the 8-element chunks tid + 256 p: row (chunk >> 2), k (chunk & 3) * 8
+  bf16x8 ra[PA], rb[PB];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int p = 0; p < PA; ++p) {
+      const int idx = tid + 256 * p, r = idx >> 2, c = (idx & 3) * 8;
+      ra[p] = *(const bf16x8 *)&A[(size_t)(m0 + r) * lda + k0 + c];
+    }
+#pragma unroll
+    for (int p = 0; p < PB; ++p) {
+      const int idx = tid + 256 * p, r = idx >> 2, c = (idx & 3) * 8;
+      rb[p] = *(const bf16x8 *)&B[(size_t)(n0 + r) * ldb + k0 + c];
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < PA; ++p) {
+      const int idx = tid + 256 * p, r = idx >> 2, c = (idx & 3) * 8;
+      *(bf16x8 *)&As[buf][r][c] = ra[p];
+    }
+#pragma unroll
+    for (int p = 0; p < PB; ++p) {
+      const int idx = tid + 256 * p, r = idx >> 2, c = (idx & 3) * 8;
+      *(bf16x8 *)&Bs[buf][r][c] = rb[p];
+    }
+  };
+  gload(kbeg);
+  lstore(0);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = kbeg; k0 < kend; k0 += 32) {
+    const bool more = k0 + 32 < kend;
+    if (more) gload(k0 + 32); // in flight during the MFMAs below
+    bf16x8 a[FM], b[FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i) a[i] = *(const bf16x8 *)&As[buf][wm + i * 16 + (lane & 15)][(lane >> 4) * 8];
+#pragma unroll
+    for (int j = 0; j < FN; ++j) b[j] = *(const bf16x8 *)&Bs[buf][wn + j * 16 + (lane & 15)][(lane >> 4) * 8];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    if (more) lstore(buf ^ 1); // that buffer's reads ended at the last barrier
+    __syncthreads();
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = m0 + wm + i * 16 + (lane >> 4) * 4 + q;
+        const int col = n0 + wn + j * 16 + (lane & 15);
+        if (FUSED) {
+          float v = acc[i][j][q] + epi.bias[col];
+          if (epi.relu) v = v > 0.f ? v : 0.f;
+          if (epi.H) epi.H[(size_t)row * epi.ldh + col] = (__bf16)v;
+          if (epi.F) epi.F[(size_t)row * epi.ldf + col] = v;
+        } else if (gridDim.z == 1) {
+          C[(size_t)row * ldc + col] = acc[i][j][q];
+        } else {
+          atomicAdd(&C[(size_t)row * ldc + col], acc[i][j][q]);
+        }
+      }
+}
+
 __global__ void zero_kernel(int M, int N, float *__restrict__ C, int ldc) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < (size_t)M * N) C[(i / N) * ldc + i % N] = 0.f;
@@ -421,6 +514,21 @@ void launch_gemm(bool big, dim3 grid, hipStream_t s, int M, int N, int K, const 
                         int lda, const __bf16 *b, int ldb, float *C, int ldc,
                         const sg::Epilogue *epi = nullptr) {
   const sg::Epilogue none{nullptr, 0, nullptr, 0, nullptr, 0};
+  if (!TA && TB) { // k-contiguous operands: the pipelined kernel
+    if (big && epi)
+      hipLaunchKernelGGL((sg::gemm_nt_kernel<128, 128, true>), grid, dim3(256), 0, s, M, N, K, a, lda,
+                         b, ldb, C, ldc, *epi);
+    else if (epi)
+      hipLaunchKernelGGL((sg::gemm_nt_kernel<64, 64, true>), grid, dim3(256), 0, s, M, N, K, a, lda,
+                         b, ldb, C, ldc, *epi);
+    else if (big)
+      hipLaunchKernelGGL((sg::gemm_nt_kernel<128, 128, false>), grid, dim3(256), 0, s, M, N, K, a,
+                         lda, b, ldb, C, ldc, none);
+    else
+      hipLaunchKernelGGL((sg::gemm_nt_kernel<64, 64, false>), grid, dim3(256), 0, s, M, N, K, a, lda,
+                         b, ldb, C, ldc, none);
+    return;
+  }
   if (epi && big)
     hipLaunchKernelGGL((sg::gemm_kernel<TA, TB, 128, 128, true>), grid, dim3(256), 0, s, M, N, K, a,
                        lda, b, ldb, C, ldc, *epi);

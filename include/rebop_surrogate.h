@@ -89,6 +89,36 @@ int rebop_sg_features(int B, const uint32_t *item_point, const uint32_t *item_sa
 int rebop_sg_head_offsets(int B, int S, int K, float *head, int ldh, const float *bound,
                           void *stream);
 
+/* Wp = the k-slice-major BF16 copy of W [K][N] (FP32, row-major) that the
+ * fused forward pass reads: Wp[k / 32][n][k % 32] = bf16(W[k][n]), so the
+ * weights of 16 columns over one 32-wide k slice are 1 KB of consecutive
+ * bytes.  K a multiple of 32. */
+int rebop_sg_pack_kslices_bf16(int K, int N, const float *W, void *Wp, void *stream);
+
+/* The whole network's forward pass in one launch (inference): layer l maps
+ * width dims[l] to dims[l+1] with weights Wp[l] in the layout of
+ * rebop_sg_pack_kslices_bf16 (BF16) and bias[l] (FP32); layers 0..depth-2
+ * apply ReLU and round to BF16, the last writes out[M][dims[depth]] (FP32,
+ * ldo).  Same values, bit for bit, as rebop_sg_gemm_bias_act_bf16 applied
+ * layer by layer, with the activations kept on chip.  depth 1..8; dims[0] a
+ * multiple of 64 up to 512, the others 128, 256 or 512; X [M][ldx] BF16
+ * (ldx >= dims[0], a multiple of 8); ldo >= dims[depth], a multiple of 4;
+ * X, out, Wp[l] and bias[l] 16-byte aligned.  Wp and bias are host arrays of
+ * device pointers, dims a host array. */
+int rebop_sg_mlp_forward_bf16(int M, const void *X, int ldx, int depth, const int *dims,
+                              const void *const *Wp, const float *const *bias, float *out, int ldo,
+                              void *stream);
+
+/* rebop_sg_features, rebop_sg_mlp_forward_bf16 and rebop_sg_head_offsets
+ * (S = 4, K = REBOP_SG_COMPONENTS) in one launch: head[b] = the mixture
+ * parameters of item b (point item_point[b], sample item_sample[b]), the same
+ * bits as the three calls.  dims[0] >= REBOP_SG_FEATURES (inputs past the
+ * features are zero), dims[depth] >= 12 K, ldh >= dims[depth]. */
+int rebop_sg_infer(int B, const uint32_t *item_point, const uint32_t *item_sample,
+                   const double *rates, const int64_t *x0, double tmax, int nb_steps, int depth,
+                   const int *dims, const void *const *Wp, const float *const *bias, float *head,
+                   int ldh, void *stream);
+
 /* Negative log-likelihood of J SSA samples per item under the predicted
  * per-species K-component negative-binomial mixtures (K = REBOP_SG_COMPONENTS),
  * and its gradient: grad[b][...] = scale * d NLL / d head[b][...];

@@ -256,6 +256,14 @@ __global__ void transpose_bf16_kernel(int R, int C, const float *__restrict__ x,
   y[(size_t)c * R + r] = (__bf16)x[i];
 }
 
+// Wp[k / 32][n][k % 32] = bf16(W[k][n]) for W [K][N] (K a multiple of 32)
+__global__ void pack_kslices_kernel(int K, int N, const float *__restrict__ W, __bf16 *__restrict__ Wp) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)K * N) return;
+  const int kk = (int)(i % 32), n = (int)((i / 32) % N), k = (int)(i / (32 * (size_t)N)) * 32 + kk;
+  Wp[i] = (__bf16)W[(size_t)k * N + n];
+}
+
 __global__ void cast_bf16_kernel(size_t n, const float *__restrict__ x, __bf16 *__restrict__ y) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = (__bf16)x[i];
@@ -300,28 +308,22 @@ __global__ void adam_kernel(size_t n, float *__restrict__ w, const float *__rest
   if (wb) wb[i] = (__bf16)wi;
 }
 
-// Inputs of item b: (point p, sample i) -> [log10 k (3), E0, S0, t] scaled to
-// about [-1, 1]; columns 6..ldx-1 are zero.
-// Input features of item b (columns 0..REBOP_SG_FEATURES-1, the rest zero),
+// Inputs of item b: (point p, sample i) -> the REBOP_SG_FEATURES features,
 // scaled to about [-1, 1]: log10 of the rate constants, E0, S0, t, their
 // logs, and the logs of the kinetics' dimensionless groups (kcat t, kf E0 t,
 // kf S0 t, K_M / S0 with K_M = (kb + kcat) / kf, kb / kcat, E0 / S0,
-// kcat E0 t / S0).  bound[b][s] (nullable) = the count that bounds species s
-// (E0 for E and ES, S0 for S and P; exact in FP32).
-__global__ void features_kernel(int B, const uint32_t *__restrict__ item_point,
-                                const uint32_t *__restrict__ item_sample, const double *__restrict__ rates,
-                                const int64_t *__restrict__ x0, double tmax, int nb_steps,
-                                __bf16 *__restrict__ X, int ldx, float *__restrict__ bound) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  const uint32_t p = item_point[b], i = item_sample[b];
+// kcat E0 t / S0); e0, s0 = the counts that bound the species (E0 for E and
+// ES, S0 for S and P; exact in FP32).
+__device__ __forceinline__ void item_features(uint32_t p, uint32_t i, const double *__restrict__ rates,
+                                              const int64_t *__restrict__ x0, double tmax, int nb_steps,
+                                              float (&f)[REBOP_SG_FEATURES], float &e0f, float &s0f) {
+#pragma clang fp contract(off)
   const double kf = rates[3 * p + 0], kb = rates[3 * p + 1], kc = rates[3 * p + 2];
   const double e0 = (double)x0[4 * p + 0], s0 = (double)x0[4 * p + 1];
   const double t = tmax * (double)i / (double)nb_steps;
   const double lf = log10(kf), lb = log10(kb), lc = log10(kc);
   const double le = log10(e0), ls = log10(s0), lt = log10(t);
   const double lkm = log10(kb + kc) - lf;
-  float f[REBOP_SG_FEATURES];
   f[0] = (float)((lf + 1.0) * 0.5);
   f[1] = (float)((lb + 1.0) * 0.5);
   f[2] = (float)((lc + 1.0) * 0.5);
@@ -338,12 +340,26 @@ __global__ void features_kernel(int B, const uint32_t *__restrict__ item_point,
   f[13] = (float)((lb - lc) / 4.0);
   f[14] = (float)((le - ls + 0.7) / 1.3);
   f[15] = (float)((lc + le + lt - ls + 0.75) / 4.05);
+  e0f = (float)e0;
+  s0f = (float)s0;
+}
+
+// X[b][0..ldx) = item b's features, then zeros; bound[b][s] (nullable) = the
+// count that bounds species s (E = E0 - ES, S <= S0, ES <= E0, P <= S0).
+__global__ void features_kernel(int B, const uint32_t *__restrict__ item_point,
+                                const uint32_t *__restrict__ item_sample, const double *__restrict__ rates,
+                                const int64_t *__restrict__ x0, double tmax, int nb_steps,
+                                __bf16 *__restrict__ X, int ldx, float *__restrict__ bound) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  float f[REBOP_SG_FEATURES], e0, s0;
+  item_features(item_point[b], item_sample[b], rates, x0, tmax, nb_steps, f, e0, s0);
   for (int c = 0; c < ldx; ++c) X[(size_t)b * ldx + c] = (__bf16)(c < REBOP_SG_FEATURES ? f[c] : 0.f);
   if (bound) {
-    bound[4 * b + 0] = (float)e0; // E  = E0 - ES
-    bound[4 * b + 1] = (float)s0; // S <= S0
-    bound[4 * b + 2] = (float)e0; // ES <= E0
-    bound[4 * b + 3] = (float)s0; // P <= S0
+    bound[4 * b + 0] = e0;
+    bound[4 * b + 1] = s0;
+    bound[4 * b + 2] = e0;
+    bound[4 * b + 3] = s0;
   }
 }
 
@@ -356,6 +372,202 @@ __global__ void head_offsets_kernel(int B, int S, int K, float *__restrict__ hea
   if (idx >= B * S * half) return;
   const int b = idx / (S * half), s = (idx / half) % S, c = idx % half;
   head[(size_t)b * ldh + (size_t)s * 3 * K + K + c] += logf(bound[(size_t)b * S + s]);
+}
+
+// ---------------------------------------------------------------------------
+// Inference through the whole network in one launch.  A workgroup of eight
+// waves takes 128 items, whose activations stay in LDS (128 rows of up to 512
+// BF16, padded to 1040 B against bank conflicts) from the input layer to the
+// last hidden layer: HBM sees the inputs and the head only.  Wave w computes
+// all 128 rows and columns (N / 8) w .. +N/8 of each layer, so each weight
+// fragment is fetched once per workgroup.  The weights come straight from
+// global memory (every workgroup reads the same ones, so they stay in L2) in
+// the k-slice-major layout Wp[k / 32][n][k % 32] (rebop_sg_pack_kslices_bf16):
+// a wave's fragment of one 32-wide slice is 1 KB of consecutive bytes.  The
+// MFMA takes the weights as A and the activations as B, so a lane's four
+// outputs are four consecutive columns of one row (one 8-byte LDS store per
+// fragment).  Per output, the arithmetic is gemm_nt_kernel's: the same
+// 16x16x32 MFMAs over k slices 0, 32, 64, ... (a product of two BF16 values
+// is exact in FP32, so which operand is A does not matter), then + bias,
+// ReLU and the BF16 cast -- the bits of the layer-by-layer forward pass.
+// FEAT: the input layer is item_features of items m0.. (columns >=
+// REBOP_SG_FEATURES zero) and the last layer adds head_offsets_kernel's
+// ln bound terms, so one launch turns (point, sample) items into mixture
+// parameters; otherwise X [M][ldx] is the input.
+constexpr int MLP_BM = 128, MLP_LDH = 520, MLP_MAXW = 512, MLP_MAXL = 8;
+
+struct MlpNet {
+  const __bf16 *Wp[MLP_MAXL]; // layer l: [dims[l] / 32][dims[l+1]][32]
+  const float *bias[MLP_MAXL];
+  int dims[MLP_MAXL + 1]; // dims[0] multiple of 64 up to 512, the others 128, 256 or 512
+  int depth;
+};
+
+struct MlpItems {
+  const uint32_t *item_point, *item_sample;
+  const double *rates;
+  const int64_t *x0;
+  double tmax;
+  int nb_steps;
+};
+
+// NB: 16-column fragments per wave (N = 128 NB); KT: the input width when it
+// is known at compile time (the k loop is then unrolled whole, which also
+// spares the loop header's wait for every outstanding load), else 0.
+template <int NB, int KT>
+__device__ __forceinline__ void mlp_layer(const __bf16 *__restrict__ Wp, const float *__restrict__ bias,
+                                          int Kr, bool last, __bf16 (*Hs)[MLP_LDH], const float (*lnb)[4],
+                                          int m0, int M, float *__restrict__ out, int ldo) {
+  constexpr int N = 128 * NB;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int K = KT ? KT : Kr, n0 = wave * 16 * NB;
+  // fragment c of the slice at k: Wp[k N + lane_off + 512 c .. +8]
+  const uint32_t lane_off = (uint32_t)(n0 + (lane & 15)) * 32u + (uint32_t)(lane >> 4) * 8u;
+  auto wload = [&](bf16x8 &dst, int c, int k) {
+    // a wave-uniform base (scalar registers) plus the lane's fixed offset,
+    // so the unrolled loop needs no vector register per address
+    const __bf16 *base = Wp + ((size_t)k * N + (size_t)c * 512);
+    dst = *(const bf16x8 *)(base + lane_off);
+  };
+  // Row fragment r at k: byte offset hb[r / 4] + (16 (r % 4) MLP_LDH + k) 2
+  // from Hs.  Rows 64.. get a base register of their own (kept opaque to the
+  // compiler) so that every offset fits a DS instruction's 16-bit immediate:
+  // one address register for the unrolled loop instead of one per load.
+  const uint32_t hb0 = (uint32_t)((lane & 15) * MLP_LDH + (lane >> 4) * 8) * 2u;
+  uint32_t hb1 = hb0 + 64u * MLP_LDH * 2u;
+  asm volatile("" : "+v"(hb1));
+  auto hload = [&](bf16x8 &dst, int r, int k) {
+    const uint32_t off = (r < 4 ? hb0 : hb1) + (uint32_t)((r & 3) * 16 * MLP_LDH + k) * 2u;
+    dst = *(const bf16x8 *)((const char *)&Hs[0][0] + off);
+  };
+  floatx4 acc[NB][8];
+#pragma unroll
+  for (int c = 0; c < NB; ++c)
+#pragma unroll
+    for (int r = 0; r < 8; ++r) acc[c][r] = floatx4{0.f, 0.f, 0.f, 0.f};
+  // Software pipeline, fixed with scheduling barriers (left alone, the
+  // compiler sinks each load to just before its use): row fragment r of the
+  // next slice loads as soon as the MFMAs of this slice have read slot r, and
+  // the next slice's weight fragments (a second set of registers) load after
+  // the first row block, so every load has most of a slice to arrive in.
+  bf16x8 wa[NB], wb[NB], h[8];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) wload(wa[c], c, 0);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) hload(h[r], r, 0);
+  auto slice = [&](const bf16x8(&w)[NB], bf16x8(&wn)[NB], int kn) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c) acc[c][r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[c], h[r], acc[c][r], 0, 0, 0);
+      hload(h[r], r, kn);
+      if (r == 0) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) wload(wn[c], c, kn);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  };
+  constexpr int UNROLL = KT ? KT / 64 : 1;
+#pragma unroll UNROLL
+  for (int k0 = 0; k0 < K; k0 += 64) { // K is a multiple of 64
+    slice(wa, wb, k0 + 32);
+    // the last slice's loads fetch that slice again (no branch in the loop)
+    slice(wb, wa, k0 + 64 < K ? k0 + 64 : k0 + 32);
+  }
+  __syncthreads(); // every wave has read this layer's input
+#pragma unroll
+  for (int c = 0; c < NB; ++c) {
+    const int n = n0 + c * 16 + (lane >> 4) * 4;
+    const floatx4 b = *(const floatx4 *)&bias[n];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int m = r * 16 + (lane & 15);
+      floatx4 v;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = acc[c][r][q] + b[q];
+      if (!last) {
+        __bf16 hv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) hv[q] = (__bf16)(v[q] > 0.f ? v[q] : 0.f);
+        __builtin_memcpy(&Hs[m][n], hv, sizeof hv);
+      } else if (m0 + m < M) {
+        if (lnb) {
+          // head_offsets_kernel: + ln bound[s] on the negative-binomial log
+          // means, columns s*3K + K + c for c < K/2
+          constexpr int KC = REBOP_SG_COMPONENTS;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int s = (n + q) / (3 * KC), j = (n + q) % (3 * KC);
+            if (s < 4 && j >= KC && j < KC + KC / 2) v[q] += lnb[m][s];
+          }
+        }
+        *(floatx4 *)&out[(size_t)(m0 + m) * ldo + n] = v;
+      }
+    }
+  }
+  __syncthreads(); // the layer's output is whole before the next layer reads it
+}
+
+template <bool FEAT>
+__global__ void __launch_bounds__(512) mlp_infer_kernel(int M, const __bf16 *__restrict__ X, int ldx,
+                                                        MlpItems items, MlpNet net,
+                                                        float *__restrict__ out, int ldo) {
+  __shared__ __attribute__((aligned(16))) __bf16 Hs[MLP_BM][MLP_LDH];
+  __shared__ float lnb[FEAT ? MLP_BM : 1][4];
+  const int tid = threadIdx.x, m0 = blockIdx.x * MLP_BM, K0 = net.dims[0];
+  if (FEAT) {
+    if (tid < MLP_BM) {
+      float f[REBOP_SG_FEATURES], e0 = 1.f, s0 = 1.f;
+#pragma unroll
+      for (int c = 0; c < REBOP_SG_FEATURES; ++c) f[c] = 0.f;
+      if (m0 + tid < M)
+        item_features(items.item_point[m0 + tid], items.item_sample[m0 + tid], items.rates, items.x0,
+                      items.tmax, items.nb_steps, f, e0, s0);
+      for (int c = 0; c < K0; ++c) Hs[tid][c] = (__bf16)(c < REBOP_SG_FEATURES ? f[c] : 0.f);
+      const float le = logf(e0), ls = logf(s0);
+      lnb[tid][0] = le;
+      lnb[tid][1] = ls;
+      lnb[tid][2] = le;
+      lnb[tid][3] = ls;
+    }
+  } else {
+    // rows m0.. of X, 8 columns per thread and pass; rows past M read as zero
+    const int per_row = K0 / 8;
+    for (int idx = tid; idx < MLP_BM * per_row; idx += 512) {
+      const int r = idx / per_row, c = (idx % per_row) * 8;
+      bf16x8 v = {};
+      if (m0 + r < M) v = *(const bf16x8 *)&X[(size_t)(m0 + r) * ldx + c];
+      *(bf16x8 *)&Hs[r][c] = v;
+    }
+  }
+  __syncthreads();
+  const float(*lp)[4] = FEAT ? lnb : nullptr;
+  for (int l = 0; l < net.depth; ++l) {
+    // layer l's table entries, selected without indexing the argument block
+    // (a runtime index would copy it to scratch memory)
+    const __bf16 *W = net.Wp[0];
+    const float *bias = net.bias[0];
+    int K = net.dims[0], N = net.dims[1];
+#pragma unroll
+    for (int j = 1; j < MLP_MAXL; ++j)
+      if (j == l) {
+        W = net.Wp[j];
+        bias = net.bias[j];
+        K = net.dims[j];
+        N = net.dims[j + 1];
+      }
+    const bool last = l == net.depth - 1;
+    if (K == 512 && N == 512) {
+      mlp_layer<4, 512>(W, bias, K, last, Hs, lp, m0, M, out, ldo);
+      continue;
+    }
+    switch (N) {
+    case 128: mlp_layer<1, 0>(W, bias, K, last, Hs, lp, m0, M, out, ldo); break;
+    case 256: mlp_layer<2, 0>(W, bias, K, last, Hs, lp, m0, M, out, ldo); break;
+    default: mlp_layer<4, 0>(W, bias, K, last, Hs, lp, m0, M, out, ldo); break;
+    }
+  }
 }
 
 // ---- special functions (FP64): recurrence to z >= 10, then the asymptotic
@@ -509,6 +721,25 @@ int ret(hipError_t e) {
 }
 unsigned nblk(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
+// The layer table of the fused forward pass; 3 if the shapes are outside
+// what mlp_infer_kernel handles.
+int mlp_net(int depth, const int *dims, const void *const *Wp, const float *const *bias,
+            sg::MlpNet &net) {
+  if (depth < 1 || depth > sg::MLP_MAXL || !dims || !Wp || !bias) return 3;
+  if (dims[0] <= 0 || dims[0] % 64 || dims[0] > sg::MLP_MAXW) return 3;
+  net.depth = depth;
+  net.dims[0] = dims[0];
+  for (int l = 0; l < depth; ++l) {
+    const int n = dims[l + 1];
+    if (n != 128 && n != 256 && n != 512) return 3;
+    if (!Wp[l] || !bias[l] || (uintptr_t)Wp[l] % 16 || (uintptr_t)bias[l] % 16) return 3;
+    net.dims[l + 1] = n;
+    net.Wp[l] = (const __bf16 *)Wp[l];
+    net.bias[l] = bias[l];
+  }
+  return 0;
+}
+
 template <bool TA, bool TB>
 void launch_gemm(bool big, dim3 grid, hipStream_t s, int M, int N, int K, const __bf16 *a,
                         int lda, const __bf16 *b, int ldb, float *C, int ldc,
@@ -647,6 +878,41 @@ int rebop_sg_head_offsets(int B, int S, int K, float *head, int ldh, const float
   if (ldh < S * 3 * K) return 3;
   hipLaunchKernelGGL(sg::head_offsets_kernel, dim3(nblk((size_t)B * S * K, 256)), dim3(256), 0,
                      (hipStream_t)stream, B, S, K, head, ldh, bound);
+  return ret(hipGetLastError());
+}
+
+int rebop_sg_pack_kslices_bf16(int K, int N, const float *W, void *Wp, void *stream) {
+  if (K <= 0 || N <= 0 || K % 32 || !W || !Wp) return 3;
+  hipLaunchKernelGGL(sg::pack_kslices_kernel, dim3(nblk((size_t)K * N, 256)), dim3(256), 0,
+                     (hipStream_t)stream, K, N, W, (__bf16 *)Wp);
+  return ret(hipGetLastError());
+}
+
+int rebop_sg_mlp_forward_bf16(int M, const void *X, int ldx, int depth, const int *dims,
+                              const void *const *Wp, const float *const *bias, float *out, int ldo,
+                              void *stream) {
+  sg::MlpNet net;
+  if (mlp_net(depth, dims, Wp, bias, net) || M <= 0 || !X || !out || (uintptr_t)X % 16 ||
+      (uintptr_t)out % 16 || ldx < dims[0] || ldx % 8 || ldo < dims[depth] || ldo % 4)
+    return 3;
+  const sg::MlpItems none{nullptr, nullptr, nullptr, nullptr, 0.0, 1};
+  hipLaunchKernelGGL(sg::mlp_infer_kernel<false>, dim3(nblk((size_t)M, sg::MLP_BM)), dim3(512), 0,
+                     (hipStream_t)stream, M, (const __bf16 *)X, ldx, none, net, out, ldo);
+  return ret(hipGetLastError());
+}
+
+int rebop_sg_infer(int B, const uint32_t *item_point, const uint32_t *item_sample,
+                   const double *rates, const int64_t *x0, double tmax, int nb_steps, int depth,
+                   const int *dims, const void *const *Wp, const float *const *bias, float *head,
+                   int ldh, void *stream) {
+  sg::MlpNet net;
+  if (mlp_net(depth, dims, Wp, bias, net) || B <= 0 || !item_point || !item_sample || !rates ||
+      !x0 || !head || (uintptr_t)head % 16 || dims[0] < REBOP_SG_FEATURES ||
+      dims[depth] < 4 * 3 * REBOP_SG_COMPONENTS || ldh < dims[depth] || ldh % 4 || nb_steps <= 0)
+    return 3;
+  const sg::MlpItems items{item_point, item_sample, rates, x0, tmax, nb_steps};
+  hipLaunchKernelGGL(sg::mlp_infer_kernel<true>, dim3(nblk((size_t)B, sg::MLP_BM)), dim3(512), 0,
+                     (hipStream_t)stream, B, nullptr, 0, items, net, head, ldh);
   return ret(hipGetLastError());
 }
 

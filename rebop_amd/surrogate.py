@@ -57,6 +57,9 @@ SIGNATURES: dict[str, tuple] = {
     "rebop_sg_adam": (_i, [_z, _v, _v, _v, _v, _v, _f, _f, _f, _f, _f, _f, _v]),
     "rebop_sg_features": (_i, [_i, _v, _v, _v, _v, _d, _i, _v, _i, _v, _v]),
     "rebop_sg_head_offsets": (_i, [_i, _i, _i, _v, _i, _v, _v]),
+    "rebop_sg_pack_kslices_bf16": (_i, [_i, _i, _v, _v, _v]),
+    "rebop_sg_mlp_forward_bf16": (_i, [_i, _v, _i, _i, _v, _v, _v, _v, _i, _v]),
+    "rebop_sg_infer": (_i, [_i, _v, _v, _v, _v, _d, _i, _i, _v, _v, _v, _v, _i, _v]),
     "rebop_sg_nb_mixture_nll": (_i, [_i, _i, _i, _i, _v, _i, _v, _v, _v, c_uint64, _v, _v, _d, _v, _v]),
 }
 
@@ -160,6 +163,50 @@ def head_offsets(head, bound, stream=None):
     _ck(load().rebop_sg_head_offsets(B, S, K, _p(head), head.shape[1], _p(bound), stream))
 
 
+def pack_kslices(W, Wp, stream=None):
+    """Wp [K/32][N][32] bfloat16 = the k-slice-major copy of W [K][N] float32
+    (the layout the fused forward pass reads)."""
+    Kd, N = W.shape
+    _ck(load().rebop_sg_pack_kslices_bf16(Kd, N, _p(W), _p(Wp), stream))
+
+
+def packed(W):
+    import torch
+
+    Wp = torch.empty((W.shape[0] // 32, W.shape[1], 32), dtype=torch.bfloat16, device="cuda")
+    pack_kslices(W, Wp)
+    return Wp
+
+
+def _layer_table(Wp, bias):
+    """Host arrays (dims, weight pointers, bias pointers) of a packed layer stack."""
+    depth = len(Wp)
+    dims = (ctypes.c_int * (depth + 1))(Wp[0].shape[0] * 32, *[w.shape[1] for w in Wp])
+    wp = (ctypes.c_void_p * depth)(*[_p(w) for w in Wp])
+    bp = (ctypes.c_void_p * depth)(*[_p(b) for b in bias])
+    return depth, dims, wp, bp
+
+
+def mlp_forward(X, Wp, bias, out, stream=None):
+    """out = the forward pass of X through the layers (Wp[l] packed by
+    pack_kslices, bias[l] float32; ReLU and BF16 between layers) in one
+    launch, with the activations kept on chip: the bits of gemm_bias_act
+    layer by layer."""
+    depth, dims, wp, bp = _layer_table(Wp, bias)
+    _ck(load().rebop_sg_mlp_forward_bf16(X.shape[0], _p(X), X.shape[1], depth, ctypes.addressof(dims),
+                                         ctypes.addressof(wp), ctypes.addressof(bp), _p(out),
+                                         out.shape[1], stream))
+
+
+def infer(Wp, bias, item_point, item_sample, rates, x0, head, stream=None):
+    """head[b] = the mixture parameters of item b: features, the forward pass
+    and head_offsets in one launch (the same bits as the three calls)."""
+    depth, dims, wp, bp = _layer_table(Wp, bias)
+    _ck(load().rebop_sg_infer(item_point.numel(), _p(item_point), _p(item_sample), _p(rates), _p(x0), TMAX,
+                              NB, depth, ctypes.addressof(dims), ctypes.addressof(wp),
+                              ctypes.addressof(bp), _p(head), head.shape[1], stream))
+
+
 def nb_mixture_nll(head, item_point, item_sample, samples, grad, loss_sum, scale, bound=None,
                    stream=None):
     B = item_point.numel()
@@ -230,9 +277,10 @@ class MLP:
         dims = [IN] + [width] * (depth - 1) + [OUT]
         self.dims = dims
         # FP32 master weights W [in][out], their BF16 copy Wb (the backward's
-        # operand) and its transpose Wt [out][in] (the forward's: the layout
-        # the GEMM loads fastest)
-        self.W, self.b, self.Wb, self.Wt = [], [], [], []
+        # operand), its transpose Wt [out][in] (the training forward's: the
+        # layout the GEMM loads fastest) and its k-slice-major copy Wp (the
+        # fused inference pass's)
+        self.W, self.b, self.Wb, self.Wt, self.Wp = [], [], [], [], []
         for l in range(depth):
             fan_in = NF if l == 0 else dims[l]
             w = torch.randn(dims[l], dims[l + 1], generator=g, dtype=torch.float64) * math.sqrt(2.0 / fan_in)
@@ -251,6 +299,7 @@ class MLP:
             self.b.append(b.float().cuda())
             self.Wb.append(self.W[-1].to(torch.bfloat16))
             self.Wt.append(self.W[-1].t().contiguous().to(torch.bfloat16))
+            self.Wp.append(packed(self.W[-1]))
         self.mW = [torch.zeros_like(w) for w in self.W]
         self.vW = [torch.zeros_like(w) for w in self.W]
         self.mb = [torch.zeros_like(b) for b in self.b]
@@ -310,6 +359,7 @@ class MLP:
         for l in range(self.depth):
             adam(self.W[l], dW[l], self.mW[l], self.vW[l], self.Wb[l], lr, b1, b2, eps, self.t)
             transpose_bf16(self.W[l], self.Wt[l])
+            pack_kslices(self.W[l], self.Wp[l])
             adam(self.b[l], db[l], self.mb[l], self.vb[l], None, lr, b1, b2, eps, self.t)
 
 
@@ -381,7 +431,18 @@ def predict(mlp: MLP, rates: np.ndarray, x0: np.ndarray, sample_idx: np.ndarray,
 
 
 def infer_device(mlp: MLP, d_rates, d_x0, d_point, d_sample, B: int):
-    """Device-only inference: the mixture parameters [B][OUT] of B queries."""
+    """Device-only inference: the mixture parameters [B][OUT] of B queries,
+    in one launch (features, forward pass and head offsets fused)."""
+    import torch
+
+    head = torch.empty((B, OUT), dtype=torch.float32, device="cuda")
+    infer(mlp.Wp, mlp.b, d_point, d_sample, d_rates, d_x0, head)
+    return head
+
+
+def infer_device_layers(mlp: MLP, d_rates, d_x0, d_point, d_sample, B: int):
+    """infer_device as three steps (features, layer-by-layer forward pass,
+    head offsets): the same values, with the activations through memory."""
     import torch
 
     X = torch.empty((B, IN), dtype=torch.bfloat16, device="cuda")

@@ -99,6 +99,118 @@ def test_transposed_bf16_copy(sg):
     assert torch.equal(y.view(torch.int16), x.t().contiguous().to(torch.bfloat16).view(torch.int16))
 
 
+def _stack(sg, rng, dims):
+    """Random layers: (Wt [N][K] bf16 for gemm_bias_act, the packed copy for
+    the fused pass, bias)."""
+    import torch
+
+    W = [torch.as_tensor(rng.normal(size=(dims[l], dims[l + 1])) * np.sqrt(2.0 / dims[l]), dtype=torch.float32,
+                         device="cuda") for l in range(len(dims) - 1)]
+    Wt = [w.t().contiguous().to(torch.bfloat16) for w in W]
+    Wp = [sg.packed(w) for w in W]
+    b = [torch.as_tensor(rng.normal(size=dims[l + 1]) * 0.1, dtype=torch.float32, device="cuda")
+         for l in range(len(dims) - 1)]
+    return Wt, Wp, b
+
+
+def _layer_by_layer(sg, X, Wt, b):
+    import torch
+
+    H = X
+    for l in range(len(Wt)):
+        N = Wt[l].shape[0]
+        if l < len(Wt) - 1:
+            h = torch.empty((X.shape[0], N), dtype=torch.bfloat16, device="cuda")
+            sg.gemm_bias_act(H, Wt[l], b[l], True, H=h, tb=True)
+            H = h
+        else:
+            F = torch.empty((X.shape[0], N), dtype=torch.float32, device="cuda")
+            sg.gemm_bias_act(H, Wt[l], b[l], False, F=F, tb=True)
+    return F
+
+
+def test_packed_weight_layout(sg):
+    import torch
+
+    W = torch.randn(96, 256, dtype=torch.float32, device="cuda")
+    Wp = sg.packed(W)
+    torch.cuda.synchronize()
+    ref = W.to(torch.bfloat16).reshape(3, 32, 256).permute(0, 2, 1).contiguous()
+    assert torch.equal(Wp.view(torch.int16), ref.view(torch.int16))
+
+
+@pytest.mark.parametrize("dims", [[64, 512, 512, 512, 512, 128], [128, 256, 128, 512, 128, 256], [64, 128]])
+@pytest.mark.parametrize("M", [1088, 4096])
+def test_fused_forward_equals_layer_by_layer(sg, dims, M):
+    """One launch through the whole stack, activations kept in LDS, writes
+    the bits of gemm_bias_act applied layer by layer (512 x 512 layers run
+    the fully unrolled k loop, the others the generic one; 1088 rows end in a
+    partial block of 128)."""
+    import torch
+
+    rng = np.random.default_rng(len(dims) * 7 + M)
+    Wt, Wp, b = _stack(sg, rng, dims)
+    X = _bf(rng.normal(size=(M, dims[0])))
+    ref = _layer_by_layer(sg, X, Wt, b)
+    out = torch.full((M, dims[-1]), float("nan"), dtype=torch.float32, device="cuda")
+    sg.mlp_forward(X, Wp, b, out)
+    torch.cuda.synchronize()
+    assert torch.isfinite(out).all()
+    assert torch.equal(out.view(torch.int32), ref.view(torch.int32))
+
+
+def test_fused_forward_ragged_rows(sg):
+    """A row count that is not a multiple of 64: rows past M are neither read
+    nor written."""
+    import torch
+
+    dims = [64, 512, 256, 128]
+    rng = np.random.default_rng(17)
+    Wt, Wp, b = _stack(sg, rng, dims)
+    X = _bf(rng.normal(size=(1024, dims[0])))
+    ref = _layer_by_layer(sg, X, Wt, b)
+    out = torch.full((1024, dims[-1]), float("nan"), dtype=torch.float32, device="cuda")
+    sg.mlp_forward(X[:1000], Wp, b, out[:1000])
+    torch.cuda.synchronize()
+    assert torch.equal(out[:1000].view(torch.int32), ref[:1000].view(torch.int32))
+    assert torch.isnan(out[1000:]).all()
+
+
+def test_fused_forward_rejects_unsupported_shapes(sg):
+    import torch
+
+    from rebop_amd._native import RebopError
+
+    rng = np.random.default_rng(2)
+    X = _bf(rng.normal(size=(128, 64)))
+    for dims in ([64, 96], [64, 1024], [32, 128], [64, 64]):
+        _, Wp, b = _stack(sg, rng, dims)
+        out = torch.empty((128, dims[-1]), dtype=torch.float32, device="cuda")
+        with pytest.raises(RebopError):
+            sg.mlp_forward(X[:, : dims[0]].contiguous(), Wp, b, out)
+
+
+def test_fused_inference_equals_three_steps(sg):
+    """rebop_sg_infer (features, forward pass and head offsets in one launch)
+    against the three separate launches: the same bits, for queries over many
+    parameter points and sample times (5056 items: a partial last block)."""
+    import torch
+
+    rates, x0 = sg.sample_points(300, 11)
+    d_r = torch.as_tensor(rates, device="cuda")
+    d_x = torch.as_tensor(x0, device="cuda")
+    rng = np.random.default_rng(12)
+    B = 5056
+    pts = torch.as_tensor(rng.integers(0, 300, B).astype(np.int32), device="cuda")
+    smp = torch.as_tensor(rng.integers(1, sg.NB + 1, B).astype(np.int32), device="cuda")
+    mlp = sg.MLP(seed=3)
+    fused = sg.infer_device(mlp, d_r, d_x, pts, smp, B)
+    steps = sg.infer_device_layers(mlp, d_r, d_x, pts, smp, B)
+    torch.cuda.synchronize()
+    assert torch.isfinite(fused).all()
+    assert torch.equal(fused.view(torch.int32), steps.view(torch.int32))
+
+
 def test_nll_kernel_matches_fp64_reference(sg):
     import torch
 

@@ -377,8 +377,8 @@ __global__ void head_offsets_kernel(int B, int S, int K, float *__restrict__ hea
 // ---------------------------------------------------------------------------
 // Inference through the whole network in one launch.  A workgroup of eight
 // waves takes 128 items, whose activations stay in LDS (128 rows of up to 512
-// BF16, padded to 1040 B against bank conflicts) from the input layer to the
-// last hidden layer: HBM sees the inputs and the head only.  Wave w computes
+// BF16, stored in MFMA fragment order: see hs_offset) from the input layer
+// to the last hidden layer: HBM sees the inputs and the head only.  Wave w computes
 // all 128 rows and columns (N / 8) w .. +N/8 of each layer, so each weight
 // fragment is fetched once per workgroup.  The weights come straight from
 // global memory (every workgroup reads the same ones, so they stay in L2) in
@@ -394,7 +394,18 @@ __global__ void head_offsets_kernel(int B, int S, int K, float *__restrict__ hea
 // REBOP_SG_FEATURES zero) and the last layer adds head_offsets_kernel's
 // ln bound terms, so one launch turns (point, sample) items into mixture
 // parameters; otherwise X [M][ldx] is the input.
-constexpr int MLP_BM = 128, MLP_LDH = 520, MLP_MAXW = 512, MLP_MAXL = 8;
+constexpr int MLP_BM = 128, MLP_MAXW = 512, MLP_MAXL = 8;
+
+// Byte offset of activation (m, n) in LDS.  Fragment order: the 16 x 32
+// block of rows 16 (m / 16).. and columns 32 (n / 32).. is 1 KB, in which
+// lane l's B operand (row l % 16, columns 8 (l / 16) .. +8 of the block)
+// is 16 consecutive bytes at 16 l -- so a wave's fragment read is 1 KB of
+// consecutive bytes, free of bank conflicts.  Row blocks are 16 KB apart
+// (512 columns).
+__device__ __forceinline__ uint32_t hs_offset(int m, int n) {
+  return (uint32_t)(((m >> 4) * (MLP_MAXW / 32) + (n >> 5)) * 1024 + ((m & 15) + 16 * ((n & 31) >> 3)) * 16 +
+                    (n & 7) * 2);
+}
 
 struct MlpNet {
   const __bf16 *Wp[MLP_MAXL]; // layer l: [dims[l] / 32][dims[l+1]][32]
@@ -411,12 +422,36 @@ struct MlpItems {
   int nb_steps;
 };
 
+// Layer l's table entries, selected without indexing the argument block (a
+// runtime index would copy it to scratch memory).
+__device__ __forceinline__ int net_dim(const MlpNet &net, int l) {
+  int v = net.dims[0];
+#pragma unroll
+  for (int j = 1; j <= MLP_MAXL; ++j)
+    if (j == l) v = net.dims[j];
+  return v;
+}
+__device__ __forceinline__ const __bf16 *net_weights(const MlpNet &net, int l) {
+  const __bf16 *v = net.Wp[0];
+#pragma unroll
+  for (int j = 1; j < MLP_MAXL; ++j)
+    if (j == l) v = net.Wp[j];
+  return v;
+}
+__device__ __forceinline__ const float *net_bias(const MlpNet &net, int l) {
+  const float *v = net.bias[0];
+#pragma unroll
+  for (int j = 1; j < MLP_MAXL; ++j)
+    if (j == l) v = net.bias[j];
+  return v;
+}
+
 // NB: 16-column fragments per wave (N = 128 NB); KT: the input width when it
 // is known at compile time (the k loop is then unrolled whole, which also
 // spares the loop header's wait for every outstanding load), else 0.
 template <int NB, int KT>
 __device__ __forceinline__ void mlp_layer(const __bf16 *__restrict__ Wp, const float *__restrict__ bias,
-                                          int Kr, bool last, __bf16 (*Hs)[MLP_LDH], const float (*lnb)[4],
+                                          int Kr, bool last, char *Hs, const float (*lnb)[4],
                                           int m0, int M, float *__restrict__ out, int ldo) {
   constexpr int N = 128 * NB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -429,16 +464,16 @@ __device__ __forceinline__ void mlp_layer(const __bf16 *__restrict__ Wp, const f
     const __bf16 *base = Wp + ((size_t)k * N + (size_t)c * 512);
     dst = *(const bf16x8 *)(base + lane_off);
   };
-  // Row fragment r at k: byte offset hb[r / 4] + (16 (r % 4) MLP_LDH + k) 2
-  // from Hs.  Rows 64.. get a base register of their own (kept opaque to the
+  // Row fragment r at k: byte offset hb[r / 4] + (16 (r % 4) + k / 32) 1024
+  // in Hs.  Rows 64.. get a base register of their own (kept opaque to the
   // compiler) so that every offset fits a DS instruction's 16-bit immediate:
   // one address register for the unrolled loop instead of one per load.
-  const uint32_t hb0 = (uint32_t)((lane & 15) * MLP_LDH + (lane >> 4) * 8) * 2u;
-  uint32_t hb1 = hb0 + 64u * MLP_LDH * 2u;
+  const uint32_t hb0 = (uint32_t)lane * 16u;
+  uint32_t hb1 = hb0 + 4u * 16u * 1024u;
   asm volatile("" : "+v"(hb1));
   auto hload = [&](bf16x8 &dst, int r, int k) {
-    const uint32_t off = (r < 4 ? hb0 : hb1) + (uint32_t)((r & 3) * 16 * MLP_LDH + k) * 2u;
-    dst = *(const bf16x8 *)((const char *)&Hs[0][0] + off);
+    const uint32_t off = (r < 4 ? hb0 : hb1) + (uint32_t)((r & 3) * 16 + (k >> 5)) * 1024u;
+    dst = *(const bf16x8 *)(Hs + off);
   };
   floatx4 acc[NB][8];
 #pragma unroll
@@ -448,32 +483,59 @@ __device__ __forceinline__ void mlp_layer(const __bf16 *__restrict__ Wp, const f
   // Software pipeline, fixed with scheduling barriers (left alone, the
   // compiler sinks each load to just before its use): row fragment r of the
   // next slice loads as soon as the MFMAs of this slice have read slot r, and
-  // the next slice's weight fragments (a second set of registers) load after
-  // the first row block, so every load has most of a slice to arrive in.
-  bf16x8 wa[NB], wb[NB], h[8];
-#pragma unroll
-  for (int c = 0; c < NB; ++c) wload(wa[c], c, 0);
+  // weight fragments load after the first row block of a slice into a set of
+  // registers of their own: for the slice after next when the k loop is
+  // unrolled (three sets), for the next slice otherwise (two sets).
+  bf16x8 h[8];
 #pragma unroll
   for (int r = 0; r < 8; ++r) hload(h[r], r, 0);
-  auto slice = [&](const bf16x8(&w)[NB], bf16x8(&wn)[NB], int kn) {
+  if constexpr (KT != 0) {
+    constexpr int NS = KT / 32;
+    bf16x8 w[3][NB];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-#pragma unroll
-      for (int c = 0; c < NB; ++c) acc[c][r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[c], h[r], acc[c][r], 0, 0, 0);
-      hload(h[r], r, kn);
-      if (r == 0) {
-#pragma unroll
-        for (int c = 0; c < NB; ++c) wload(wn[c], c, kn);
-      }
-      __builtin_amdgcn_sched_barrier(0);
+    for (int c = 0; c < NB; ++c) {
+      wload(w[0][c], c, 0);
+      wload(w[1][c], c, 32);
     }
-  };
-  constexpr int UNROLL = KT ? KT / 64 : 1;
-#pragma unroll UNROLL
-  for (int k0 = 0; k0 < K; k0 += 64) { // K is a multiple of 64
-    slice(wa, wb, k0 + 32);
-    // the last slice's loads fetch that slice again (no branch in the loop)
-    slice(wb, wa, k0 + 64 < K ? k0 + 64 : k0 + 32);
+#pragma unroll
+    for (int sl = 0; sl < NS; ++sl) {
+      // the last slices' loads fetch the last slice again (no branches)
+      const int kn = (sl + 1 < NS ? sl + 1 : sl) * 32, kw = (sl + 2 < NS ? sl + 2 : NS - 1) * 32;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          acc[c][r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[sl % 3][c], h[r], acc[c][r], 0, 0, 0);
+        hload(h[r], r, kn);
+        if (r == 0) {
+#pragma unroll
+          for (int c = 0; c < NB; ++c) wload(w[(sl + 2) % 3][c], c, kw);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  } else {
+    bf16x8 wa[NB], wb[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) wload(wa[c], c, 0);
+    auto slice = [&](const bf16x8(&w)[NB], bf16x8(&wn)[NB], int kn) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) acc[c][r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[c], h[r], acc[c][r], 0, 0, 0);
+        hload(h[r], r, kn);
+        if (r == 0) {
+#pragma unroll
+          for (int c = 0; c < NB; ++c) wload(wn[c], c, kn);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    };
+    for (int k0 = 0; k0 < K; k0 += 64) { // K is a multiple of 64
+      slice(wa, wb, k0 + 32);
+      // the last slice's loads fetch that slice again (no branch in the loop)
+      slice(wb, wa, k0 + 64 < K ? k0 + 64 : k0 + 32);
+    }
   }
   __syncthreads(); // every wave has read this layer's input
 #pragma unroll
@@ -490,7 +552,7 @@ __device__ __forceinline__ void mlp_layer(const __bf16 *__restrict__ Wp, const f
         __bf16 hv[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) hv[q] = (__bf16)(v[q] > 0.f ? v[q] : 0.f);
-        __builtin_memcpy(&Hs[m][n], hv, sizeof hv);
+        __builtin_memcpy(Hs + hs_offset(m, n), hv, sizeof hv);
       } else if (m0 + m < M) {
         if (lnb) {
           // head_offsets_kernel: + ln bound[s] on the negative-binomial log
@@ -502,7 +564,9 @@ __device__ __forceinline__ void mlp_layer(const __bf16 *__restrict__ Wp, const f
             if (s < 4 && j >= KC && j < KC + KC / 2) v[q] += lnb[m][s];
           }
         }
-        *(floatx4 *)&out[(size_t)(m0 + m) * ldo + n] = v;
+        // streaming store: the head is not read again, and write-allocated
+        // lines would push the weights out of L2
+        __builtin_nontemporal_store(v, (floatx4 *)&out[(size_t)(m0 + m) * ldo + n]);
       }
     }
   }
@@ -513,9 +577,14 @@ template <bool FEAT>
 __global__ void __launch_bounds__(512) mlp_infer_kernel(int M, const __bf16 *__restrict__ X, int ldx,
                                                         MlpItems items, MlpNet net,
                                                         float *__restrict__ out, int ldo) {
-  __shared__ __attribute__((aligned(16))) __bf16 Hs[MLP_BM][MLP_LDH];
+  __shared__ __attribute__((aligned(16))) char Hs[MLP_BM * MLP_MAXW * 2];
   __shared__ float lnb[FEAT ? MLP_BM : 1][4];
+  __shared__ __attribute__((aligned(16))) float Bs[MLP_MAXL * MLP_MAXW]; // every layer's bias
   const int tid = threadIdx.x, m0 = blockIdx.x * MLP_BM, K0 = net.dims[0];
+  for (int l = 0, o = 0; l < net.depth; o += net_dim(net, l + 1), ++l) {
+    const float *b = net_bias(net, l);
+    if (tid < net_dim(net, l + 1)) Bs[o + tid] = b[tid];
+  }
   if (FEAT) {
     if (tid < MLP_BM) {
       float f[REBOP_SG_FEATURES], e0 = 1.f, s0 = 1.f;
@@ -524,7 +593,12 @@ __global__ void __launch_bounds__(512) mlp_infer_kernel(int M, const __bf16 *__r
       if (m0 + tid < M)
         item_features(items.item_point[m0 + tid], items.item_sample[m0 + tid], items.rates, items.x0,
                       items.tmax, items.nb_steps, f, e0, s0);
-      for (int c = 0; c < K0; ++c) Hs[tid][c] = (__bf16)(c < REBOP_SG_FEATURES ? f[c] : 0.f);
+      for (int c = 0; c < K0; c += 8) {
+        bf16x8 v;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = (__bf16)(c + q < REBOP_SG_FEATURES ? f[c + q] : 0.f);
+        *(bf16x8 *)(Hs + hs_offset(tid, c)) = v;
+      }
       const float le = logf(e0), ls = logf(s0);
       lnb[tid][0] = le;
       lnb[tid][1] = ls;
@@ -538,25 +612,15 @@ __global__ void __launch_bounds__(512) mlp_infer_kernel(int M, const __bf16 *__r
       const int r = idx / per_row, c = (idx % per_row) * 8;
       bf16x8 v = {};
       if (m0 + r < M) v = *(const bf16x8 *)&X[(size_t)(m0 + r) * ldx + c];
-      *(bf16x8 *)&Hs[r][c] = v;
+      *(bf16x8 *)(Hs + hs_offset(r, c)) = v;
     }
   }
   __syncthreads();
   const float(*lp)[4] = FEAT ? lnb : nullptr;
-  for (int l = 0; l < net.depth; ++l) {
-    // layer l's table entries, selected without indexing the argument block
-    // (a runtime index would copy it to scratch memory)
-    const __bf16 *W = net.Wp[0];
-    const float *bias = net.bias[0];
-    int K = net.dims[0], N = net.dims[1];
-#pragma unroll
-    for (int j = 1; j < MLP_MAXL; ++j)
-      if (j == l) {
-        W = net.Wp[j];
-        bias = net.bias[j];
-        K = net.dims[j];
-        N = net.dims[j + 1];
-      }
+  for (int l = 0, o = 0; l < net.depth; o += net_dim(net, l + 1), ++l) {
+    const __bf16 *W = net_weights(net, l);
+    const float *bias = Bs + o;
+    const int K = net_dim(net, l), N = net_dim(net, l + 1);
     const bool last = l == net.depth - 1;
     if (K == 512 && N == 512) {
       mlp_layer<4, 512>(W, bias, K, last, Hs, lp, m0, M, out, ldo);

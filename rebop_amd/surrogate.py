@@ -611,7 +611,7 @@ def benchmark(n_train: int = 100_000, steps: int = 12_000, batch: int = 8192, n_
     d_r = torch.as_tensor(qr, device="cuda")
     d_x = torch.as_tensor(qx, device="cuda")
     qs = np.random.default_rng(seed + 4).integers(1, NB + 1, n_queries).astype(np.int32)
-    chunk = 1 << 17
+    chunk = 1 << 20  # the 10^6 queries in one launch of the fused kernel
     pts = [torch.arange(a, min(n_queries, a + chunk), dtype=torch.int32, device="cuda") for a in range(0, n_queries, chunk)]
     sps = [torch.as_tensor(qs[a:a + chunk], device="cuda") for a in range(0, n_queries, chunk)]
     for p_, s_ in zip(pts, sps):  # warm

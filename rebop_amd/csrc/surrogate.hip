@@ -395,6 +395,7 @@ __global__ void head_offsets_kernel(int B, int S, int K, float *__restrict__ hea
 // ln bound terms, so one launch turns (point, sample) items into mixture
 // parameters; otherwise X [M][ldx] is the input.
 constexpr int MLP_BM = 128, MLP_MAXW = 512, MLP_MAXL = 8;
+constexpr int MLP_HEAD_LD = 128 * 4 + 16; // bytes per row of a staged 128-column head
 
 // Byte offset of activation (m, n) in LDS.  Fragment order: the 16 x 32
 // block of rows 16 (m / 16).. and columns 32 (n / 32).. is 1 KB, in which
@@ -538,13 +539,18 @@ __device__ __forceinline__ void mlp_layer(const __bf16 *__restrict__ Wp, const f
     }
   }
   __syncthreads(); // every wave has read this layer's input
+  // The epilogue's indices derive from an opaque copy of the lane id: left
+  // alone, the compiler computes them for every layer variant up front and
+  // keeps them in scratch memory across the k loops (spill traffic in HBM).
+  int le = lane;
+  asm volatile("" : "+v"(le));
 #pragma unroll
   for (int c = 0; c < NB; ++c) {
-    const int n = n0 + c * 16 + (lane >> 4) * 4;
+    const int n = n0 + c * 16 + (le >> 4) * 4;
     const floatx4 b = *(const floatx4 *)&bias[n];
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-      const int m = r * 16 + (lane & 15);
+      const int m = r * 16 + (le & 15);
       floatx4 v;
 #pragma unroll
       for (int q = 0; q < 4; ++q) v[q] = acc[c][r][q] + b[q];
@@ -553,7 +559,7 @@ __device__ __forceinline__ void mlp_layer(const __bf16 *__restrict__ Wp, const f
 #pragma unroll
         for (int q = 0; q < 4; ++q) hv[q] = (__bf16)(v[q] > 0.f ? v[q] : 0.f);
         __builtin_memcpy(Hs + hs_offset(m, n), hv, sizeof hv);
-      } else if (m0 + m < M) {
+      } else {
         if (lnb) {
           // head_offsets_kernel: + ln bound[s] on the negative-binomial log
           // means, columns s*3K + K + c for c < K/2
@@ -564,13 +570,31 @@ __device__ __forceinline__ void mlp_layer(const __bf16 *__restrict__ Wp, const f
             if (s < 4 && j >= KC && j < KC + KC / 2) v[q] += lnb[m][s];
           }
         }
-        // streaming store: the head is not read again, and write-allocated
-        // lines would push the weights out of L2
-        __builtin_nontemporal_store(v, (floatx4 *)&out[(size_t)(m0 + m) * ldo + n]);
+        if constexpr (NB == 1) {
+          // a head of 128 columns goes through LDS (free now) so that the
+          // stores below write whole rows
+          *(floatx4 *)(Hs + m * MLP_HEAD_LD + n * 4) = v;
+        } else if (m0 + m < M) {
+          __builtin_nontemporal_store(v, (floatx4 *)&out[(size_t)(m0 + m) * ldo + n]);
+        }
       }
     }
   }
   __syncthreads(); // the layer's output is whole before the next layer reads it
+  if (NB == 1 && last) {
+    // 128 rows x 512 B: a wave instruction stores two whole rows (full
+    // cache lines, streaming: the head is not read again, and
+    // write-allocated lines would push the weights out of L2)
+    int te = tid;
+    asm volatile("" : "+v"(te));
+#pragma unroll
+    for (int i = 0; i < MLP_BM * 32 / 512; ++i) {
+      const int idx = te + 512 * i, m = idx >> 5, c4 = idx & 31;
+      if (m0 + m < M)
+        __builtin_nontemporal_store(*(const floatx4 *)(Hs + m * MLP_HEAD_LD + c4 * 16),
+                                    (floatx4 *)&out[(size_t)(m0 + m) * ldo + c4 * 4]);
+    }
+  }
 }
 
 template <bool FEAT>

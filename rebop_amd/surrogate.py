@@ -617,13 +617,16 @@ def benchmark(n_train: int = 100_000, steps: int = 12_000, batch: int = 8192, n_
     for p_, s_ in zip(pts, sps):  # warm
         infer_device(mlp, d_r, d_x, p_, s_, p_.numel())
     torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    for p_, s_ in zip(pts, sps):
-        infer_device(mlp, d_r, d_x, p_, s_, p_.numel())
-    ev1.record()
-    torch.cuda.synchronize()
-    sg_ms = ev0.elapsed_time(ev1)
+    times = []
+    for _ in range(5):  # the median of five timed passes
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for p_, s_ in zip(pts, sps):
+            infer_device(mlp, d_r, d_x, p_, s_, p_.numel())
+        ev1.record()
+        torch.cuda.synchronize()
+        times.append(ev0.elapsed_time(ev1))
+    sg_ms = float(np.median(times))
     sg_qps = n_queries / (sg_ms / 1e3)
     flops = 2.0 * n_queries * sum(mlp.dims[l] * mlp.dims[l + 1] for l in range(mlp.depth))
     log(f"surrogate: {n_queries} queries in {sg_ms:.2f} ms ({sg_qps:.3e} q/s, {flops / sg_ms / 1e9:.1f} TFLOP/s); "

@@ -448,8 +448,9 @@ __device__ __forceinline__ const float *net_bias(const MlpNet &net, int l) {
 }
 
 // NB: 16-column fragments per wave (N = 128 NB); KT: the input width when it
-// is known at compile time (the k loop is then unrolled whole, which also
-// spares the loop header's wait for every outstanding load), else 0.
+// is known at compile time (the k loop is then unrolled whole, which spares
+// the loop header's wait for every outstanding load and the loads past the
+// last slice), else 0.
 template <int NB, int KT>
 __device__ __forceinline__ void mlp_layer(const __bf16 *__restrict__ Wp, const float *__restrict__ bias,
                                           int Kr, bool last, char *Hs, const float (*lnb)[4],
@@ -496,21 +497,21 @@ __device__ __forceinline__ void mlp_layer(const __bf16 *__restrict__ Wp, const f
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
       wload(w[0][c], c, 0);
-      wload(w[1][c], c, 32);
+      if (NS > 1) wload(w[1][c], c, 32);
     }
 #pragma unroll
     for (int sl = 0; sl < NS; ++sl) {
-      // the last slices' loads fetch the last slice again (no branches)
-      const int kn = (sl + 1 < NS ? sl + 1 : sl) * 32, kw = (sl + 2 < NS ? sl + 2 : NS - 1) * 32;
+      // no loads past the last slice (compile-time conditions: nothing is
+      // left in flight for the barrier at the end of the layer to wait for)
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
 #pragma unroll
         for (int c = 0; c < NB; ++c)
           acc[c][r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[sl % 3][c], h[r], acc[c][r], 0, 0, 0);
-        hload(h[r], r, kn);
-        if (r == 0) {
+        if (sl + 1 < NS) hload(h[r], r, (sl + 1) * 32);
+        if (r == 0 && sl + 2 < NS) {
 #pragma unroll
-          for (int c = 0; c < NB; ++c) wload(w[(sl + 2) % 3][c], c, kw);
+          for (int c = 0; c < NB; ++c) wload(w[(sl + 2) % 3][c], c, (sl + 2) * 32);
         }
         __builtin_amdgcn_sched_barrier(0);
       }
@@ -646,8 +647,17 @@ __global__ void __launch_bounds__(512) mlp_infer_kernel(int M, const __bf16 *__r
     const float *bias = Bs + o;
     const int K = net_dim(net, l), N = net_dim(net, l + 1);
     const bool last = l == net.depth - 1;
+    // the surrogate's shapes, fully unrolled
     if (K == 512 && N == 512) {
       mlp_layer<4, 512>(W, bias, K, last, Hs, lp, m0, M, out, ldo);
+      continue;
+    }
+    if (K == 64 && N == 512) {
+      mlp_layer<4, 64>(W, bias, K, last, Hs, lp, m0, M, out, ldo);
+      continue;
+    }
+    if (K == 512 && N == 128) {
+      mlp_layer<1, 512>(W, bias, K, last, Hs, lp, m0, M, out, ldo);
       continue;
     }
     switch (N) {

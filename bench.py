@@ -62,6 +62,32 @@ def parse_traffic(path: Path):
         return None
 
 
+def parse_issue_counters(path: Path):
+    """The measured VALU issue figures of the committed rocprofv3 --pmc
+    summary (instructions per event and the issue rate the kernel reaches),
+    if any: the FP64-op fraction above counts only the reference's own
+    arithmetic, these count every instruction the kernel issues."""
+    if not path.exists():
+        return None
+    out = {}
+    keys = {
+        "VALU wave-instructions per event": "valu_wave_instructions_per_event",
+        "active-lane fraction of VALU issue": "active_lane_fraction",
+        "VALU issue cycles per CU-cycle": "valu_issue_per_cu_cycle",
+    }
+    for line in path.read_text().splitlines():
+        for k, name in keys.items():
+            if line.startswith("- " + k) and ":" in line:
+                try:
+                    out[name] = float(line.rsplit(":", 1)[1].split()[0])
+                except ValueError:
+                    pass
+    if not out:
+        return None
+    out["source"] = str(path.relative_to(ROOT))
+    return out
+
+
 def host_threads() -> int:
     try:
         n = len(os.sched_getaffinity(0))
@@ -439,6 +465,7 @@ def main() -> int:
             "ops_per_event": F,
             "kernel_ms": avg_kernel_s * 1e3,
             "timed": "SSA launch sequence: main kernel + tail launch + int64 sample conversion (HIP events)",
+            "measured_issue": parse_issue_counters(ROOT / "profiles" / f"{stem}_counters.md"),
         },
     }
     if not args.no_cpu_baseline:

@@ -314,34 +314,74 @@ __global__ void adam_kernel(size_t n, float *__restrict__ w, const float *__rest
 // kf S0 t, K_M / S0 with K_M = (kb + kcat) / kf, kb / kcat, E0 / S0,
 // kcat E0 t / S0); e0, s0 = the counts that bound the species (E0 for E and
 // ES, S0 for S and P; exact in FP32).
+// The item's inputs: rate constants, the bounding counts and the sample time.
+struct ItemInputs {
+  double kf, kb, kc, e0, s0, t;
+};
+
+__device__ __forceinline__ ItemInputs item_inputs(uint32_t p, uint32_t i, const double *__restrict__ rates,
+                                                  const int64_t *__restrict__ x0, double tmax, int nb_steps) {
+#pragma clang fp contract(off)
+  ItemInputs v;
+  v.kf = rates[3 * p + 0];
+  v.kb = rates[3 * p + 1];
+  v.kc = rates[3 * p + 2];
+  v.e0 = (double)x0[4 * p + 0];
+  v.s0 = (double)x0[4 * p + 1];
+  v.t = tmax * (double)i / (double)nb_steps;
+  return v;
+}
+
+// Log j of the seven the features use: log10 of kf, kb, kc, E0, S0, t and
+// kb + kc.
+__device__ __forceinline__ double item_log(const ItemInputs &v, int j) {
+#pragma clang fp contract(off)
+  switch (j) {
+  case 0: return log10(v.kf);
+  case 1: return log10(v.kb);
+  case 2: return log10(v.kc);
+  case 3: return log10(v.e0);
+  case 4: return log10(v.s0);
+  case 5: return log10(v.t);
+  default: return log10(v.kb + v.kc);
+  }
+}
+
+// Feature j (before the FP32 cast) from the inputs and their logs lg[0..6].
+__device__ __forceinline__ double item_feature(const ItemInputs &v, const double *lg, double tmax, int j) {
+#pragma clang fp contract(off)
+  const double lf = lg[0], lb = lg[1], lc = lg[2], le = lg[3], ls = lg[4], lt = lg[5];
+  switch (j) {
+  case 0: return (lf + 1.0) * 0.5;
+  case 1: return (lb + 1.0) * 0.5;
+  case 2: return (lc + 1.0) * 0.5;
+  case 3: return (v.e0 - 105.0) / 95.0;
+  case 4: return (v.s0 - 525.0) / 475.0;
+  case 5: return 2.0 * v.t / tmax - 1.0;
+  case 6: return (le - 1.65) / 0.65;
+  case 7: return (ls - 2.35) / 0.65;
+  case 8: return (lt - 0.95) / 0.75;
+  case 9: return (lc + lt) / 3.0;
+  case 10: return (lf + le + lt - 1.6) / 3.4;
+  case 11: return (lf + ls + lt - 2.3) / 3.4;
+  case 12: return ((lg[6] - lf) - ls + 2.05) / 4.65; // K_M = (kb + kcat) / kf
+  case 13: return (lb - lc) / 4.0;
+  case 14: return (le - ls + 0.7) / 1.3;
+  default: return (lc + le + lt - ls + 0.75) / 4.05;
+  }
+}
+
 __device__ __forceinline__ void item_features(uint32_t p, uint32_t i, const double *__restrict__ rates,
                                               const int64_t *__restrict__ x0, double tmax, int nb_steps,
                                               float (&f)[REBOP_SG_FEATURES], float &e0f, float &s0f) {
-#pragma clang fp contract(off)
-  const double kf = rates[3 * p + 0], kb = rates[3 * p + 1], kc = rates[3 * p + 2];
-  const double e0 = (double)x0[4 * p + 0], s0 = (double)x0[4 * p + 1];
-  const double t = tmax * (double)i / (double)nb_steps;
-  const double lf = log10(kf), lb = log10(kb), lc = log10(kc);
-  const double le = log10(e0), ls = log10(s0), lt = log10(t);
-  const double lkm = log10(kb + kc) - lf;
-  f[0] = (float)((lf + 1.0) * 0.5);
-  f[1] = (float)((lb + 1.0) * 0.5);
-  f[2] = (float)((lc + 1.0) * 0.5);
-  f[3] = (float)((e0 - 105.0) / 95.0);
-  f[4] = (float)((s0 - 525.0) / 475.0);
-  f[5] = (float)(2.0 * t / tmax - 1.0);
-  f[6] = (float)((le - 1.65) / 0.65);
-  f[7] = (float)((ls - 2.35) / 0.65);
-  f[8] = (float)((lt - 0.95) / 0.75);
-  f[9] = (float)((lc + lt) / 3.0);
-  f[10] = (float)((lf + le + lt - 1.6) / 3.4);
-  f[11] = (float)((lf + ls + lt - 2.3) / 3.4);
-  f[12] = (float)((lkm - ls + 2.05) / 4.65);
-  f[13] = (float)((lb - lc) / 4.0);
-  f[14] = (float)((le - ls + 0.7) / 1.3);
-  f[15] = (float)((lc + le + lt - ls + 0.75) / 4.05);
-  e0f = (float)e0;
-  s0f = (float)s0;
+  const ItemInputs v = item_inputs(p, i, rates, x0, tmax, nb_steps);
+  double lg[7];
+#pragma unroll
+  for (int j = 0; j < 7; ++j) lg[j] = item_log(v, j);
+#pragma unroll
+  for (int j = 0; j < REBOP_SG_FEATURES; ++j) f[j] = (float)item_feature(v, lg, tmax, j);
+  e0f = (float)v.e0;
+  s0f = (float)v.s0;
 }
 
 // X[b][0..ldx) = item b's features, then zeros; bound[b][s] (nullable) = the
@@ -604,6 +644,7 @@ __global__ void __launch_bounds__(512) mlp_infer_kernel(int M, const __bf16 *__r
                                                         float *__restrict__ out, int ldo) {
   __shared__ __attribute__((aligned(16))) char Hs[MLP_BM * MLP_MAXW * 2];
   __shared__ float lnb[FEAT ? MLP_BM : 1][4];
+  __shared__ double lg[FEAT ? MLP_BM : 1][8]; // the items' logs, shared by their four threads
   __shared__ __attribute__((aligned(16))) float Bs[MLP_MAXL * MLP_MAXW]; // every layer's bias
   const int tid = threadIdx.x, m0 = blockIdx.x * MLP_BM, K0 = net.dims[0];
   for (int l = 0, o = 0; l < net.depth; o += net_dim(net, l + 1), ++l) {
@@ -611,24 +652,36 @@ __global__ void __launch_bounds__(512) mlp_infer_kernel(int M, const __bf16 *__r
     if (tid < net_dim(net, l + 1)) Bs[o + tid] = b[tid];
   }
   if (FEAT) {
-    if (tid < MLP_BM) {
-      float f[REBOP_SG_FEATURES], e0 = 1.f, s0 = 1.f;
+    // four threads per item: thread q computes logs 2q, 2q+1 (of seven),
+    // which the item's threads share through LDS, then features 4q .. 4q+3
+    // (the arithmetic of item_features, so the bits of features_kernel)
+    static_assert(REBOP_SG_FEATURES == 16 && MLP_BM * 4 == 512, "four threads x four features per item");
+    const int item = tid >> 2, q = tid & 3;
+    ItemInputs v{1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
+    if (m0 + item < M)
+      v = item_inputs(items.item_point[m0 + item], items.item_sample[m0 + item], items.rates, items.x0,
+                      items.tmax, items.nb_steps);
+    lg[item][2 * q] = item_log(v, 2 * q);
+    if (q < 3) lg[item][2 * q + 1] = item_log(v, 2 * q + 1);
+    __syncthreads();
+    double l7[7];
 #pragma unroll
-      for (int c = 0; c < REBOP_SG_FEATURES; ++c) f[c] = 0.f;
-      if (m0 + tid < M)
-        item_features(items.item_point[m0 + tid], items.item_sample[m0 + tid], items.rates, items.x0,
-                      items.tmax, items.nb_steps, f, e0, s0);
-      for (int c = 0; c < K0; c += 8) {
-        bf16x8 v;
+    for (int j = 0; j < 7; ++j) l7[j] = lg[item][j];
+    __bf16 h4[4];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = (__bf16)(c + q < REBOP_SG_FEATURES ? f[c + q] : 0.f);
-        *(bf16x8 *)(Hs + hs_offset(tid, c)) = v;
-      }
-      const float le = logf(e0), ls = logf(s0);
-      lnb[tid][0] = le;
-      lnb[tid][1] = ls;
-      lnb[tid][2] = le;
-      lnb[tid][3] = ls;
+    for (int j = 0; j < 4; ++j) {
+      const float f = m0 + item < M ? (float)item_feature(v, l7, items.tmax, 4 * q + j) : 0.f;
+      h4[j] = (__bf16)f;
+    }
+    __builtin_memcpy(Hs + hs_offset(item, 4 * q), h4, sizeof h4);
+    const __bf16 z4[4] = {};
+    for (int c = REBOP_SG_FEATURES + 4 * q; c < K0; c += 16) __builtin_memcpy(Hs + hs_offset(item, c), z4, sizeof z4);
+    if (q == 0) {
+      const float le = logf((float)v.e0), ls = logf((float)v.s0);
+      lnb[item][0] = le;
+      lnb[item][1] = ls;
+      lnb[item][2] = le;
+      lnb[item][3] = ls;
     }
   } else {
     // rows m0.. of X, 8 columns per thread and pass; rows past M read as zero

@@ -99,17 +99,19 @@ def host_threads() -> int:
 CPU_MIN_WALL_S = 1.0  # per random-stream variant: >= 16 s of CPU work on 16 threads
 
 
-def _time_oracle(jobs, threads: int, rng: int):
-    """Runs the job list (repeated with further seeds until it has taken
-    CPU_MIN_WALL_S of wall time); returns (events, seconds, repetitions)."""
+def oracle_lma_net(spec):
     from oracle import pyoracle
 
-    nets = []
-    for spec, seed, tmax, nb, n, tb in jobs:
-        net = pyoracle.OracleNet(spec[0], sparse=True)
-        for k, o, d in spec[1]:
-            net.add_lma(k, o, d)
-        nets.append(net)
+    net = pyoracle.OracleNet(spec[0], sparse=True)
+    for k, o, d in spec[1]:
+        net.add_lma(k, o, d)
+    return net
+
+
+def _time_oracle(jobs, threads: int, rng: int, make_net=oracle_lma_net):
+    """Runs the job list (repeated with further seeds until it has taken
+    CPU_MIN_WALL_S of wall time); returns (events, seconds, repetitions)."""
+    nets = [make_net(spec) for spec, *_ in jobs]
     events, reps = 0, 0
     t0 = time.perf_counter()
     while reps == 0 or time.perf_counter() - t0 < CPU_MIN_WALL_S:
@@ -121,7 +123,7 @@ def _time_oracle(jobs, threads: int, rng: int):
     return events, time.perf_counter() - t0, reps
 
 
-def cpu_baseline(jobs, note: str) -> dict:
+def cpu_baseline(jobs, note: str, make_net=oracle_lma_net) -> dict:
     """Times the CPU oracle over `jobs` = [(spec, seed, tmax, nb_steps, n_traj, traj_begin)]
     twice: with the device's Philox stream and shared ln (identical arithmetic),
     and with xoshiro256++ + libm log (closer to the reference's own per-event
@@ -129,8 +131,8 @@ def cpu_baseline(jobs, note: str) -> dict:
     from oracle import pyoracle
 
     threads = host_threads()
-    ev_p, dt_p, rep_p = _time_oracle(jobs, threads, pyoracle.PHILOX)
-    ev_x, dt_x, rep_x = _time_oracle(jobs, threads, pyoracle.XOSHIRO)
+    ev_p, dt_p, rep_p = _time_oracle(jobs, threads, pyoracle.PHILOX, make_net)
+    ev_x, dt_x, rep_x = _time_oracle(jobs, threads, pyoracle.XOSHIRO, make_net)
     rate_p, rate_x = ev_p / dt_p, ev_x / dt_x
     return {
         "value": max(rate_p, rate_x),
@@ -260,6 +262,151 @@ class Heterogeneous(Workload):
         return jobs, f"{len(idx)} networks (every 64th) x 16 trajectories of the same batch"
 
 
+def system_terms(orders, diffs):
+    """define_system! reactant and product terms of a mass-action reaction."""
+    reac = [(s, int(o)) for s, o in enumerate(orders) if o > 0]
+    prod = [(s, int(o) + int(d)) for s, (o, d) in enumerate(zip(orders, diffs)) if int(o) + int(d) > 0]
+    return reac, prod
+
+
+class SystemNetwork(SingleNetwork):
+    """The network compiled with define_system!'s arithmetic
+    (src/gillespie_macro.rs:49-170): integer mass-action products and the
+    first-match choice (the system mode of the engine)."""
+
+    def setup(self):
+        import torch
+
+        from rebop_amd.engine import GillespieNet
+
+        self.out = torch.empty((self.nb + 1, self.spec[0], self.n), dtype=torch.int64, device="cuda")
+        self.stream = torch.cuda.current_stream().cuda_stream
+        self.g = GillespieNet.system(self.spec[2], 0)
+        for k, o, d in self.spec[1]:
+            self.g.add_system_reaction(k, *system_terms(o, d))
+
+    def cpu_make_net(self):
+        from oracle import pyoracle
+
+        def make(spec):
+            net = pyoracle.OracleNet.system(spec[0])
+            for k, o, d in spec[1]:
+                net.add_system(k, *system_terms(o, d))
+            return net
+
+        return make
+
+
+class ExprNetwork(SingleNetwork):
+    """The network with every rate a Rate::Expr (src/expr.rs), compiled into
+    the kernel; exprs[r] is reaction r's expression over the species names."""
+
+    def __init__(self, *a, names, exprs, **kw):
+        super().__init__(*a, **kw)
+        self.names, self.exprs = names, exprs
+
+    def _postfix(self):
+        from rebop_amd import expr as _expr
+
+        species = {n: i for i, n in enumerate(self.names)}
+        return [_expr.to_postfix(_expr.parse(e), species, {}) for e in self.exprs]
+
+    def setup(self):
+        import torch
+
+        from rebop_amd.engine import GillespieNet, RateExpr
+
+        self.out = torch.empty((self.nb + 1, self.spec[0], self.n), dtype=torch.int64, device="cuda")
+        self.stream = torch.cuda.current_stream().cuda_stream
+        self.g = GillespieNet(self.spec[2], seed=0)
+        for (code, consts), (_, _, d) in zip(self._postfix(), self.spec[1]):
+            self.g.add_reaction(RateExpr.from_postfix(code, consts), d)
+
+    def cpu_make_net(self):
+        from oracle import pyoracle
+
+        pf = self._postfix()
+
+        def make(spec):
+            net = pyoracle.OracleNet(spec[0])
+            for (code, consts), (_, _, d) in zip(pf, spec[1]):
+                net.add_expr(code, consts, d)
+            return net
+
+        return make
+
+
+class EventsNetwork(Workload):
+    """The nb_steps == 0 mode (src/pyo3_gillespie.rs:209-223): every event of
+    every trajectory recorded, two passes (record counts, then the records),
+    into host buffers (rebop_run_all_events)."""
+
+    def __init__(self, name, spec, tmax, n, description, cpu_n):
+        self.name, self.spec, self.tmax, self.n = name, spec, tmax, n
+        self.description, self.cpu_n = description, cpu_n
+        self.records = 0
+
+    def setup(self):
+        from rebop_amd import networks
+
+        self.g = networks.build(self.spec, 0)
+
+    def step(self, seed):
+        import ctypes
+
+        from rebop_amd import _native
+
+        lib = _native.load()
+        self.g.seed(seed)
+        counts = np.zeros(self.n, dtype=np.uint64)
+        cp = counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+        st1, st2 = _native.Stats(), _native.Stats()
+        _native.check(lib.rebop_run_all_events(self.g.handle, self.tmax, 0, self.n, cp, None, None,
+                                               ctypes.byref(st1)))
+        total = int(counts.sum())
+        if getattr(self, "_cap", 0) < total:
+            self._times = np.empty(total, dtype=np.float64)
+            self._states = np.empty((total, self.spec[0]), dtype=np.int64)
+            self._cap = total
+        _native.check(lib.rebop_run_all_events(
+            self.g.handle, self.tmax, 0, self.n, cp,
+            self._times.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            self._states.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), ctypes.byref(st2)))
+        self.records = total
+        st2.kernel_ms += st1.kernel_ms  # both passes
+        return st2
+
+    def ops_per_event(self):
+        return algorithmic_ops_per_event(self.spec[1])
+
+    def output_bytes(self):
+        return self.records * (8 + 8 * self.spec[0])
+
+    def cpu_jobs(self):
+        return None, f"the oracle's all-events loop, one thread, trajectories 0.. of the same workload"
+
+
+def cpu_baseline_events(wl: EventsNetwork, note: str) -> dict:
+    """The oracle's nb_steps == 0 loop over trajectories 0, 1, ... of the
+    workload (one thread) until CPU_MIN_WALL_S has passed; events exclude the
+    initial and absorbing records."""
+    from oracle import pyoracle
+
+    net = oracle_lma_net(wl.spec)
+    out = {}
+    for name, rng in (("philox_ln64_events_per_s", pyoracle.PHILOX), ("xoshiro_libm_events_per_s", pyoracle.XOSHIRO)):
+        ev, j, t0 = 0, 0, time.perf_counter()
+        while time.perf_counter() - t0 < CPU_MIN_WALL_S:
+            times, _ = net.run_all_events(42, j, wl.spec[2], wl.tmax, rng=rng)
+            ev += len(times) - 1 - (1 if np.isinf(times[-1]) else 0)
+            j += 1
+        out[name] = ev / (time.perf_counter() - t0)
+        out[name + "_trajectories"] = j
+    return {"value": max(out["philox_ln64_events_per_s"], out["xoshiro_libm_events_per_s"]),
+            "unit": "events/s", "cores": 1, "kind": "port", **out,
+            "sample": note + f", >= {CPU_MIN_WALL_S:.0f} s wall per random stream; the faster is the value"}
+
+
 def make_workload(name: str, kernel: int) -> Workload:
     from rebop_amd import networks, workloads
 
@@ -278,6 +425,22 @@ def make_workload(name: str, kernel: int) -> Workload:
                              "trajectories, tmax=10, nb_steps=200", 64)
     if name == "het":
         return Heterogeneous(4096, 256, 50)
+    if name == "dimer-system":
+        return SystemNetwork("dimer-system", networks.dimers(), TMAX, NB_STEPS, N_TRAJ, kernel,
+                             "dimer network in define_system! form (integer mass-action products, "
+                             "first-match choice), 10^6 trajectories, tmax=1.0, nb_steps=100",
+                             CPU_SAMPLE)
+    if name == "dimer-expr":
+        return ExprNetwork("dimer-expr", networks.dimers(), TMAX, NB_STEPS, N_TRAJ, kernel,
+                           "dimer network with Rate::Expr rates (25*gene, 1000*mRNA, "
+                           "0.001*protein*(protein-1), 0.1*mRNA, protein), 10^6 trajectories, "
+                           "tmax=1.0, nb_steps=100", CPU_SAMPLE,
+                           names=["gene", "mRNA", "protein", "dimer"],
+                           exprs=["25*gene", "1000*mRNA", "0.001*protein*(protein-1)", "0.1*mRNA", "protein"])
+    if name == "sir-events":
+        return EventsNetwork("sir-events", networks.sir(), 250.0, 10_000,
+                             "SIR (3 species, 2 reactions), 10^4 trajectories, tmax=250, nb_steps=0 "
+                             "(every event recorded; two passes into host buffers)", 0)
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -359,9 +522,12 @@ def main() -> int:
     ap.add_argument("--autotune", action="store_true",
                     help="before timing, time every kernel variant and launch shape at the "
                          "workload's full size (rebop_autotune); AUTO steps then use the fastest")
-    ap.add_argument("--workload", default="dimer", choices=["dimer", "sir", "syn", "het", "surrogate"],
+    ap.add_argument("--workload", default="dimer", choices=["dimer", "sir", "syn", "het", "surrogate", "dimer-system",
+                                                              "dimer-expr", "sir-events"],
                     help="dimer (the headline configuration), sir, syn (128x512), het (4096 "
-                         "networks), surrogate (Michaelis-Menten sweep -> MLP -> W1 gate)")
+                         "networks), surrogate (Michaelis-Menten sweep -> MLP -> W1 gate), "
+                         "dimer-system (define_system! arithmetic), dimer-expr (Rate::Expr rates), "
+                         "sir-events (nb_steps = 0)")
     ap.add_argument("--surrogate-points", type=int, default=100_000)
     ap.add_argument("--surrogate-train-steps", type=int, default=12_000)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -430,7 +596,8 @@ def main() -> int:
     F = wl.ops_per_event()
     achieved = ev_per_launch * F / avg_kernel_s / 1e12
     peak = PEAK_FP64_OPS / 1e12
-    stem = {"dimer": "ssa_dimer_1e6", "sir": "ssa_sir_1e4", "syn": "ssa_syn", "het": "ssa_het"}[wl.name]
+    stem = {"dimer": "ssa_dimer_1e6", "sir": "ssa_sir_1e4", "syn": "ssa_syn", "het": "ssa_het"}.get(
+        wl.name, "ssa_" + wl.name.replace("-", "_"))
     traffic = parse_traffic(ROOT / "profiles" / f"{stem}_traffic.txt")
     line = {
         "metric": "SSA reaction events/sec on 1 MI355X",
@@ -464,13 +631,20 @@ def main() -> int:
             "traffic": traffic,
             "ops_per_event": F,
             "kernel_ms": avg_kernel_s * 1e3,
-            "timed": "SSA launch sequence: main kernel + tail launch + int64 sample conversion (HIP events)",
+            "timed": ("all-events mode: the counting and the recording pass (HIP events); value over "
+                      "wall time includes the device-to-host copy of the records"
+                      if isinstance(wl, EventsNetwork) else
+                      "SSA launch sequence: main kernel + tail launch + int64 sample conversion (HIP events)"),
             "measured_issue": parse_issue_counters(ROOT / "profiles" / f"{stem}_counters.md"),
         },
     }
     if not args.no_cpu_baseline:
         jobs, note = wl.cpu_jobs()
-        line["cpu_baseline"] = cpu_baseline(jobs, note)
+        if isinstance(wl, EventsNetwork):
+            line["cpu_baseline"] = cpu_baseline_events(wl, note)
+        else:
+            make_net = wl.cpu_make_net() if hasattr(wl, "cpu_make_net") else oracle_lma_net
+            line["cpu_baseline"] = cpu_baseline(jobs, note, make_net)
         line["speedup_vs_cpu_baseline"] = value / line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)
     return 0

@@ -338,8 +338,9 @@ class ExprNetwork(SingleNetwork):
 
 class EventsNetwork(Workload):
     """The nb_steps == 0 mode (src/pyo3_gillespie.rs:209-223): every event of
-    every trajectory recorded, two passes (record counts, then the records),
-    into host buffers (rebop_run_all_events)."""
+    every trajectory recorded, two passes (record counts, then the records)
+    on device buffers (rebop_run_all_events_device), the offsets an exclusive
+    scan of the counts on the device."""
 
     def __init__(self, name, spec, tmax, n, description, cpu_n):
         self.name, self.spec, self.tmax, self.n = name, spec, tmax, n
@@ -347,34 +348,40 @@ class EventsNetwork(Workload):
         self.records = 0
 
     def setup(self):
+        import torch
+
         from rebop_amd import networks
 
         self.g = networks.build(self.spec, 0)
+        self.stream = torch.cuda.current_stream().cuda_stream
+        self.counts = torch.zeros(self.n, dtype=torch.int64, device="cuda")  # uint64 bits
+        self.cap = 0
 
     def step(self, seed):
-        import ctypes
+        import torch
 
-        from rebop_amd import _native
-
-        lib = _native.load()
         self.g.seed(seed)
-        counts = np.zeros(self.n, dtype=np.uint64)
-        cp = counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
-        st1, st2 = _native.Stats(), _native.Stats()
-        _native.check(lib.rebop_run_all_events(self.g.handle, self.tmax, 0, self.n, cp, None, None,
-                                               ctypes.byref(st1)))
-        total = int(counts.sum())
-        if getattr(self, "_cap", 0) < total:
-            self._times = np.empty(total, dtype=np.float64)
-            self._states = np.empty((total, self.spec[0]), dtype=np.int64)
-            self._cap = total
-        _native.check(lib.rebop_run_all_events(
-            self.g.handle, self.tmax, 0, self.n, cp,
-            self._times.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-            self._states.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), ctypes.byref(st2)))
+        st1 = self.g.run_all_events_device(self.tmax, self.n, self.counts.data_ptr(), stream=self.stream)
+        offs = torch.cumsum(self.counts, 0) - self.counts
+        total = int(self.counts.sum().item())
+        if total > self.cap:
+            self.times = torch.empty(total, dtype=torch.float64, device="cuda")
+            self.states = torch.empty((total, self.spec[0]), dtype=torch.int64, device="cuda")
+            self.cap = total
+        st2 = self.g.run_all_events_device(self.tmax, self.n, self.counts.data_ptr(), offs.data_ptr(),
+                                           self.times.data_ptr(), self.states.data_ptr(), stream=self.stream)
         self.records = total
         st2.kernel_ms += st1.kernel_ms  # both passes
         return st2
+
+    def host_buffer_rate(self, seed: int = 7) -> float:
+        """events/s of the same two passes through rebop_run_all_events (host
+        buffers: the records cross PCIe), wall time."""
+        t0 = time.perf_counter()
+        ts, _ = self.g.run_all_events(self.tmax, n_traj=self.n)
+        dt = time.perf_counter() - t0
+        ev = sum(len(t) - 1 - (1 if np.isinf(t[-1]) else 0) for t in ts)
+        return ev / dt
 
     def ops_per_event(self):
         return algorithmic_ops_per_event(self.spec[1])
@@ -383,27 +390,32 @@ class EventsNetwork(Workload):
         return self.records * (8 + 8 * self.spec[0])
 
     def cpu_jobs(self):
-        return None, f"the oracle's all-events loop, one thread, trajectories 0.. of the same workload"
+        return None, (f"the oracle's all-events loop over trajectories 0.. of the same workload, "
+                      f"blocks of 64 over {host_threads()} threads")
 
 
 def cpu_baseline_events(wl: EventsNetwork, note: str) -> dict:
     """The oracle's nb_steps == 0 loop over trajectories 0, 1, ... of the
-    workload (one thread) until CPU_MIN_WALL_S has passed; events exclude the
-    initial and absorbing records."""
+    workload on the host's threads (blocks of 64), repeated over further
+    trajectories until CPU_MIN_WALL_S has passed; events exclude the initial
+    and absorbing records."""
     from oracle import pyoracle
 
+    threads = host_threads()
     net = oracle_lma_net(wl.spec)
     out = {}
-    for name, rng in (("philox_ln64_events_per_s", pyoracle.PHILOX), ("xoshiro_libm_events_per_s", pyoracle.XOSHIRO)):
+    for name, rng in (("philox_ln64_events_per_s", pyoracle.PHILOX),
+                      ("xoshiro_libm_events_per_s", pyoracle.XOSHIRO)):
         ev, j, t0 = 0, 0, time.perf_counter()
         while time.perf_counter() - t0 < CPU_MIN_WALL_S:
-            times, _ = net.run_all_events(42, j, wl.spec[2], wl.tmax, rng=rng)
-            ev += len(times) - 1 - (1 if np.isinf(times[-1]) else 0)
-            j += 1
+            e, _ = net.run_all_events_many(42, j, 64 * 16 * threads, wl.spec[2], wl.tmax, rng=rng,
+                                           nthreads=threads)
+            ev += e
+            j += 64 * 16 * threads
         out[name] = ev / (time.perf_counter() - t0)
         out[name + "_trajectories"] = j
     return {"value": max(out["philox_ln64_events_per_s"], out["xoshiro_libm_events_per_s"]),
-            "unit": "events/s", "cores": 1, "kind": "port", **out,
+            "unit": "events/s", "cores": threads, "kind": "port", **out,
             "sample": note + f", >= {CPU_MIN_WALL_S:.0f} s wall per random stream; the faster is the value"}
 
 
@@ -440,7 +452,7 @@ def make_workload(name: str, kernel: int) -> Workload:
     if name == "sir-events":
         return EventsNetwork("sir-events", networks.sir(), 250.0, 10_000,
                              "SIR (3 species, 2 reactions), 10^4 trajectories, tmax=250, nb_steps=0 "
-                             "(every event recorded; two passes into host buffers)", 0)
+                             "(every event recorded; two passes into device buffers)", 0)
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -631,13 +643,15 @@ def main() -> int:
             "traffic": traffic,
             "ops_per_event": F,
             "kernel_ms": avg_kernel_s * 1e3,
-            "timed": ("all-events mode: the counting and the recording pass (HIP events); value over "
-                      "wall time includes the device-to-host copy of the records"
+            "timed": ("all-events mode on device buffers: the counting and the recording pass "
+                      "(HIP events)"
                       if isinstance(wl, EventsNetwork) else
                       "SSA launch sequence: main kernel + tail launch + int64 sample conversion (HIP events)"),
             "measured_issue": parse_issue_counters(ROOT / "profiles" / f"{stem}_counters.md"),
         },
     }
+    if isinstance(wl, EventsNetwork):
+        line["config"]["host_buffer_events_per_s"] = wl.host_buffer_rate()
     if not args.no_cpu_baseline:
         jobs, note = wl.cpu_jobs()
         if isinstance(wl, EventsNetwork):

@@ -246,6 +246,21 @@ int rebop_run_all_events(const rebop_gillespie *g, double tmax, uint64_t traj_be
                          uint64_t n_traj, uint64_t *n_records, double *times, int64_t *states,
                          rebop_stats *stats);
 
+/* rebop_run_all_events on device buffers, stream-ordered on `stream`
+ * (hipStream_t, NULL = default), for callers whose records stay on the GPU.
+ * Pass 1 (d_offsets, d_times, d_states NULL): d_n_records[j] = trajectory
+ * j's record count.  Pass 2: d_offsets[j] = sum_{i<j} d_n_records[i] (the
+ * caller's exclusive scan of pass 1's counts, which d_n_records must still
+ * hold) and the records go to d_times[offset + r] and
+ * d_states[offset + r][nb_species]; no trajectory writes more than
+ * d_n_records[j] records, and a count that does not match its trajectory is
+ * refused (REBOP_ERR_VALUE).  Same records, bit for bit, as the host
+ * entry point.  Returns once the pass has finished. */
+int rebop_run_all_events_device(const rebop_gillespie *g, double tmax, uint64_t traj_begin,
+                                uint64_t n_traj, uint64_t *d_n_records, const uint64_t *d_offsets,
+                                double *d_times, int64_t *d_states, void *stream,
+                                rebop_stats *stats);
+
 /* One network's run inside rebop_run_jobs_device. */
 typedef struct rebop_job {
   const rebop_gillespie *net; /* network, initial species and seed */

@@ -631,6 +631,104 @@ int orc_run_all_events(const orc_net *net, uint64_t seed, uint64_t traj, const i
   return ORC_OK;
 }
 
+/* orc_run_all_events over n_traj trajectories, pooled over threads (blocks
+ * of ORC_CHUNK): each thread records its trajectories into a buffer of its
+ * own (the records are written, then dropped: a timing harness); events and
+ * records are summed. */
+typedef struct {
+  const orc_net *net;
+  uint64_t seed, traj_begin, n_traj;
+  const int64_t *x0;
+  double tmax;
+  int rng_kind;
+  atomic_uint_fast64_t next;
+  atomic_uint_fast64_t events;
+  atomic_uint_fast64_t records;
+  atomic_int err;
+} events_job;
+
+static void *events_worker(void *arg) {
+  events_job *J = (events_job *)arg;
+  const uint32_t S = J->net->S;
+  uint64_t cap = 4096;
+  double *times = (double *)malloc(sizeof(double) * cap);
+  int64_t *xs = (int64_t *)malloc(sizeof(int64_t) * cap * (S ? S : 1));
+  uint64_t ev = 0, recs = 0;
+  if (!times || !xs) {
+    atomic_store(&J->err, ORC_ERR_ALLOC);
+    free(times);
+    free(xs);
+    return NULL;
+  }
+  for (;;) {
+    const uint64_t b = atomic_fetch_add(&J->next, ORC_CHUNK);
+    if (b >= J->n_traj) break;
+    const uint64_t e = b + ORC_CHUNK < J->n_traj ? b + ORC_CHUNK : J->n_traj;
+    for (uint64_t j = b; j < e; ++j) {
+      uint64_t n = 0;
+      int rc = orc_run_all_events(J->net, J->seed, J->traj_begin + j, J->x0, J->tmax, cap, times, xs,
+                                  &n, J->rng_kind);
+      if (rc == ORC_OK && n > cap) { /* longer than the buffer: grow it, run again */
+        free(times);
+        free(xs);
+        cap = n;
+        times = (double *)malloc(sizeof(double) * cap);
+        xs = (int64_t *)malloc(sizeof(int64_t) * cap * (S ? S : 1));
+        if (!times || !xs) rc = ORC_ERR_ALLOC;
+        else rc = orc_run_all_events(J->net, J->seed, J->traj_begin + j, J->x0, J->tmax, cap, times, xs,
+                                     &n, J->rng_kind);
+      }
+      if (rc != ORC_OK) {
+        atomic_store(&J->err, rc);
+        break;
+      }
+      /* records = the initial state, one per event, and one t = +inf record
+       * for an absorbing state */
+      ev += n - 1 - (isinf(times[n - 1]) ? 1 : 0);
+      recs += n;
+    }
+  }
+  atomic_fetch_add(&J->events, ev);
+  atomic_fetch_add(&J->records, recs);
+  free(times);
+  free(xs);
+  return NULL;
+}
+
+int orc_run_all_events_many(const orc_net *net, uint64_t seed, uint64_t traj_begin, uint64_t n_traj,
+                            const int64_t *x0, double tmax, uint64_t *events, uint64_t *records,
+                            int rng_kind, int nthreads) {
+  events_job J;
+  J.net = net;
+  J.seed = seed;
+  J.traj_begin = traj_begin;
+  J.n_traj = n_traj;
+  J.x0 = x0;
+  J.tmax = tmax;
+  J.rng_kind = rng_kind;
+  atomic_init(&J.next, 0);
+  atomic_init(&J.events, 0);
+  atomic_init(&J.records, 0);
+  atomic_init(&J.err, 0);
+  if (nthreads <= 1) {
+    events_worker(&J);
+  } else {
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    if (!th) return ORC_ERR_ALLOC;
+    int started = 0;
+    for (int i = 0; i < nthreads; ++i) {
+      if (pthread_create(&th[i], NULL, events_worker, &J) != 0) break;
+      started++;
+    }
+    if (started == 0) events_worker(&J);
+    for (int i = 0; i < started; ++i) pthread_join(th[i], NULL);
+    free(th);
+  }
+  if (events) *events = atomic_load(&J.events);
+  if (records) *records = atomic_load(&J.records);
+  return atomic_load(&J.err);
+}
+
 /* ------------------------------------------------------------------ */
 
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {

@@ -133,6 +133,15 @@ int orc_run_all_events(const orc_net *net, uint64_t seed, uint64_t traj, const i
                        double tmax, uint64_t cap, double *times, int64_t *xs,
                        uint64_t *n_out, int rng_kind);
 
+/* orc_run_all_events for trajectories traj_begin .. traj_begin + n_traj - 1
+ * over nthreads threads (a timing harness: each thread writes the records
+ * into a buffer of its own and drops them); *events and *records receive the
+ * totals (events exclude the initial record and an absorbing state's
+ * t = +inf record). */
+int orc_run_all_events_many(const orc_net *net, uint64_t seed, uint64_t traj_begin, uint64_t n_traj,
+                            const int64_t *x0, double tmax, uint64_t *events, uint64_t *records,
+                            int rng_kind, int nthreads);
+
 /* RNG primitives, exported so tests can pin them. */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 double orc_ln64(double x);

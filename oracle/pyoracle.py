@@ -71,6 +71,11 @@ def load() -> ctypes.CDLL:
             [c_void_p, c_uint64, c_uint64, P(c_int64), c_double, c_uint64, P(c_double),
              P(c_int64), P(c_uint64), c_int],
         ),
+        "orc_run_all_events_many": (
+            c_int,
+            [c_void_p, c_uint64, c_uint64, c_uint64, P(c_int64), c_double, P(c_uint64), P(c_uint64), c_int,
+             c_int],
+        ),
         "orc_philox4x32_10": (None, [P(c_uint32), P(c_uint32), P(c_uint32)]),
         "orc_ln64": (c_double, [c_double]),
         "orc_exp64": (c_double, [c_double]),
@@ -225,6 +230,19 @@ class OracleNet:
             if n.value <= cap:
                 return times[: n.value], xs[: n.value]
             cap = int(n.value)
+
+    def run_all_events_many(self, seed, traj_begin, n_traj, x0, tmax, rng=PHILOX, nthreads=1):
+        """(events, records) of run_all_events over n_traj trajectories on
+        nthreads threads (the records are written and dropped)."""
+        x0a = np.ascontiguousarray(np.asarray(x0, dtype=np.int64).reshape(-1))
+        ev = c_uint64(0)
+        rec = c_uint64(0)
+        rc = self.lib.orc_run_all_events_many(self.h, int(seed) & (2**64 - 1), int(traj_begin), int(n_traj),
+                                              _p(x0a, c_int64), float(tmax), ctypes.byref(ev), ctypes.byref(rec),
+                                              int(rng), int(nthreads))
+        if rc:
+            raise OracleError(rc)
+        return int(ev.value), int(rec.value)
 
     def state(self, x0, seed, traj=0, rng=PHILOX) -> "OracleTrajectory":
         return OracleTrajectory(self, x0, seed, traj, rng)

@@ -87,6 +87,10 @@ SIGNATURES: dict[str, tuple] = {
         c_int,
         [c_void_p, c_double, c_uint64, c_uint64, POINTER(c_uint64), POINTER(c_double), POINTER(c_int64), c_void_p],
     ),
+    "rebop_run_all_events_device": (
+        c_int,
+        [c_void_p, c_double, c_uint64, c_uint64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    ),
     "rebop_autotune": (c_int, [c_void_p, c_double, c_uint64, c_uint64, c_void_p]),
     "rebop_run_sweep_device": (
         c_int,

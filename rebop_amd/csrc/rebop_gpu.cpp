@@ -1307,6 +1307,49 @@ int rebop_run_all_events(const rebop_gillespie *gc, double tmax, uint64_t traj_b
   return REBOP_OK;
 }
 
+int rebop_run_all_events_device(const rebop_gillespie *gc, double tmax, uint64_t traj_begin,
+                                uint64_t n_traj, uint64_t *d_n_records, const uint64_t *d_offsets,
+                                double *d_times, int64_t *d_states, void *stream,
+                                rebop_stats *stats) {
+  if (int rc = check_handle(gc, "gillespie")) return rc;
+  if (!d_n_records && n_traj) return fail(REBOP_ERR_VALUE, "d_n_records is NULL");
+  if (std::isinf(tmax) && tmax > 0)
+    return fail(REBOP_ERR_VALUE, "tmax must be finite: every event up to +inf never ends");
+  const bool want = d_offsets != nullptr || d_times != nullptr || d_states != nullptr;
+  if (want && n_traj && (!d_offsets || !d_times || !d_states))
+    return fail(REBOP_ERR_VALUE, "pass 2 needs d_offsets, d_times and d_states");
+  rebop_gillespie *g = const_cast<rebop_gillespie *>(gc);
+  if (int rc = check_x0(g->x0.data(), g->x0.size())) return rc;
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  int rc = ensure_device();
+  if (rc) return rc;
+  rc = upload_net(g);
+  if (rc) return rc;
+  rebop_dev_run run{};
+  run.seed = g->seed;
+  run.traj_begin = traj_begin;
+  run.n_traj = n_traj;
+  run.tmax = tmax;
+  run.nb_steps = 0;
+  run.mode = REBOP_MODE_EVENTS;
+  run.x0 = (const int64_t *)g->dx0.p;
+  run.x0_div = 0;
+  run.n_records = d_n_records;
+  if (want) {
+    run.rec_offsets = d_offsets;
+    run.rec_times = d_times;
+    run.rec_states = d_states;
+  }
+  rebop_stats local{};
+  rc = launch_run(g, run, REBOP_KERNEL_AUTO, (hipStream_t)stream, &local);
+  if (want && rc == REBOP_ERR_RANGE)
+    return fail(REBOP_ERR_VALUE, "d_n_records does not match this network and seed (or a count "
+                                 "left the exactly simulated range)");
+  if (rc) return rc;
+  if (stats) *stats = local;
+  return REBOP_OK;
+}
+
 int rebop_batch_new(const rebop_gillespie *g, uint64_t traj_begin, uint64_t n_traj,
                     const int64_t *x0, rebop_batch **out) {
   if (!out) return fail(REBOP_ERR_VALUE, "out is NULL");

@@ -458,10 +458,13 @@ __device__ __forceinline__ LaneTotals lane_events(Net &net, const rebop_dev_run 
     if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
     const bool write = run.rec_offsets != nullptr;
     const uint64_t base = write ? run.rec_offsets[jl] : 0ull;
+    // the caller's count bounds the writes: a count that does not match this
+    // trajectory is reported (below), never written past
+    const uint64_t cap = write ? run.n_records[jl] : 0ull;
     double t = 0.0;
     uint64_t draw = 0, rec = 0;
     auto emit = [&]() {
-      if (write) {
+      if (write && rec < cap) {
         run.rec_times[base + rec] = t;
         net.write_counts(x, run.rec_states + (base + rec) * S, 1);
       }

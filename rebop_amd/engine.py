@@ -322,6 +322,18 @@ class GillespieNet:
             return ts[0], xs[0]
         return ts, xs
 
+    def run_all_events_device(self, tmax: float, n_traj: int, d_counts: int, d_offsets: int | None = None,
+                              d_times: int | None = None, d_states: int | None = None, traj_begin: int = 0,
+                              stream=None) -> "_native.Stats":
+        """The nb_steps == 0 loop on device buffers (rebop_run_all_events_device):
+        pass 1 fills ``d_counts`` (device pointer, uint64 [n_traj]); pass 2, given
+        ``d_offsets`` (its exclusive scan), writes the records to ``d_times`` and
+        ``d_states`` [total][S]."""
+        st = _native.Stats()
+        check(self._lib.rebop_run_all_events_device(self._h, float(tmax), int(traj_begin), int(n_traj), d_counts,
+                                                    d_offsets, d_times, d_states, stream, ctypes.byref(st)))
+        return st
+
     def run_sweep_device(self, n_points: int, traj_per_point: int, tmax: float, nb_steps: int,
                          out_ptr: int | None, rates_ptr: int | None = None,
                          x0_ptr: int | None = None, final_ptr: int | None = None,

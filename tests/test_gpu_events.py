@@ -31,6 +31,41 @@ def test_all_events_bit_exact(name, spec, tmax, n):
         assert np.array_equal(xs[j], ox), (name, j)
 
 
+@pytest.mark.parametrize("name,spec,tmax,n", [("sir", sir(), 250.0, 500), ("dimers", dimers(), 0.05, 64)])
+def test_all_events_device_buffers_equal_host(name, spec, tmax, n):
+    """rebop_run_all_events_device (counts, a device scan for the offsets,
+    records in device memory) writes the host entry point's records bit for
+    bit; a pass 2 with counts that do not match is refused."""
+    import torch
+
+    g = build_device(spec, 23)
+    ts, xs = g.run_all_events(tmax, n_traj=n, traj_begin=9)
+    counts = torch.zeros(n, dtype=torch.int64, device="cuda")
+    st1 = g.run_all_events_device(tmax, n, counts.data_ptr(), traj_begin=9)
+    offs = torch.cumsum(counts, 0) - counts
+    total = int(counts.sum().item())
+    times = torch.full((total,), float("nan"), dtype=torch.float64, device="cuda")
+    states = torch.full((total, spec[0]), -1, dtype=torch.int64, device="cuda")
+    st2 = g.run_all_events_device(tmax, n, counts.data_ptr(), offs.data_ptr(), times.data_ptr(),
+                                  states.data_ptr(), traj_begin=9)
+    torch.cuda.synchronize()
+    c = counts.cpu().numpy()
+    assert c.tolist() == [len(t) for t in ts]
+    assert st1.events == st2.events == sum(len(t) - 1 - (1 if np.isinf(t[-1]) else 0) for t in ts)
+    o = np.concatenate([[0], np.cumsum(c)])
+    tt, ss = times.cpu().numpy(), states.cpu().numpy()
+    for j in range(n):
+        assert np.array_equal(tt[o[j]:o[j + 1]].view(np.uint64), ts[j].view(np.uint64)), (name, j)
+        assert np.array_equal(ss[o[j]:o[j + 1]], xs[j]), (name, j)
+    counts[0] += 1  # no longer this network's count
+    offs = torch.cumsum(counts, 0) - counts
+    big_t = torch.empty(total + 1, dtype=torch.float64, device="cuda")
+    big_s = torch.empty((total + 1, spec[0]), dtype=torch.int64, device="cuda")
+    with pytest.raises(_native.RebopValueError):
+        g.run_all_events_device(tmax, n, counts.data_ptr(), offs.data_ptr(), big_t.data_ptr(), big_s.data_ptr(),
+                                traj_begin=9)
+
+
 def test_all_events_absorbing_records_inf():
     g = GillespieNet([3], seed=2)
     g.add_reaction(Rate.lma(1.0, [1]), [-1])

@@ -258,6 +258,25 @@ def test_all_events_invariants(oracle):
         assert set(d[:, 0]) <= {-1, 0} and set(d[:, 1]) <= {-1, 1} and set(d[:, 2]) <= {0, 1}
 
 
+def test_all_events_many_sums_the_single_trajectory_loop(oracle):
+    """The threaded timing harness of the nb_steps == 0 loop counts what the
+    per-trajectory entry point returns, for any thread count, including
+    trajectories longer than its initial buffer and absorbed ones."""
+    n = build_oracle(sir())
+    ev, rec = 0, 0
+    for j in range(150):
+        t, _ = n.run_all_events(5, 7 + j, [999, 1, 0], 250.0)
+        ev += len(t) - 1 - (1 if np.isinf(t[-1]) else 0)
+        rec += len(t)
+    for th in (1, 3, 8):
+        assert n.run_all_events_many(5, 7, 150, [999, 1, 0], 250.0, nthreads=th) == (ev, rec)
+    big = build_oracle(dimers())
+    t, _ = big.run_all_events(1, 0, dimers()[2], 0.5)
+    assert len(t) > 4096  # grows the harness's buffer
+    e1, r1 = big.run_all_events_many(1, 0, 1, dimers()[2], 0.5, nthreads=2)
+    assert r1 == len(t) and e1 == len(t) - 1 - (1 if np.isinf(t[-1]) else 0)
+
+
 # ---- exact laws ----------------------------------------------------------
 
 

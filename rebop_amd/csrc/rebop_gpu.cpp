@@ -618,10 +618,10 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
     if (!spec)
       return fail(REBOP_ERR_UNSUPPORTED,
                   "lanes per trajectory > 1 needs the specialised kernel (REBOP_KERNEL_SPECIALIZED)");
-    if (variant == REBOP_KERNEL_PERSISTENT || run.mode == REBOP_MODE_EVENTS || run.rates)
+    if (variant == REBOP_KERNEL_PERSISTENT || run.rates)
       return fail(REBOP_ERR_UNSUPPORTED,
-                  "lanes per trajectory > 1 applies to plain grid runs without rate tables");
-  } else if (variant == REBOP_KERNEL_AUTO && spec && run.mode != REBOP_MODE_EVENTS && !run.rates) {
+                  "lanes per trajectory > 1 applies to plain runs without rate tables");
+  } else if (variant == REBOP_KERNEL_AUTO && spec && !run.rates) {
     const uint64_t lanes = (uint64_t)g_dev.cus * 4u * 64u;
     if (run.n_traj * 4u <= lanes)
       group = 4;

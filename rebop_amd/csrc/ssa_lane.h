@@ -496,6 +496,78 @@ __device__ __forceinline__ LaneTotals lane_events(Net &net, const rebop_dev_run 
   return acc;
 }
 
+// lane_events with K lanes per trajectory (K = 2 or 4; plain launches:
+// trajectory j is group j), for runs too small to give every SIMD more than
+// one wave: lane `sub` of a group computes the Philox block and exponential
+// of draw (draw + sub), and the group runs up to K events of the loop above
+// using block q for its q-th event (read from lane q by DPP) -- each event
+// takes the next draw and an absorbing state takes none, so the blocks of a
+// round are used in order or not at all (the argument of lane_loop_group).
+// All lanes hold the same state; lane 0 of the group writes and counts.
+template <class Net, int K>
+__device__ __forceinline__ LaneTotals lane_events_group(Net &net, const rebop_dev_run &run, uint64_t gtid) {
+  LaneTotals acc;
+  const uint64_t jl = gtid / K;
+  if (jl >= run.n_traj) return acc;
+  const uint32_t sub = __lane_id() % K;
+  const bool lead = sub == 0;
+  const uint32_t S = net.S();
+  double x[Net::MS];
+  double cum[Net::MR];
+#pragma unroll
+  for (int s = 0; s < Net::MS; ++s) x[s] = 0.0;
+#pragma unroll
+  for (int r = 0; r < Net::MR; ++r) cum[r] = 0.0;
+  net.start(run, jl);
+  net.load_x0(x, run.x0_div ? run.x0 + (jl / run.x0_div) * S : run.x0);
+  if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
+  const bool write = run.rec_offsets != nullptr;
+  const uint64_t base = write ? run.rec_offsets[jl] : 0ull;
+  const uint64_t cap = write ? run.n_records[jl] : 0ull;
+  double t = 0.0;
+  uint64_t draw = 0, rec = 0;
+  auto emit = [&]() {
+    if (write && lead && rec < cap) {
+      run.rec_times[base + rec] = t;
+      net.write_counts(x, run.rec_states + (base + rec) * S, 1);
+    }
+    rec += 1;
+  };
+  emit();
+  while (t < run.tmax) {
+    const rebop_u32x4 blk = rebop_draw_block(run.seed, run.traj_begin + jl, draw + sub);
+    const double e_own = rebop_exp1_of(blk), u_own = rebop_unif_of(blk);
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const double e = group_read<K>(e_own, q);
+      const double u = group_read<K>(u_own, q);
+      if (t < run.tmax) {
+        const double tot = net.props(x, cum);
+        if (!(0.0 < tot)) {
+          t = __builtin_inf();
+        } else if (tot == __builtin_inf()) {
+          acc.flags |= REBOP_FLAG_RANGE; // time would stop (E / tot == 0)
+          t = __builtin_inf();
+        } else {
+          draw += 1;
+          t = t + e / tot;
+          net.affect(x, net.choose(x, cum, tot * u));
+          if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
+        }
+        emit();
+      }
+    }
+  }
+  if (lead) {
+    if (!write) run.n_records[jl] = rec;
+    else if (run.n_records[jl] != rec) acc.flags |= REBOP_FLAG_RANGE;
+    acc.events += draw;
+    acc.draws += draw;
+    acc.trajectories += 1;
+  }
+  return acc;
+}
+
 // Sums lane totals over the workgroup in LDS (blk_ctr zeroed before use).
 __device__ __forceinline__ void add_totals(const LaneTotals &acc, unsigned long long *blk_ctr) {
   atomicAdd(&blk_ctr[0], (unsigned long long)acc.events);

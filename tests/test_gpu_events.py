@@ -31,6 +31,21 @@ def test_all_events_bit_exact(name, spec, tmax, n):
         assert np.array_equal(xs[j], ox), (name, j)
 
 
+@pytest.mark.parametrize("n", [20_000, 70_000])
+def test_all_events_lanes_per_trajectory(n):
+    """Small runs take 4 lanes per trajectory (the bit-exact test above), a
+    run of 20 000 trajectories 2 lanes and one of 70 000 a single lane
+    (one wave per SIMD at most): the records equal the oracle's in each."""
+    spec = sir()
+    g = build_device(spec, 31)
+    ts, xs = g.run_all_events(40.0, n_traj=n)
+    o = build_oracle(spec)
+    for j in list(range(0, n, 997)) + [n - 1]:
+        ot, ox = o.run_all_events(31, j, spec[2], 40.0)
+        assert np.array_equal(ts[j].view(np.uint64), ot.view(np.uint64)), j
+        assert np.array_equal(xs[j], ox), j
+
+
 @pytest.mark.parametrize("name,spec,tmax,n", [("sir", sir(), 250.0, 500), ("dimers", dimers(), 0.05, 64)])
 def test_all_events_device_buffers_equal_host(name, spec, tmax, n):
     """rebop_run_all_events_device (counts, a device scan for the offsets,

@@ -503,6 +503,7 @@ def surrogate_main(args) -> int:
             "w1_pass_fraction_reproducible": res["w1_pass_fraction_reproducible"],
             "ssa_self_w1_over_2pct": res["ssa_self_w1_over_2pct"],
             "trusted_cells": res["trusted_cells"],
+            "trusted_validated_cells": res["trusted_validated_cells"],
             "rel_w1_median": res["rel_w1_median"],
             "train_s": res["train_s"],
             "nll_first": res["nll_first"],

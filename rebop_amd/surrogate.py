@@ -491,10 +491,14 @@ def w1_to_samples(params: np.ndarray, samples: np.ndarray, bound: float | None =
 
 @dataclass
 class TrustMap:
-    """Cells of (log kf, log kb, log kcat, E0, S0) where the surrogate passed."""
+    """Where the surrogate may answer: per cell of (log kf, log kb, log kcat,
+    E0, S0), validated sample time and species, whether every validated point
+    of the cell passed there."""
 
     bins: int
-    passed: np.ndarray  # bool [bins]^3 x 2 x 2
+    times: tuple  # the validated sample indices
+    passed: np.ndarray  # bool [bins]^3 x 2 x 2 x len(times) x S
+    seen: np.ndarray  # bool [bins]^3 x 2 x 2: cells holding a validated point
 
     @staticmethod
     def cell(rates: np.ndarray, x0: np.ndarray, bins: int) -> tuple:
@@ -503,16 +507,20 @@ class TrustMap:
         s = (x0[:, 1] > 525).astype(int)
         return lk[:, 0], lk[:, 1], lk[:, 2], e, s
 
-    def trusted(self, rates: np.ndarray, x0: np.ndarray) -> np.ndarray:
-        return self.passed[self.cell(rates, x0, self.bins)]
+    def trusted(self, rates: np.ndarray, x0: np.ndarray, sample: int, species: int) -> np.ndarray:
+        """Per query point: may the surrogate answer for this species at the
+        validated sample time nearest to `sample` (at or after it)?"""
+        ti = min(int(np.searchsorted(np.asarray(self.times), sample)), len(self.times) - 1)
+        return self.passed[self.cell(rates, x0, self.bins) + (ti, species)]
 
 
 def validate(mlp: MLP, rates: np.ndarray, x0: np.ndarray, n_ref: int, seed: int, tol: float = 0.02,
              bins: int = 2, times=(4, 8, 16, 32)):
     """W1 of the surrogate against fresh n_ref-trajectory SSA histograms.
 
-    Returns (relative W1 [P][len(times)][S], TrustMap): a cell is trusted if
-    every validated (point, time, species) in it has W1 <= tol * mean.
+    Returns (relative W1 [P][len(times)][S], TrustMap): (cell, time,
+    species) is trusted if every validated point of the cell has W1 <= tol *
+    mean there.
     """
     data = generate(rates, x0, n_ref, seed=seed, traj_begin=1 << 40)
     samp = data.samples.cpu().numpy()
@@ -526,16 +534,16 @@ def validate(mlp: MLP, rates: np.ndarray, x0: np.ndarray, n_ref: int, seed: int,
                 w1 = w1_to_samples(params[p, s], xs, bounds[p, s])
                 mean = xs.mean()
                 rel[p, ti, s] = w1 / mean if mean > 0 else (0.0 if w1 <= tol else np.inf)
-    ok = (rel <= tol).all(axis=(1, 2))
-    passed = np.ones((bins,) * 3 + (2, 2), dtype=bool)
-    seen = np.zeros_like(passed)
+    ok = rel <= tol  # [P][T][S]
+    passed = np.ones((bins,) * 3 + (2, 2, len(times), S), dtype=bool)
+    seen = np.zeros((bins,) * 3 + (2, 2), dtype=bool)
     cells = TrustMap.cell(rates, x0, bins)
     for p in range(P):
         c = tuple(ci[p] for ci in cells)
         seen[c] = True
         passed[c] &= ok[p]
-    passed &= seen
-    return rel, TrustMap(bins, passed), data.stats
+    passed &= seen[..., None, None]
+    return rel, TrustMap(bins, tuple(times), passed, seen), data.stats
 
 
 def ssa_noise_floor(rates: np.ndarray, x0: np.ndarray, n_ref: int, seed: int,
@@ -634,6 +642,7 @@ def benchmark(n_train: int = 100_000, steps: int = 12_000, batch: int = 8192, n_
     return dict(surrogate_qps=sg_qps, ssa_qps=ssa_qps, speedup=sg_qps / ssa_qps, surrogate_ms=sg_ms,
                 ssa_ms=ssa_ms, w1_pass_fraction=pass_frac, point_pass_fraction=point_pass,
                 w1_pass_fraction_reproducible=pass_repro, ssa_self_w1_over_2pct=float(1 - repro.mean()),
-                trusted_cells=float(trust.passed.mean()), train_s=train_s, gen_s=gen_s, val_s=val_s,
+                trusted_cells=float(trust.passed.mean()),
+                trusted_validated_cells=float(trust.passed[trust.seen].mean()), train_s=train_s, gen_s=gen_s, val_s=val_s,
                 nll_first=hist[0][1], nll_last=hist[-1][1], gemm_tflops=flops / sg_ms / 1e9,
                 train_events=int(vstats.events), rel_w1_median=float(np.median(rel)))

@@ -559,7 +559,7 @@ def main() -> int:
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the SSA shards by trajectory id with no "
-              f"communication, and this build runs on a single MI355X (--gpus {args.gpus} "
+              f"communication, and this engine runs on a single MI355X (--gpus {args.gpus} "
               "is not supported)", file=sys.stderr)
         return 2
 

@@ -8,11 +8,11 @@ launch over every parameter point) by the project's own device kernels
 (include/rebop_surrogate.h): BF16 matrix-core GEMMs with FP32 accumulation
 for the dense layers, an FP64 mixture-likelihood head, Adam.
 
-The SSA stays the judge and the fallback: ``TrustMap`` validates the
+The SSA stays the reference and the fallback: ``TrustMap`` validates the
 surrogate against fresh SSA histograms (10^4 trajectories per point) on a
-grid of parameter cells; a query is answered by the surrogate only inside
-cells where every validated Wasserstein-1 distance was <= 2 % of the species
-mean, and by the SSA elsewhere.
+grid of parameter cells; a query is answered by the surrogate only for the
+(cell, sample time, species) where every validated Wasserstein-1 distance
+was <= 2 % of the species mean, and by the SSA elsewhere.
 """
 
 from __future__ import annotations

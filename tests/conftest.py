@@ -29,5 +29,5 @@ def oracle():
 
 @pytest.fixture(scope="session")
 def nthreads():
-    # the GPU box gives a 16-thread CPU share; the container has 8 cores
+    # at most 16 oracle threads (the CPU share next to one GPU)
     return max(1, min(16, os.cpu_count() or 1))

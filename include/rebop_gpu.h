@@ -246,7 +246,8 @@ int rebop_run_all_events(const rebop_gillespie *g, double tmax, uint64_t traj_be
                          uint64_t n_traj, uint64_t *n_records, double *times, int64_t *states,
                          rebop_stats *stats);
 
-/* rebop_run_all_events on device buffers, stream-ordered on `stream`
+/* rebop_run_all_events (src/pyo3_gillespie.rs:209-223 over
+ * src/gillespie.rs:275-297) on device buffers, stream-ordered on `stream`
  * (hipStream_t, NULL = default), for callers whose records stay on the GPU.
  * Pass 1 (d_offsets, d_times, d_states NULL): d_n_records[j] = trajectory
  * j's record count.  Pass 2: d_offsets[j] = sum_{i<j} d_n_records[i] (the

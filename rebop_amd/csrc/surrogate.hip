@@ -647,10 +647,12 @@ __global__ void __launch_bounds__(512) mlp_infer_kernel(int M, const __bf16 *__r
   __shared__ double lg[FEAT ? MLP_BM : 1][8]; // the items' logs, shared by their four threads
   __shared__ __attribute__((aligned(16))) float Bs[MLP_MAXL * MLP_MAXW]; // every layer's bias
   const int tid = threadIdx.x, m0 = blockIdx.x * MLP_BM, K0 = net.dims[0];
-  for (int l = 0, o = 0; l < net.depth; o += net_dim(net, l + 1), ++l) {
-    const float *b = net_bias(net, l);
-    if (tid < net_dim(net, l + 1)) Bs[o + tid] = b[tid];
-  }
+  // every layer's bias: loaded into registers here, stored to LDS after the
+  // inputs, so its latency overlaps theirs
+  float bv[MLP_MAXL];
+#pragma unroll
+  for (int l = 0; l < MLP_MAXL; ++l)
+    bv[l] = l < net.depth && tid < net.dims[l + 1] ? net.bias[l][tid] : 0.f;
   if (FEAT) {
     // four threads per item: thread q computes logs 2q, 2q+1 (of seven),
     // which the item's threads share through LDS, then features 4q .. 4q+3
@@ -693,6 +695,9 @@ __global__ void __launch_bounds__(512) mlp_infer_kernel(int M, const __bf16 *__r
       *(bf16x8 *)(Hs + hs_offset(r, c)) = v;
     }
   }
+#pragma unroll
+  for (int l = 0, o = 0; l < MLP_MAXL; o += net.dims[l + 1], ++l)
+    if (l < net.depth && tid < net.dims[l + 1]) Bs[o + tid] = bv[l];
   __syncthreads();
   const float(*lp)[4] = FEAT ? lnb : nullptr;
   for (int l = 0, o = 0; l < net.depth; o += net_dim(net, l + 1), ++l) {
@@ -1042,7 +1047,7 @@ int rebop_sg_pack_kslices_bf16(int K, int N, const float *W, void *Wp, void *str
 int rebop_sg_mlp_forward_bf16(int M, const void *X, int ldx, int depth, const int *dims,
                               const void *const *Wp, const float *const *bias, float *out, int ldo,
                               void *stream) {
-  sg::MlpNet net;
+  sg::MlpNet net{};
   if (mlp_net(depth, dims, Wp, bias, net) || M <= 0 || !X || !out || (uintptr_t)X % 16 ||
       (uintptr_t)out % 16 || ldx < dims[0] || ldx % 8 || ldo < dims[depth] || ldo % 4)
     return 3;
@@ -1056,7 +1061,7 @@ int rebop_sg_infer(int B, const uint32_t *item_point, const uint32_t *item_sampl
                    const double *rates, const int64_t *x0, double tmax, int nb_steps, int depth,
                    const int *dims, const void *const *Wp, const float *const *bias, float *head,
                    int ldh, void *stream) {
-  sg::MlpNet net;
+  sg::MlpNet net{};
   if (mlp_net(depth, dims, Wp, bias, net) || B <= 0 || !item_point || !item_sample || !rates ||
       !x0 || !head || (uintptr_t)head % 16 || dims[0] < REBOP_SG_FEATURES ||
       dims[depth] < 4 * 3 * REBOP_SG_COMPONENTS || ldh < dims[depth] || ldh % 4 || nb_steps <= 0)

@@ -159,6 +159,46 @@ def test_fused_forward_equals_layer_by_layer(sg, dims, M):
     assert torch.equal(out.view(torch.int32), ref.view(torch.int32))
 
 
+def test_fused_forward_limits(sg):
+    """The largest stack the kernel takes: eight layers, a 512-wide input
+    (every LDS fragment block of a row in use) and all three output widths."""
+    import torch
+
+    dims = [512, 512, 256, 512, 128, 512, 512, 256, 128]
+    rng = np.random.default_rng(29)
+    Wt, Wp, b = _stack(sg, rng, dims)
+    X = _bf(rng.normal(size=(640, dims[0])))
+    ref = _layer_by_layer(sg, X, Wt, b)
+    out = torch.full((640, dims[-1]), float("nan"), dtype=torch.float32, device="cuda")
+    sg.mlp_forward(X, Wp, b, out)
+    torch.cuda.synchronize()
+    assert torch.equal(out.view(torch.int32), ref.view(torch.int32))
+
+
+def test_fused_inference_wider_input(sg):
+    """rebop_sg_infer with a 128-wide first layer: the features fill columns
+    0..15 and the kernel zeroes 16..127, as features_kernel does."""
+    import torch
+
+    rates, x0 = sg.sample_points(40, 13)
+    d_r, d_x = torch.as_tensor(rates, device="cuda"), torch.as_tensor(x0, device="cuda")
+    rng = np.random.default_rng(14)
+    B = 704
+    pts = torch.as_tensor(rng.integers(0, 40, B).astype(np.int32), device="cuda")
+    smp = torch.as_tensor(rng.integers(1, sg.NB + 1, B).astype(np.int32), device="cuda")
+    dims = [128, 256, sg.OUT]
+    Wt, Wp, b = _stack(sg, rng, dims)
+    head = torch.full((B, sg.OUT), float("nan"), dtype=torch.float32, device="cuda")
+    sg.infer(Wp, b, pts, smp, d_r, d_x, head)
+    X = torch.empty((B, dims[0]), dtype=torch.bfloat16, device="cuda")
+    off = torch.empty((B, sg.S), dtype=torch.float32, device="cuda")
+    sg.features(pts, smp, d_r, d_x, X, off)
+    ref = _layer_by_layer(sg, X, Wt, b)
+    sg.head_offsets(ref, off)
+    torch.cuda.synchronize()
+    assert torch.equal(head.view(torch.int32), ref.view(torch.int32))
+
+
 def test_fused_forward_ragged_rows(sg):
     """A row count that is not a multiple of 64: rows past M are neither read
     nor written."""

@@ -88,12 +88,28 @@ def parse_issue_counters(path: Path):
     return out
 
 
+# The GPU host allots 16 CPU cores to each GPU (its process and thread
+# limits are set for that share, whatever os.cpu_count() reports), so the CPU
+# baseline uses at most 16 threads; the counts the host reports are printed
+# beside it.
+HOST_CORE_SHARE = 16
+
+
+def host_cpu_facts() -> dict:
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = None
+    return {"sched_getaffinity": aff, "os_cpu_count": os.cpu_count(),
+            "core_share_per_gpu": HOST_CORE_SHARE}
+
+
 def host_threads() -> int:
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    return max(1, min(HOST_CORE_SHARE, n))
 
 
 CPU_MIN_WALL_S = 1.0  # per random-stream variant: >= 16 s of CPU work on 16 threads
@@ -109,18 +125,23 @@ def oracle_lma_net(spec):
 
 
 def _time_oracle(jobs, threads: int, rng: int, make_net=oracle_lma_net):
-    """Runs the job list (repeated with further seeds until it has taken
-    CPU_MIN_WALL_S of wall time); returns (events, seconds, repetitions)."""
+    """Runs every job of the list in one thread pool (the threads take
+    trajectories of any job), repeated with further seeds until it has taken
+    CPU_MIN_WALL_S of wall time; returns (events, seconds, repetitions, the
+    fewest threads that ran trajectories in a repetition)."""
+    from oracle import pyoracle
+
     nets = [make_net(spec) for spec, *_ in jobs]
-    events, reps = 0, 0
+    events, reps, busy = 0, 0, threads
     t0 = time.perf_counter()
     while reps == 0 or time.perf_counter() - t0 < CPU_MIN_WALL_S:
-        for net, (spec, seed, tmax, nb, n, tb) in zip(nets, jobs):
-            _, _, _, ev, _ = net.run_grid(seed + 1000 * reps, spec[2], tmax, nb, n_traj=n,
-                                          traj_begin=tb, nthreads=threads, want_out=False, rng=rng)
-            events += ev
+        batch = [(net, seed + 1000 * reps, spec[2], tmax, nb, n, tb)
+                 for net, (spec, seed, tmax, nb, n, tb) in zip(nets, jobs)]
+        ev, _, b = pyoracle.run_grid_jobs(batch, rng=rng, nthreads=threads)
+        events += ev
+        busy = min(busy, b)
         reps += 1
-    return events, time.perf_counter() - t0, reps
+    return events, time.perf_counter() - t0, reps, busy
 
 
 def cpu_baseline(jobs, note: str, make_net=oracle_lma_net) -> dict:
@@ -131,14 +152,16 @@ def cpu_baseline(jobs, note: str, make_net=oracle_lma_net) -> dict:
     from oracle import pyoracle
 
     threads = host_threads()
-    ev_p, dt_p, rep_p = _time_oracle(jobs, threads, pyoracle.PHILOX, make_net)
-    ev_x, dt_x, rep_x = _time_oracle(jobs, threads, pyoracle.XOSHIRO, make_net)
+    ev_p, dt_p, rep_p, busy_p = _time_oracle(jobs, threads, pyoracle.PHILOX, make_net)
+    ev_x, dt_x, rep_x, busy_x = _time_oracle(jobs, threads, pyoracle.XOSHIRO, make_net)
     rate_p, rate_x = ev_p / dt_p, ev_x / dt_x
     return {
         "value": max(rate_p, rate_x),
         "unit": "events/s",
         "cores": threads,
         "kind": "port",
+        "threads_busy": min(busy_p, busy_x),
+        "host": host_cpu_facts(),
         "philox_ln64_events_per_s": rate_p,
         "xoshiro_libm_events_per_s": rate_x,
         "sample": f"{note}, repeated with new seeds to >= {CPU_MIN_WALL_S:.0f} s wall per "
@@ -256,10 +279,11 @@ class Heterogeneous(Workload):
         return sum((self.nb + 1) * sp[0] * 8 * self.n_traj for sp in self.specs)
 
     def cpu_jobs(self):
-        # every 64th network, 16 trajectories each
-        idx = list(range(0, self.n_networks, 64))
+        # every 32nd network, 16 trajectories each, all in one thread pool
+        idx = list(range(0, self.n_networks, 32))
         jobs = [(self.specs[i], 42, float(self.tmax[i]), self.nb, 16, 0) for i in idx]
-        return jobs, f"{len(idx)} networks (every 64th) x 16 trajectories of the same batch"
+        return jobs, (f"{len(idx)} networks (every 32nd) x 16 trajectories of the same batch, "
+                      "run concurrently in one pool")
 
 
 def system_terms(orders, diffs):
@@ -404,18 +428,21 @@ def cpu_baseline_events(wl: EventsNetwork, note: str) -> dict:
     threads = host_threads()
     net = oracle_lma_net(wl.spec)
     out = {}
+    busy = threads
     for name, rng in (("philox_ln64_events_per_s", pyoracle.PHILOX),
                       ("xoshiro_libm_events_per_s", pyoracle.XOSHIRO)):
         ev, j, t0 = 0, 0, time.perf_counter()
         while time.perf_counter() - t0 < CPU_MIN_WALL_S:
-            e, _ = net.run_all_events_many(42, j, 64 * 16 * threads, wl.spec[2], wl.tmax, rng=rng,
-                                           nthreads=threads)
+            e, _, b = net.run_all_events_many(42, j, 64 * 16 * threads, wl.spec[2], wl.tmax, rng=rng,
+                                              nthreads=threads, with_busy=True)
             ev += e
+            busy = min(busy, b)
             j += 64 * 16 * threads
         out[name] = ev / (time.perf_counter() - t0)
         out[name + "_trajectories"] = j
     return {"value": max(out["philox_ln64_events_per_s"], out["xoshiro_libm_events_per_s"]),
-            "unit": "events/s", "cores": threads, "kind": "port", **out,
+            "unit": "events/s", "cores": threads, "kind": "port", "threads_busy": busy,
+            "host": host_cpu_facts(), **out,
             "sample": note + f", >= {CPU_MIN_WALL_S:.0f} s wall per random stream; the faster is the value"}
 
 
@@ -434,7 +461,7 @@ def make_workload(name: str, kernel: int) -> Workload:
     if name == "syn":
         return SingleNetwork("syn", workloads.synthetic_large(), 10.0, 200, 2**18, kernel,
                              "synthetic network, 128 species x 512 reactions (seed 7), 2^18 "
-                             "trajectories, tmax=10, nb_steps=200", 64)
+                             "trajectories, tmax=10, nb_steps=200", 256)
     if name == "het":
         return Heterogeneous(4096, 256, 50)
     if name == "dimer-system":

@@ -495,62 +495,134 @@ double orc_grid_time(double tmax, uint64_t i, uint64_t nb_steps) {
 /* ------------------------------------------------------------------ */
 /* the grid driver, pooled over threads */
 
+/* Trajectories a worker takes per atomic claim: at most 64, fewer when the
+ * call is small, so that every thread gets work (four claims per thread at
+ * least, down to one trajectory per claim). */
+static uint64_t pool_chunk(uint64_t total, int nthreads) {
+  const uint64_t per = nthreads > 1 ? (uint64_t)nthreads * 4u : 1u;
+  uint64_t c = (total + per - 1) / per;
+  if (c < 1) c = 1;
+  if (c > 64) c = 64;
+  return c;
+}
+
 typedef struct {
-  const orc_net *net;
-  uint64_t seed, traj_begin, n_traj;
-  const int64_t *x0;
-  int x0_per_traj;
-  double tmax;
-  uint64_t nb_steps;
-  int64_t *out;
-  int64_t *final_x;
+  const orc_grid_job *jobs;
+  uint32_t n_jobs;
+  const uint64_t *chunk_end; /* [n_jobs] prefix sums of each job's chunk count */
+  uint64_t chunk;
+  uint32_t max_s, max_r;
   int rng_kind;
   atomic_uint_fast64_t next;
   atomic_uint_fast64_t events;
   atomic_uint_fast64_t draws;
+  atomic_int busy;
   atomic_int err;
-} grid_job;
+} grid_pool;
 
-#define ORC_CHUNK 64u
+/* One trajectory of a job: the pyo3 grid loop (pyo3_gillespie.rs:200-208). */
+static void grid_one(const orc_grid_job *J, uint64_t j, int rng_kind, int64_t *x, double *cum,
+                     uint64_t *ev, uint64_t *dr) {
+  const uint32_t S = J->net->S;
+  const int64_t *src = J->x0_per_traj ? J->x0 + j * S : J->x0;
+  memcpy(x, src, sizeof(int64_t) * S);
+  orc_state st;
+  orc_state_init(&st, x, J->seed, J->traj_begin + j, rng_kind);
+  for (uint64_t i = 0; i <= J->nb_steps; ++i) {
+    const double t = orc_grid_time(J->tmax, i, J->nb_steps);
+    orc_advance_until(J->net, &st, t, cum);
+    if (J->out)
+      for (uint32_t s = 0; s < S; ++s) J->out[(i * S + s) * J->n_traj + j] = x[s];
+  }
+  if (J->final_x) memcpy(J->final_x + j * S, x, sizeof(int64_t) * S);
+  *ev += st.events;
+  *dr += st.draw;
+}
 
 static void *grid_worker(void *arg) {
-  grid_job *J = (grid_job *)arg;
-  const uint32_t S = J->net->S;
-  const uint32_t R = J->net->R;
-  int64_t *x = (int64_t *)malloc(sizeof(int64_t) * (S ? S : 1));
-  double *cum = (double *)malloc(sizeof(double) * (R ? R : 1));
+  grid_pool *P = (grid_pool *)arg;
+  int64_t *x = (int64_t *)malloc(sizeof(int64_t) * (P->max_s ? P->max_s : 1));
+  double *cum = (double *)malloc(sizeof(double) * (P->max_r ? P->max_r : 1));
   if (!x || !cum) {
-    atomic_store(&J->err, ORC_ERR_ALLOC);
+    atomic_store(&P->err, ORC_ERR_ALLOC);
     free(x);
     free(cum);
     return NULL;
   }
+  const uint64_t n_chunks = P->n_jobs ? P->chunk_end[P->n_jobs - 1] : 0;
   uint64_t ev = 0, dr = 0;
+  int ran = 0;
+  uint32_t jb = 0; /* claims only grow: the job index only moves forward */
   for (;;) {
-    const uint64_t b = atomic_fetch_add(&J->next, ORC_CHUNK);
-    if (b >= J->n_traj) break;
-    const uint64_t e = b + ORC_CHUNK < J->n_traj ? b + ORC_CHUNK : J->n_traj;
-    for (uint64_t j = b; j < e; ++j) {
-      const int64_t *src = J->x0_per_traj ? J->x0 + j * S : J->x0;
-      memcpy(x, src, sizeof(int64_t) * S);
-      orc_state st;
-      orc_state_init(&st, x, J->seed, J->traj_begin + j, J->rng_kind);
-      for (uint64_t i = 0; i <= J->nb_steps; ++i) {
-        const double t = orc_grid_time(J->tmax, i, J->nb_steps);
-        orc_advance_until(J->net, &st, t, cum);
-        if (J->out)
-          for (uint32_t s = 0; s < S; ++s) J->out[(i * S + s) * J->n_traj + j] = x[s];
-      }
-      if (J->final_x) memcpy(J->final_x + j * S, x, sizeof(int64_t) * S);
-      ev += st.events;
-      dr += st.draw;
-    }
+    const uint64_t c = atomic_fetch_add(&P->next, 1);
+    if (c >= n_chunks) break;
+    while (P->chunk_end[jb] <= c) jb++;
+    const orc_grid_job *J = &P->jobs[jb];
+    const uint64_t c0 = jb ? P->chunk_end[jb - 1] : 0;
+    const uint64_t b = (c - c0) * P->chunk;
+    const uint64_t e = b + P->chunk < J->n_traj ? b + P->chunk : J->n_traj;
+    for (uint64_t j = b; j < e; ++j) grid_one(J, j, P->rng_kind, x, cum, &ev, &dr);
+    ran = 1;
   }
-  atomic_fetch_add(&J->events, ev);
-  atomic_fetch_add(&J->draws, dr);
+  atomic_fetch_add(&P->events, ev);
+  atomic_fetch_add(&P->draws, dr);
+  if (ran) atomic_fetch_add(&P->busy, 1);
   free(x);
   free(cum);
   return NULL;
+}
+
+int orc_run_grid_jobs(const orc_grid_job *jobs, uint32_t n_jobs, int rng_kind, int nthreads,
+                      uint64_t *events, uint64_t *draws, int *threads_busy) {
+  grid_pool P;
+  uint64_t total = 0;
+  for (uint32_t q = 0; q < n_jobs; ++q) {
+    if (jobs[q].nb_steps == 0) return ORC_ERR_VALUE;
+    total += jobs[q].n_traj;
+  }
+  P.jobs = jobs;
+  P.n_jobs = n_jobs;
+  P.chunk = pool_chunk(total, nthreads);
+  P.rng_kind = rng_kind;
+  P.max_s = 0;
+  P.max_r = 0;
+  uint64_t *ends = (uint64_t *)malloc(sizeof(uint64_t) * (n_jobs ? n_jobs : 1));
+  if (!ends) return ORC_ERR_ALLOC;
+  uint64_t acc = 0;
+  for (uint32_t q = 0; q < n_jobs; ++q) {
+    acc += (jobs[q].n_traj + P.chunk - 1) / P.chunk;
+    ends[q] = acc;
+    if (jobs[q].net->S > P.max_s) P.max_s = jobs[q].net->S;
+    if (jobs[q].net->R > P.max_r) P.max_r = jobs[q].net->R;
+  }
+  P.chunk_end = ends;
+  atomic_init(&P.next, 0);
+  atomic_init(&P.events, 0);
+  atomic_init(&P.draws, 0);
+  atomic_init(&P.busy, 0);
+  atomic_init(&P.err, 0);
+  if (nthreads <= 1) {
+    grid_worker(&P);
+  } else {
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    if (!th) {
+      free(ends);
+      return ORC_ERR_ALLOC;
+    }
+    int started = 0;
+    for (int i = 0; i < nthreads; ++i) {
+      if (pthread_create(&th[i], NULL, grid_worker, &P) != 0) break;
+      started++;
+    }
+    if (started == 0) grid_worker(&P);
+    for (int i = 0; i < started; ++i) pthread_join(th[i], NULL);
+    free(th);
+  }
+  free(ends);
+  if (events) *events = atomic_load(&P.events);
+  if (draws) *draws = atomic_load(&P.draws);
+  if (threads_busy) *threads_busy = atomic_load(&P.busy);
+  return atomic_load(&P.err);
 }
 
 int orc_run_grid(const orc_net *net, uint64_t seed, uint64_t traj_begin, uint64_t n_traj,
@@ -560,7 +632,7 @@ int orc_run_grid(const orc_net *net, uint64_t seed, uint64_t traj_begin, uint64_
   if (nb_steps == 0) return ORC_ERR_VALUE;
   if (times)
     for (uint64_t i = 0; i <= nb_steps; ++i) times[i] = orc_grid_time(tmax, i, nb_steps);
-  grid_job J;
+  orc_grid_job J;
   J.net = net;
   J.seed = seed;
   J.traj_begin = traj_begin;
@@ -571,28 +643,7 @@ int orc_run_grid(const orc_net *net, uint64_t seed, uint64_t traj_begin, uint64_
   J.nb_steps = nb_steps;
   J.out = out;
   J.final_x = final_x;
-  J.rng_kind = rng_kind;
-  atomic_init(&J.next, 0);
-  atomic_init(&J.events, 0);
-  atomic_init(&J.draws, 0);
-  atomic_init(&J.err, 0);
-  if (nthreads <= 1) {
-    grid_worker(&J);
-  } else {
-    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
-    if (!th) return ORC_ERR_ALLOC;
-    int started = 0;
-    for (int i = 0; i < nthreads; ++i) {
-      if (pthread_create(&th[i], NULL, grid_worker, &J) != 0) break;
-      started++;
-    }
-    if (started == 0) grid_worker(&J);
-    for (int i = 0; i < started; ++i) pthread_join(th[i], NULL);
-    free(th);
-  }
-  if (events) *events = atomic_load(&J.events);
-  if (draws) *draws = atomic_load(&J.draws);
-  return atomic_load(&J.err);
+  return orc_run_grid_jobs(&J, 1, rng_kind, nthreads, events, draws, NULL);
 }
 
 /* pyo3_gillespie.rs:209-223 for one trajectory */
@@ -632,7 +683,7 @@ int orc_run_all_events(const orc_net *net, uint64_t seed, uint64_t traj, const i
 }
 
 /* orc_run_all_events over n_traj trajectories, pooled over threads (blocks
- * of ORC_CHUNK): each thread records its trajectories into a buffer of its
+ * of pool_chunk trajectories): each thread records its trajectories into a buffer of its
  * own (the records are written, then dropped: a timing harness); events and
  * records are summed. */
 typedef struct {
@@ -641,9 +692,11 @@ typedef struct {
   const int64_t *x0;
   double tmax;
   int rng_kind;
+  uint64_t chunk;
   atomic_uint_fast64_t next;
   atomic_uint_fast64_t events;
   atomic_uint_fast64_t records;
+  atomic_int busy;
   atomic_int err;
 } events_job;
 
@@ -654,6 +707,7 @@ static void *events_worker(void *arg) {
   double *times = (double *)malloc(sizeof(double) * cap);
   int64_t *xs = (int64_t *)malloc(sizeof(int64_t) * cap * (S ? S : 1));
   uint64_t ev = 0, recs = 0;
+  int ran = 0;
   if (!times || !xs) {
     atomic_store(&J->err, ORC_ERR_ALLOC);
     free(times);
@@ -661,9 +715,10 @@ static void *events_worker(void *arg) {
     return NULL;
   }
   for (;;) {
-    const uint64_t b = atomic_fetch_add(&J->next, ORC_CHUNK);
+    const uint64_t b = atomic_fetch_add(&J->next, J->chunk);
     if (b >= J->n_traj) break;
-    const uint64_t e = b + ORC_CHUNK < J->n_traj ? b + ORC_CHUNK : J->n_traj;
+    const uint64_t e = b + J->chunk < J->n_traj ? b + J->chunk : J->n_traj;
+    ran = 1;
     for (uint64_t j = b; j < e; ++j) {
       uint64_t n = 0;
       int rc = orc_run_all_events(J->net, J->seed, J->traj_begin + j, J->x0, J->tmax, cap, times, xs,
@@ -690,6 +745,7 @@ static void *events_worker(void *arg) {
   }
   atomic_fetch_add(&J->events, ev);
   atomic_fetch_add(&J->records, recs);
+  if (ran) atomic_fetch_add(&J->busy, 1);
   free(times);
   free(xs);
   return NULL;
@@ -697,8 +753,10 @@ static void *events_worker(void *arg) {
 
 int orc_run_all_events_many(const orc_net *net, uint64_t seed, uint64_t traj_begin, uint64_t n_traj,
                             const int64_t *x0, double tmax, uint64_t *events, uint64_t *records,
-                            int rng_kind, int nthreads) {
+                            int rng_kind, int nthreads, int *threads_busy) {
   events_job J;
+  J.chunk = pool_chunk(n_traj, nthreads);
+  atomic_init(&J.busy, 0);
   J.net = net;
   J.seed = seed;
   J.traj_begin = traj_begin;
@@ -726,6 +784,7 @@ int orc_run_all_events_many(const orc_net *net, uint64_t seed, uint64_t traj_beg
   }
   if (events) *events = atomic_load(&J.events);
   if (records) *records = atomic_load(&J.records);
+  if (threads_busy) *threads_busy = atomic_load(&J.busy);
   return atomic_load(&J.err);
 }
 

@@ -116,12 +116,32 @@ void orc_advance_one_reaction(const orc_net *net, orc_state *st, double *scratch
  * final_x (nullable): [n_traj][nb_species] counts after the last grid point.
  * times: [nb_steps+1] (nullable).  events (nullable): total applied reactions.
  * nthreads <= 1 runs on the calling thread; otherwise a pool hands out
- * chunks of 64 trajectories from an atomic counter.
+ * chunks of trajectories from an atomic counter (64, or fewer for a small
+ * call so that every thread gets at least four claims).
  */
 int orc_run_grid(const orc_net *net, uint64_t seed, uint64_t traj_begin, uint64_t n_traj,
                  const int64_t *x0, int x0_per_traj, double tmax, uint64_t nb_steps,
                  int64_t *out, int64_t *final_x, double *times, uint64_t *events,
                  uint64_t *draws, int rng_kind, int nthreads);
+
+/* One network's grid run inside orc_run_grid_jobs (fields as orc_run_grid). */
+typedef struct {
+  const orc_net *net;
+  uint64_t seed, traj_begin, n_traj;
+  const int64_t *x0;
+  int x0_per_traj;
+  double tmax;
+  uint64_t nb_steps;
+  int64_t *out;     /* nullable */
+  int64_t *final_x; /* nullable */
+} orc_grid_job;
+
+/* Many grid runs (one per network) in one thread pool: the threads claim
+ * chunks of trajectories of any job from one atomic counter, so a list of
+ * small jobs keeps every thread busy.  *threads_busy (nullable) receives the
+ * number of threads that ran at least one trajectory. */
+int orc_run_grid_jobs(const orc_grid_job *jobs, uint32_t n_jobs, int rng_kind, int nthreads,
+                      uint64_t *events, uint64_t *draws, int *threads_busy);
 
 /*
  * The pyo3 nb_steps == 0 loop for one trajectory: records (t, x) before the
@@ -137,10 +157,10 @@ int orc_run_all_events(const orc_net *net, uint64_t seed, uint64_t traj, const i
  * over nthreads threads (a timing harness: each thread writes the records
  * into a buffer of its own and drops them); *events and *records receive the
  * totals (events exclude the initial record and an absorbing state's
- * t = +inf record). */
+ * t = +inf record); *threads_busy (nullable): threads that ran a trajectory. */
 int orc_run_all_events_many(const orc_net *net, uint64_t seed, uint64_t traj_begin, uint64_t n_traj,
                             const int64_t *x0, double tmax, uint64_t *events, uint64_t *records,
-                            int rng_kind, int nthreads);
+                            int rng_kind, int nthreads, int *threads_busy);
 
 /* RNG primitives, exported so tests can pin them. */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

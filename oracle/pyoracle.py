@@ -74,7 +74,11 @@ def load() -> ctypes.CDLL:
         "orc_run_all_events_many": (
             c_int,
             [c_void_p, c_uint64, c_uint64, c_uint64, P(c_int64), c_double, P(c_uint64), P(c_uint64), c_int,
-             c_int],
+             c_int, P(c_int)],
+        ),
+        "orc_run_grid_jobs": (
+            c_int,
+            [c_void_p, c_uint32, c_int, c_int, P(c_uint64), P(c_uint64), P(c_int)],
         ),
         "orc_philox4x32_10": (None, [P(c_uint32), P(c_uint32), P(c_uint32)]),
         "orc_ln64": (c_double, [c_double]),
@@ -231,21 +235,68 @@ class OracleNet:
                 return times[: n.value], xs[: n.value]
             cap = int(n.value)
 
-    def run_all_events_many(self, seed, traj_begin, n_traj, x0, tmax, rng=PHILOX, nthreads=1):
+    def run_all_events_many(self, seed, traj_begin, n_traj, x0, tmax, rng=PHILOX, nthreads=1,
+                            with_busy=False):
         """(events, records) of run_all_events over n_traj trajectories on
-        nthreads threads (the records are written and dropped)."""
+        nthreads threads (the records are written and dropped); with_busy adds
+        the number of threads that ran a trajectory."""
         x0a = np.ascontiguousarray(np.asarray(x0, dtype=np.int64).reshape(-1))
         ev = c_uint64(0)
         rec = c_uint64(0)
+        busy = c_int(0)
         rc = self.lib.orc_run_all_events_many(self.h, int(seed) & (2**64 - 1), int(traj_begin), int(n_traj),
                                               _p(x0a, c_int64), float(tmax), ctypes.byref(ev), ctypes.byref(rec),
-                                              int(rng), int(nthreads))
+                                              int(rng), int(nthreads), ctypes.byref(busy))
         if rc:
             raise OracleError(rc)
+        if with_busy:
+            return int(ev.value), int(rec.value), int(busy.value)
         return int(ev.value), int(rec.value)
 
     def state(self, x0, seed, traj=0, rng=PHILOX) -> "OracleTrajectory":
         return OracleTrajectory(self, x0, seed, traj, rng)
+
+
+class OrcGridJob(ctypes.Structure):
+    _fields_ = [
+        ("net", c_void_p),
+        ("seed", c_uint64),
+        ("traj_begin", c_uint64),
+        ("n_traj", c_uint64),
+        ("x0", POINTER(c_int64)),
+        ("x0_per_traj", c_int),
+        ("tmax", c_double),
+        ("nb_steps", c_uint64),
+        ("out", POINTER(c_int64)),
+        ("final_x", POINTER(c_int64)),
+    ]
+
+
+def run_grid_jobs(jobs, rng=PHILOX, nthreads=1):
+    """Grid runs of several networks in one thread pool (no sample output).
+
+    jobs: [(OracleNet, seed, x0, tmax, nb_steps, n_traj, traj_begin)].
+    Returns (events, draws, threads that ran at least one trajectory)."""
+    lib = load()
+    arr = (OrcGridJob * max(len(jobs), 1))()
+    keep = []
+    for q, (net, seed, x0, tmax, nb, n, tb) in enumerate(jobs):
+        x0a = np.ascontiguousarray(np.asarray(x0, dtype=np.int64).reshape(-1))
+        keep.append(x0a)
+        arr[q].net = net.h
+        arr[q].seed = int(seed) & (2**64 - 1)
+        arr[q].traj_begin = int(tb)
+        arr[q].n_traj = int(n)
+        arr[q].x0 = _p(x0a, c_int64)
+        arr[q].x0_per_traj = 0
+        arr[q].tmax = float(tmax)
+        arr[q].nb_steps = int(nb)
+    ev, dr, busy = c_uint64(0), c_uint64(0), c_int(0)
+    rc = lib.orc_run_grid_jobs(ctypes.cast(arr, c_void_p), len(jobs), int(rng), int(nthreads),
+                               ctypes.byref(ev), ctypes.byref(dr), ctypes.byref(busy))
+    if rc:
+        raise OracleError(rc)
+    return int(ev.value), int(dr.value), int(busy.value)
 
 
 class OracleTrajectory:

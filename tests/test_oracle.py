@@ -275,6 +275,30 @@ def test_all_events_many_sums_the_single_trajectory_loop(oracle):
     assert len(t) > 4096  # grows the harness's buffer
     e1, r1 = big.run_all_events_many(1, 0, 1, dimers()[2], 0.5, nthreads=2)
     assert r1 == len(t) and e1 == len(t) - 1 - (1 if np.isinf(t[-1]) else 0)
+    # a small call still spreads over the threads (claims of one trajectory)
+    _, _, busy = n.run_all_events_many(5, 7, 64, [999, 1, 0], 250.0, nthreads=4, with_busy=True)
+    assert 2 <= busy <= 4
+
+
+def test_grid_jobs_pool_equals_separate_runs(oracle):
+    """The multi-network pool (the CPU baseline's runner) gives each job the
+    events and draws of its own grid run, for any thread count, and a sample
+    of a few trajectories per job still occupies several threads."""
+    from tests.networks import synthetic
+
+    specs = [sir(), dimers(), synthetic(12, 30, 4, init_mean=20.0)]
+    runs = [(build_oracle(sp), 11 + q, sp[2], t, nb, n, 3 * q)
+            for q, (sp, t, nb, n) in enumerate(zip(specs, (250.0, 0.2, 2.0), (50, 20, 10), (40, 7, 33)))]
+    ev, dr = 0, 0
+    for net, seed, x0, t, nb, n, tb in runs:
+        _, _, _, e, d = net.run_grid(seed, x0, t, nb, n_traj=n, traj_begin=tb)
+        ev, dr = ev + e, dr + d
+    for th in (1, 2, 5):
+        e, d, busy = oracle.run_grid_jobs(runs, nthreads=th)
+        assert (e, d) == (ev, dr)
+        assert 1 <= busy <= th
+    _, _, busy = oracle.run_grid_jobs(runs, nthreads=4)
+    assert busy >= 2
 
 
 # ---- exact laws ----------------------------------------------------------

@@ -524,13 +524,23 @@ def surrogate_main(args) -> int:
             "device": dev_name,
             "compute_units": cus,
             "ssa_queries_per_s": res["ssa_qps"],
-            "speedup_vs_gpu_ssa": res["speedup"],
+            # at the W1 <= 2 % gate: surrogate where the trust map admits a
+            # query, the SSA for the rest
+            "speedup_vs_gpu_ssa": res["effective_speedup"],
+            # every query answered by the surrogate, gate ignored
+            "raw_speedup_all_queries": res["speedup"],
             "w1_pass_fraction": res["w1_pass_fraction"],
             "point_pass_fraction": res["point_pass_fraction"],
             "w1_pass_fraction_reproducible": res["w1_pass_fraction_reproducible"],
             "ssa_self_w1_over_2pct": res["ssa_self_w1_over_2pct"],
             "trusted_cells": res["trusted_cells"],
             "trusted_validated_cells": res["trusted_validated_cells"],
+            "trusted_query_fraction": res["trusted_query_fraction"],
+            "effective_speedup_gated_sweep": res["effective_speedup"],
+            "holdout_admitted_entries": res["holdout_admitted_entries"],
+            "holdout_admitted_queries": res["holdout_admitted_queries"],
+            "holdout_admitted_pass_fraction": res["holdout_admitted_pass_fraction"],
+            "ssa_point_ms": res["ssa_point_ms"],
             "rel_w1_median": res["rel_w1_median"],
             "train_s": res["train_s"],
             "nll_first": res["nll_first"],

@@ -19,6 +19,11 @@
  * REBOP_ERR_* codes; rebop_last_error() then describes the failure (the
  * message is thread-local and valid until the next failing call on that
  * thread).  Where the reference panics (`assert!`), the ABI returns an error.
+ *
+ * Streams: a handle keeps per-handle device state (launch counters, the
+ * sample-time table, tail hand-off buffers), so the runs of one handle must
+ * be issued on one stream at a time; distinct handles may run concurrently
+ * on different streams.
  */
 #ifndef REBOP_GPU_H
 #define REBOP_GPU_H
@@ -30,7 +35,7 @@
 extern "C" {
 #endif
 
-#define REBOP_GPU_ABI_VERSION 1
+#define REBOP_GPU_ABI_VERSION 2
 
 enum rebop_status {
   REBOP_OK = 0,
@@ -40,7 +45,8 @@ enum rebop_status {
   REBOP_ERR_DEVICE = 4, /* HIP runtime failure, or no usable gfx950 device */
   REBOP_ERR_ALLOC = 5,  /* host or device allocation failed */
   REBOP_ERR_RANGE = 6,  /* a count left the exactly simulated range during a run */
-  REBOP_ERR_UNSUPPORTED = 7 /* network shape beyond the compiled kernels */
+  REBOP_ERR_UNSUPPORTED = 7, /* network shape beyond the compiled kernels */
+  REBOP_ERR_CAP = 8          /* a trajectory reached the draw cap (rebop_gillespie_set_draw_cap) */
 };
 
 /* Counters of one run, summed over its trajectories. */
@@ -51,14 +57,16 @@ typedef struct rebop_stats {
   uint64_t kernel;       /* which kernel ran: REBOP_KERNEL_* */
   double kernel_ms;      /* device time of the SSA kernel(s) (0 if not timed) */
   uint64_t tail;         /* trajectories finished by the grouped tail launch */
+  uint64_t capped;       /* trajectories stopped by the draw cap (then REBOP_ERR_CAP) */
 } rebop_stats;
 
 enum rebop_kernel_kind {
   REBOP_KERNEL_AUTO = 0,
   REBOP_KERNEL_PLAIN = 1,      /* one trajectory per lane, lanes retire when done */
   REBOP_KERNEL_PERSISTENT = 2, /* lanes refill from an atomic trajectory queue */
-  REBOP_KERNEL_LDS = 3,        /* counts in LDS (int32): 256 bytes per species and 16 per
-                                  reaction must fit one workgroup's LDS (160 KB) */
+  REBOP_KERNEL_LDS = 3,        /* counts in LDS: 256 bytes per species (int32; 512 with
+                                  REBOP_KERNEL_WIDE) and 16 per reaction must fit one
+                                  workgroup's LDS (160 KB) */
   /* flag: the network compiled into the kernel (run-time compilation,
    * cached); combine with PLAIN or PERSISTENT */
   REBOP_KERNEL_SPECIALIZED = 16,
@@ -72,7 +80,14 @@ enum rebop_kernel_kind {
    * empty, waves with at most 16 lanes still running hand those trajectories
    * (state written to device memory) to a second launch that runs them with
    * 4 lanes each, which shortens the end of the run; results are identical */
-  REBOP_KERNEL_WHOLE = 128
+  REBOP_KERNEL_WHOLE = 128,
+  /* flag for the LDS kernel: counts as f64 (exact integers, |x| <= 2^52, the
+   * register kernels' range) instead of int32 (|x| <= 2^30; half the LDS).
+   * Without it an LDS run starts narrow unless the handle's initial counts
+   * are beyond int32, and a run whose counters are read (stats non-NULL, or
+   * events_per_job) that meets a count beyond int32 is rerun wide; resumable
+   * batches always run wide. */
+  REBOP_KERNEL_WIDE = 256
 };
 
 /* Opaque network + initial state: rebop's `gillespie::Gillespie` (src/gillespie.rs:157-163). */
@@ -92,6 +107,20 @@ void rebop_gillespie_free(rebop_gillespie *g);
 
 /* Gillespie::seed(seed) -- src/gillespie.rs:189-191. */
 int rebop_gillespie_seed(rebop_gillespie *g, uint64_t seed);
+
+/* Watchdog, no counterpart in the reference (whose advance_until loop,
+ * src/gillespie.rs:317-343, runs until tmax however many events that takes):
+ * every later run of this handle stops a trajectory once it has made
+ * max_draws random draws (events plus discarded overshoot draws) in that
+ * call, treats its state as absorbing from there (no more draws; later
+ * samples repeat its counts), and returns REBOP_ERR_CAP, with
+ * rebop_stats.capped the number of such trajectories.  0 (the default)
+ * means no cap, except for horizons that are not finite (tmax = +inf or
+ * NaN, where only an absorbing state ends the reference's loop): those runs
+ * use a cap of 2^20 draws unless the handle sets one.  The cap is checked
+ * after each event, so a trajectory whose propensity keeps growing cannot
+ * hold its wave, and the launch, indefinitely. */
+int rebop_gillespie_set_draw_cap(rebop_gillespie *g, uint64_t max_draws);
 
 /* Gillespie::nb_species / nb_reactions -- src/gillespie.rs:199-211. */
 uint32_t rebop_gillespie_nb_species(const rebop_gillespie *g);
@@ -172,10 +201,11 @@ int rebop_system_add_reaction(rebop_gillespie *g, double rate, const uint32_t *r
                               const uint32_t *products, const uint32_t *product_counts,
                               uint32_t n_products);
 
-/* Compiles and caches the kernel `kernel` (REBOP_KERNEL_*) will use for this
- * network, so that the first run does not pay for it.  Without a device it
- * only compiles, which checks the specialised kernel for gfx950.  No
- * counterpart in the reference (which has no device code to prepare). */
+/* Uploads the network to the current device and compiles (cached) the
+ * kernel `kernel` (REBOP_KERNEL_*; AUTO: the one an AUTO run would use) for
+ * it, so that the first run does not pay for them.  Without a device it only
+ * compiles, which checks the specialised kernel for gfx950.  No counterpart
+ * in the reference (which has no device code to prepare). */
 int rebop_gillespie_prepare(const rebop_gillespie *g, int kernel);
 
 /* The autotuner's choice: which kernel and launch shape AUTO runs. */
@@ -240,7 +270,9 @@ int rebop_run_device(const rebop_gillespie *g, double tmax, uint64_t nb_steps,
  * states NULL it fills n_records[j], trajectory j's record count (initial
  * state included); then, given those counts, it writes trajectory j's records
  * at row offset sum_{i<j} n_records[i] of times[] and states[][nb_species].
- * tmax must not be +inf.  Host buffers.
+ * tmax may be +inf (every event until an absorbing state, as the reference's
+ * loop does; a trajectory that does not reach one within the draw cap makes
+ * the call return REBOP_ERR_CAP).  Host buffers.
  */
 int rebop_run_all_events(const rebop_gillespie *g, double tmax, uint64_t traj_begin,
                          uint64_t n_traj, uint64_t *n_records, double *times, int64_t *states,
@@ -281,9 +313,11 @@ typedef struct rebop_job {
  * src/pyo3_gillespie.rs:197-208 loop for its own network -- fused into one
  * launch of the LDS kernel: waves take (job, 64 trajectories) items from one
  * queue.  Results are identical to running the jobs one by one.  Enqueued on
- * `stream`; the call waits for completion.  events_per_job (nullable host
- * array [n_jobs]) receives each job's applied-reaction count.  No counterpart
- * in the reference beyond calling `Gillespie.run` once per network.
+ * `stream`; the call waits for completion only when stats or events_per_job
+ * (nullable host array [n_jobs]: each job's applied-reaction count) is
+ * given.  Its device tables are reused by later calls on the same stream.
+ * No counterpart in the reference beyond calling `Gillespie.run` once per
+ * network.
  */
 int rebop_run_jobs_device(const rebop_job *jobs, uint32_t n_jobs, void *stream,
                           rebop_stats *stats, uint64_t *events_per_job);

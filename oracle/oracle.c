@@ -428,8 +428,14 @@ void orc_state_init(orc_state *st, int64_t *x, uint64_t seed, uint64_t traj, int
   st->draw = 0;
   st->rng_kind = rng_kind;
   st->events = 0;
+  st->draw_lim = UINT64_MAX;
+  st->capped = 0;
   uint64_t sm = seed ^ (traj * 0xd1342543de82ef95ull) ^ 0x5851f42d4c957f2dull;
   for (int i = 0; i < 4; ++i) st->xs[i] = splitmix64(&sm);
+}
+
+void orc_state_set_cap(orc_state *st, uint64_t cap) {
+  st->draw_lim = (cap == 0 || st->draw + cap < st->draw) ? UINT64_MAX : st->draw + cap;
 }
 
 /* one draw: the exponential, and (for Philox) the uniform of the same block */
@@ -453,7 +459,7 @@ static inline double draw_exp1(orc_state *st, double *u_out) {
 void orc_advance_until(const orc_net *net, orc_state *st, double tmax, double *cum) {
   for (;;) {
     const double total = orc_make_cumrates(net, st->x, cum);
-    if (!(0.0 < total)) { /* :323-326 absorbing: no draw */
+    if (!(0.0 < total) || st->capped) { /* :323-326 absorbing: no draw */
       st->t = tmax;
       return;
     }
@@ -468,6 +474,7 @@ void orc_advance_until(const orc_net *net, orc_state *st, double tmax, double *c
     const uint64_t k = choose(net, chosen, cum);
     orc_affect(net, (uint32_t)k, st->x); /* :342 */
     st->events++;
+    if (st->draw >= st->draw_lim) st->capped = 1; /* the device's watchdog */
   }
 }
 
@@ -485,6 +492,7 @@ void orc_advance_one_reaction(const orc_net *net, orc_state *st, double *cum) {
   const uint64_t k = choose(net, chosen, cum);
   orc_affect(net, (uint32_t)k, st->x);
   st->events++;
+  if (st->draw >= st->draw_lim) st->capped = 1; /* the device's watchdog */
 }
 
 /* pyo3_gillespie.rs:202 */
@@ -528,6 +536,7 @@ static void grid_one(const orc_grid_job *J, uint64_t j, int rng_kind, int64_t *x
   memcpy(x, src, sizeof(int64_t) * S);
   orc_state st;
   orc_state_init(&st, x, J->seed, J->traj_begin + j, rng_kind);
+  orc_state_set_cap(&st, J->draw_cap);
   for (uint64_t i = 0; i <= J->nb_steps; ++i) {
     const double t = orc_grid_time(J->tmax, i, J->nb_steps);
     orc_advance_until(J->net, &st, t, cum);
@@ -628,7 +637,7 @@ int orc_run_grid_jobs(const orc_grid_job *jobs, uint32_t n_jobs, int rng_kind, i
 int orc_run_grid(const orc_net *net, uint64_t seed, uint64_t traj_begin, uint64_t n_traj,
                  const int64_t *x0, int x0_per_traj, double tmax, uint64_t nb_steps,
                  int64_t *out, int64_t *final_x, double *times, uint64_t *events,
-                 uint64_t *draws, int rng_kind, int nthreads) {
+                 uint64_t *draws, int rng_kind, int nthreads, uint64_t draw_cap) {
   if (nb_steps == 0) return ORC_ERR_VALUE;
   if (times)
     for (uint64_t i = 0; i <= nb_steps; ++i) times[i] = orc_grid_time(tmax, i, nb_steps);
@@ -643,13 +652,14 @@ int orc_run_grid(const orc_net *net, uint64_t seed, uint64_t traj_begin, uint64_
   J.nb_steps = nb_steps;
   J.out = out;
   J.final_x = final_x;
+  J.draw_cap = draw_cap;
   return orc_run_grid_jobs(&J, 1, rng_kind, nthreads, events, draws, NULL);
 }
 
 /* pyo3_gillespie.rs:209-223 for one trajectory */
 int orc_run_all_events(const orc_net *net, uint64_t seed, uint64_t traj, const int64_t *x0,
                        double tmax, uint64_t cap, double *times, int64_t *xs,
-                       uint64_t *n_out, int rng_kind) {
+                       uint64_t *n_out, int rng_kind, uint64_t draw_cap) {
   const uint32_t S = net->S;
   int64_t *x = (int64_t *)malloc(sizeof(int64_t) * (S ? S : 1));
   double *cum = (double *)malloc(sizeof(double) * (net->R ? net->R : 1));
@@ -661,6 +671,7 @@ int orc_run_all_events(const orc_net *net, uint64_t seed, uint64_t traj, const i
   memcpy(x, x0, sizeof(int64_t) * S);
   orc_state st;
   orc_state_init(&st, x, seed, traj, rng_kind);
+  orc_state_set_cap(&st, draw_cap);
   uint64_t n = 0;
 #define REC()                                                        \
   do {                                                               \
@@ -671,7 +682,7 @@ int orc_run_all_events(const orc_net *net, uint64_t seed, uint64_t traj, const i
     n++;                                                             \
   } while (0)
   REC();
-  while (st.t < tmax) {
+  while (st.t < tmax && !st.capped) {
     orc_advance_one_reaction(net, &st, cum);
     REC();
   }
@@ -722,7 +733,7 @@ static void *events_worker(void *arg) {
     for (uint64_t j = b; j < e; ++j) {
       uint64_t n = 0;
       int rc = orc_run_all_events(J->net, J->seed, J->traj_begin + j, J->x0, J->tmax, cap, times, xs,
-                                  &n, J->rng_kind);
+                                  &n, J->rng_kind, 0);
       if (rc == ORC_OK && n > cap) { /* longer than the buffer: grow it, run again */
         free(times);
         free(xs);
@@ -731,7 +742,7 @@ static void *events_worker(void *arg) {
         xs = (int64_t *)malloc(sizeof(int64_t) * cap * (S ? S : 1));
         if (!times || !xs) rc = ORC_ERR_ALLOC;
         else rc = orc_run_all_events(J->net, J->seed, J->traj_begin + j, J->x0, J->tmax, cap, times, xs,
-                                     &n, J->rng_kind);
+                                     &n, J->rng_kind, 0);
       }
       if (rc != ORC_OK) {
         atomic_store(&J->err, rc);

@@ -102,9 +102,15 @@ typedef struct {
   uint64_t xs[4];  /* xoshiro256++ state */
   int rng_kind;
   uint64_t events; /* applied reactions */
+  /* The device's watchdog, which the reference does not have: once an event
+   * leaves draw >= draw_lim the state is treated as absorbing (capped = 1). */
+  uint64_t draw_lim;
+  int capped;
 } orc_state;
 
 void orc_state_init(orc_state *st, int64_t *x, uint64_t seed, uint64_t traj, int rng_kind);
+/* Caps the draws from here on at `cap` (0: no cap). */
+void orc_state_set_cap(orc_state *st, uint64_t cap);
 void orc_advance_until(const orc_net *net, orc_state *st, double tmax, double *scratch_cum);
 void orc_advance_one_reaction(const orc_net *net, orc_state *st, double *scratch_cum);
 
@@ -122,7 +128,7 @@ void orc_advance_one_reaction(const orc_net *net, orc_state *st, double *scratch
 int orc_run_grid(const orc_net *net, uint64_t seed, uint64_t traj_begin, uint64_t n_traj,
                  const int64_t *x0, int x0_per_traj, double tmax, uint64_t nb_steps,
                  int64_t *out, int64_t *final_x, double *times, uint64_t *events,
-                 uint64_t *draws, int rng_kind, int nthreads);
+                 uint64_t *draws, int rng_kind, int nthreads, uint64_t draw_cap);
 
 /* One network's grid run inside orc_run_grid_jobs (fields as orc_run_grid). */
 typedef struct {
@@ -134,6 +140,7 @@ typedef struct {
   uint64_t nb_steps;
   int64_t *out;     /* nullable */
   int64_t *final_x; /* nullable */
+  uint64_t draw_cap; /* the device's watchdog (0: none), see orc_state */
 } orc_grid_job;
 
 /* Many grid runs (one per network) in one thread pool: the threads claim
@@ -148,10 +155,12 @@ int orc_run_grid_jobs(const orc_grid_job *jobs, uint32_t n_jobs, int rng_kind, i
  * first event and after each event while t < tmax (an absorbing state sets
  * t = +inf).  Writes at most cap records; *n_out receives the number of
  * records the run produced (which may exceed cap: then the output is cut).
+ * draw_cap (0: none) stops the loop after the event that reaches it, as the
+ * device's watchdog does.
  */
 int orc_run_all_events(const orc_net *net, uint64_t seed, uint64_t traj, const int64_t *x0,
                        double tmax, uint64_t cap, double *times, int64_t *xs,
-                       uint64_t *n_out, int rng_kind);
+                       uint64_t *n_out, int rng_kind, uint64_t draw_cap);
 
 /* orc_run_all_events for trajectories traj_begin .. traj_begin + n_traj - 1
  * over nthreads threads (a timing harness: each thread writes the records

@@ -64,12 +64,12 @@ def load() -> ctypes.CDLL:
         "orc_run_grid": (
             c_int,
             [c_void_p, c_uint64, c_uint64, c_uint64, P(c_int64), c_int, c_double, c_uint64,
-             P(c_int64), P(c_int64), P(c_double), P(c_uint64), P(c_uint64), c_int, c_int],
+             P(c_int64), P(c_int64), P(c_double), P(c_uint64), P(c_uint64), c_int, c_int, c_uint64],
         ),
         "orc_run_all_events": (
             c_int,
             [c_void_p, c_uint64, c_uint64, P(c_int64), c_double, c_uint64, P(c_double),
-             P(c_int64), P(c_uint64), c_int],
+             P(c_int64), P(c_uint64), c_int, c_uint64],
         ),
         "orc_run_all_events_many": (
             c_int,
@@ -91,6 +91,7 @@ def load() -> ctypes.CDLL:
         "orc_grid_time": (c_double, [c_double, c_uint64, c_uint64]),
         "orc_hw_threads": (c_int, []),
         "orc_state_init": (None, [c_void_p, P(c_int64), c_uint64, c_uint64, c_int]),
+        "orc_state_set_cap": (None, [c_void_p, c_uint64]),
         "orc_advance_until": (None, [c_void_p, c_void_p, c_double, P(c_double)]),
         "orc_advance_one_reaction": (None, [c_void_p, c_void_p, P(c_double)]),
     }
@@ -116,6 +117,8 @@ class OrcState(ctypes.Structure):
         ("xs", c_uint64 * 4),
         ("rng_kind", c_int),
         ("events", c_uint64),
+        ("draw_lim", c_uint64),
+        ("capped", c_int),
     ]
 
 
@@ -202,7 +205,9 @@ class OracleNet:
         return xa
 
     def run_grid(self, seed, x0, tmax, nb_steps, n_traj=1, traj_begin=0, rng=PHILOX,
-                 nthreads=1, x0_per_traj=False, want_out=True):
+                 nthreads=1, x0_per_traj=False, want_out=True, draw_cap=0):
+        """The grid loop over trajectories traj_begin ..; draw_cap (0: none) is
+        the device's watchdog, which the reference does not have."""
         x0a = np.ascontiguousarray(np.asarray(x0, dtype=np.int64).reshape(-1))
         out = np.zeros((nb_steps + 1, self.S, n_traj), dtype=np.int64) if want_out else None
         final = np.zeros((n_traj, self.S), dtype=np.int64)
@@ -213,13 +218,13 @@ class OracleNet:
             self.h, int(seed) & (2**64 - 1), int(traj_begin), int(n_traj), _p(x0a, c_int64),
             int(bool(x0_per_traj)), float(tmax), int(nb_steps),
             _p(out, c_int64) if want_out else None, _p(final, c_int64), _p(times, c_double),
-            ctypes.byref(ev), ctypes.byref(dr), int(rng), int(nthreads),
+            ctypes.byref(ev), ctypes.byref(dr), int(rng), int(nthreads), int(draw_cap),
         )
         if rc:
             raise OracleError(rc)
         return times, out, final, int(ev.value), int(dr.value)
 
-    def run_all_events(self, seed, traj, x0, tmax, cap=1 << 16, rng=PHILOX):
+    def run_all_events(self, seed, traj, x0, tmax, cap=1 << 16, rng=PHILOX, draw_cap=0):
         x0a = np.ascontiguousarray(np.asarray(x0, dtype=np.int64).reshape(-1))
         while True:
             times = np.zeros(cap, dtype=np.float64)
@@ -227,7 +232,7 @@ class OracleNet:
             n = c_uint64(0)
             rc = self.lib.orc_run_all_events(
                 self.h, int(seed) & (2**64 - 1), int(traj), _p(x0a, c_int64), float(tmax), int(cap),
-                _p(times, c_double), _p(xs, c_int64), ctypes.byref(n), int(rng),
+                _p(times, c_double), _p(xs, c_int64), ctypes.byref(n), int(rng), int(draw_cap),
             )
             if rc:
                 raise OracleError(rc)
@@ -269,6 +274,7 @@ class OrcGridJob(ctypes.Structure):
         ("nb_steps", c_uint64),
         ("out", POINTER(c_int64)),
         ("final_x", POINTER(c_int64)),
+        ("draw_cap", c_uint64),
     ]
 
 
@@ -322,6 +328,14 @@ class OracleTrajectory:
     @property
     def draws(self) -> int:
         return int(self.st.draw)
+
+    def set_cap(self, cap: int) -> None:
+        """The device's watchdog from here on: at most `cap` more draws (0: none)."""
+        self.net.lib.orc_state_set_cap(ctypes.byref(self.st), int(cap))
+
+    @property
+    def capped(self) -> bool:
+        return bool(self.st.capped)
 
 
 def philox(ctr, key):

@@ -22,6 +22,7 @@ REBOP_ERR_DEVICE = 4
 REBOP_ERR_ALLOC = 5
 REBOP_ERR_RANGE = 6
 REBOP_ERR_UNSUPPORTED = 7
+REBOP_ERR_CAP = 8
 
 REBOP_KERNEL_AUTO = 0
 REBOP_KERNEL_PLAIN = 1
@@ -31,6 +32,7 @@ REBOP_KERNEL_SPECIALIZED = 16
 REBOP_KERNEL_GROUP2 = 32
 REBOP_KERNEL_GROUP4 = 64
 REBOP_KERNEL_WHOLE = 128
+REBOP_KERNEL_WIDE = 256
 
 # rate-expression bytecode operations (token = op | arg << 8)
 REBOP_EXPR_CONST = 1
@@ -56,6 +58,7 @@ SIGNATURES: dict[str, tuple] = {
     ),
     "rebop_gillespie_free": (None, [c_void_p]),
     "rebop_gillespie_seed": (c_int, [c_void_p, c_uint64]),
+    "rebop_gillespie_set_draw_cap": (c_int, [c_void_p, c_uint64]),
     "rebop_gillespie_nb_species": (c_uint32, [c_void_p]),
     "rebop_gillespie_nb_reactions": (c_uint32, [c_void_p]),
     "rebop_gillespie_add_reaction_lma": (
@@ -128,6 +131,7 @@ class Stats(ctypes.Structure):
         ("kernel", c_uint64),
         ("kernel_ms", c_double),
         ("tail", c_uint64),
+        ("capped", c_uint64),
     ]
 
 
@@ -171,6 +175,10 @@ class RebopValueError(RebopError, ValueError):
     """Invalid shapes, indices or values (the reference panics or raises ValueError)."""
 
 
+class RebopCapError(RebopError):
+    """Some trajectory reached the draw cap (the watchdog); their results stop there."""
+
+
 _lib = None
 
 
@@ -209,6 +217,8 @@ def check(code: int) -> None:
     msg = load().rebop_last_error().decode(errors="replace")
     if code in (REBOP_ERR_SHAPE, REBOP_ERR_INDEX, REBOP_ERR_VALUE, REBOP_ERR_RANGE):
         raise RebopValueError(code, msg)
+    if code == REBOP_ERR_CAP:
+        raise RebopCapError(code, msg)
     raise RebopError(code, msg)
 
 

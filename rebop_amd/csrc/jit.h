@@ -36,8 +36,9 @@ struct JitKernel {
 
 // Kernel source for a network of S species and the given reactions; with
 // `system`, the define_system arithmetic (integer mass-action products over
-// `terms`, first-match choice).
-std::string jit_source(uint32_t S, const std::vector<JitReaction> &rx, bool system = false);
+// `terms`, first-match choice); with `cap`, the loops check the draw cap.
+std::string jit_source(uint32_t S, const std::vector<JitReaction> &rx, bool system = false,
+                       bool cap = false);
 // Compiles (or finds in the cache) and loads the kernel on the current device.
 int jit_get(const std::string &src, JitKernel *out, std::string *err);
 // Compiles for gfx950 without loading (no device needed).

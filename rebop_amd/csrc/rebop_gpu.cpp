@@ -10,6 +10,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
@@ -20,6 +22,7 @@
 
 #include "jit.h"
 #include "rebop_expr.h"
+#include "scratch.h"
 #include "ssa_layout.h"
 
 namespace rebop {
@@ -28,16 +31,18 @@ int reg_shape_ms(int c);
 int reg_shape_mr(int c);
 hipError_t reg_occupancy(int c, int threads, size_t lds, int *blocks_per_cu);
 hipError_t launch_reset_counters(unsigned long long *counters, hipStream_t stream);
+hipError_t launch_zero_u64(unsigned long long *p, uint64_t n, hipStream_t stream);
 hipError_t launch_grid(double *grid, const rebop_dev_run &run, hipStream_t stream);
 hipError_t launch_raw_counts(int64_t *out, uint64_t n, int cus, hipStream_t stream);
 hipError_t launch_reg(int c, const rebop_dev_net &net, const rebop_dev_run &run, int blocks,
                       int threads, size_t lds, hipStream_t stream);
 
 int lds_checkpoints();
-hipError_t lds_occupancy(int threads, size_t lds, int *blocks_per_cu);
+hipError_t lds_occupancy(int threads, size_t lds, bool wide, int *blocks_per_cu);
 hipError_t launch_lds(const rebop_dev_job *jobs, const rebop_dev_item *items, uint32_t n_items,
-                      unsigned long long *counters, unsigned long long *job_events, int blocks,
-                      int threads, size_t lds, uint32_t rec_word_off, hipStream_t stream);
+                      unsigned long long *counters, unsigned long long *job_events,
+                      unsigned long long *job_queue, int blocks, int threads, size_t lds,
+                      uint32_t rec_word_off, bool wide, hipStream_t stream);
 
 } // namespace rebop
 
@@ -135,12 +140,48 @@ struct DevBuf {
   }
 };
 
+// The two timing events of one launch sequence, destroyed on every return path.
+struct EventPair {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  EventPair() = default;
+  EventPair(const EventPair &) = delete;
+  EventPair &operator=(const EventPair &) = delete;
+  ~EventPair() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+  hipError_t create() {
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    return e;
+  }
+};
+
+// Device tables of rebop_run_jobs_device, one set per (device, stream).  The
+// map is never destroyed: its buffers live until the process ends (freeing
+// them from a static destructor could run after the HIP runtime is gone).
+struct JobBuffers {
+  DevBuf counters, djobs, ditems, dev_ev, dev_q;
+};
+std::mutex g_job_mu;
+std::map<std::pair<int, hipStream_t>, JobBuffers> *g_job_bufs =
+    new std::map<std::pair<int, hipStream_t>, JobBuffers>();
+
+// finish_stats: a narrow LDS launch saw a count beyond int32 (rerun it wide)
+constexpr int kRetryWide = -1;
+
+// Draws per trajectory after which a run whose horizon is not finite (tmax
+// = +inf or NaN) stops a trajectory that never reaches an absorbing state,
+// when the handle sets no cap of its own.
+constexpr uint64_t kInfiniteHorizonCap = 1ull << 20;
+
 } // namespace
 
 struct rebop_gillespie {
   uint32_t S = 0;
   bool sparse = false;
   uint64_t seed = 0;
+  uint64_t draw_cap = 0; // watchdog: draws per trajectory and call (0 = none)
   std::vector<int64_t> x0;
   std::vector<Reaction> rx;
   uint32_t n_expr = 0; // reactions with a rate expression (specialised kernels only)
@@ -160,11 +201,11 @@ struct rebop_gillespie {
   // the network part of an LDS-kernel job (sparse jump, checkpoint blocks)
   rebop_dev_job job{};
   DevBuf djobs, ditems;
-  // network-specialised kernel (run-time compiled), rebuilt when reactions change
-  bool jit_dirty = true;
-  std::string jit_src;
-  int jit_dev = -1;
-  rebop::JitKernel jit{};
+  // network-specialised kernels (run-time compiled), rebuilt when reactions
+  // change: [0] plain, [1] with the draw-cap checks (compiled when first needed)
+  bool jit_dirty[2] = {true, true};
+  int jit_dev[2] = {-1, -1};
+  rebop::JitKernel jitk[2]{};
   std::string jit_err;
   // written-out state of the trajectories a persistent launch hands to its tail
   DevBuf spill;
@@ -347,6 +388,14 @@ int upload_net(rebop_gillespie *g) {
   return REBOP_OK;
 }
 
+// The draw cap of a run of this handle up to tmax: the handle's own, else
+// one for horizons that are not finite (where the reference's loop would only
+// end in an absorbing state), else none.
+uint64_t effective_cap(const rebop_gillespie *g, double tmax) {
+  if (g->draw_cap) return g->draw_cap;
+  return std::isfinite(tmax) ? 0ull : kInfiniteHorizonCap;
+}
+
 int pick_reg_shape(uint32_t S, uint32_t R) {
   for (int c = 0; c < rebop::reg_shape_count(); ++c)
     if ((int)S <= rebop::reg_shape_ms(c) && (int)R <= rebop::reg_shape_mr(c)) return c;
@@ -388,7 +437,7 @@ bool jit_disabled() {
   return v && *v && std::strcmp(v, "0") != 0;
 }
 
-std::string jit_source_of(const rebop_gillespie *g) {
+std::string jit_source_of(const rebop_gillespie *g, bool cap = false) {
   std::vector<rebop::JitReaction> jr;
   jr.reserve(g->rx.size());
   for (auto &r : g->rx) {
@@ -404,7 +453,7 @@ std::string jit_source_of(const rebop_gillespie *g) {
     j.consts = r.consts;
     jr.push_back(std::move(j));
   }
-  std::string src = rebop::jit_source(g->S, jr, g->system);
+  std::string src = rebop::jit_source(g->S, jr, g->system, cap);
   // REBOP_DUMP_SPECIALIZED=<file>: write the generated kernel source there
   if (const char *path = std::getenv("REBOP_DUMP_SPECIALIZED")) {
     if (FILE *f = std::fopen(path, "w")) {
@@ -415,20 +464,32 @@ std::string jit_source_of(const rebop_gillespie *g) {
   return src;
 }
 
+// Is the specialised kernel (cap: its variant with the draw-cap checks) loaded
+// on device `dev` for the handle's current network?
+bool jit_ready(const rebop_gillespie *g, bool cap, int dev) {
+  const int v = cap ? 1 : 0;
+  return !g->jit_dirty[v] && g->jit_dev[v] == dev && g->jitk[v].fn;
+}
+
 // Compiles (cached) and loads the network-specialised kernel on the current device.
-int prepare_jit(rebop_gillespie *g) {
+int prepare_jit(rebop_gillespie *g, bool cap = false) {
   int cur = -1;
   HIP_TRY(hipGetDevice(&cur));
-  if (!g->jit_dirty && g->jit_dev == cur && g->jit.fn) return REBOP_OK;
-  g->jit_src = jit_source_of(g);
+  if (jit_ready(g, cap, cur)) return REBOP_OK;
+  const int v = cap ? 1 : 0;
   std::string err;
-  if (rebop::jit_get(g->jit_src, &g->jit, &err) != 0) {
+  if (rebop::jit_get(jit_source_of(g, cap), &g->jitk[v], &err) != 0) {
     g->jit_err = err;
     return fail(REBOP_ERR_DEVICE, "specialised kernel: " + err);
   }
-  g->jit_dirty = false;
-  g->jit_dev = cur;
+  g->jit_dirty[v] = false;
+  g->jit_dev[v] = cur;
   return REBOP_OK;
+}
+
+void mark_jit_dirty(rebop_gillespie *g) {
+  g->jit_dirty[0] = true;
+  g->jit_dirty[1] = true;
 }
 
 constexpr int kLdsThreads = 64;
@@ -452,17 +513,29 @@ std::vector<rebop_dev_item> make_items(const std::vector<rebop_dev_job> &jobs,
   return items;
 }
 
+// Where enqueue_lds puts the job and item tables: growable buffers, or a
+// fixed region of caller scratch (which must be large enough).
+struct TableDest {
+  DevBuf *jobs = nullptr, *items = nullptr;
+  void *fixed_jobs = nullptr, *fixed_items = nullptr;
+  size_t jobs_cap = 0, items_cap = 0; // bytes of the fixed regions
+};
+
 // Uploads the job and item tables and enqueues the LDS kernel.
+// job_queue: one zeroed 64-bit trajectory counter per job.
 int enqueue_lds(const std::vector<rebop_dev_job> &jobs, const std::vector<double> &cost,
-                DevBuf &djobs, DevBuf &ditems, unsigned long long *counters,
-                unsigned long long *job_events, hipStream_t stream) {
+                const TableDest &dst, unsigned long long *counters,
+                unsigned long long *job_events, unsigned long long *job_queue, bool wide,
+                hipStream_t stream) {
   uint32_t smax = 1, rmax = 1;
   for (auto &j : jobs) {
     smax = std::max(smax, j.S);
     rmax = std::max(rmax, j.R);
   }
-  // counts [S_max][64] int32, then the reaction records (16-byte aligned)
-  const size_t xbytes = ((size_t)smax * kLdsThreads * sizeof(int32_t) + 15) & ~(size_t)15;
+  // counts [S_max][64] (int32, or f64 in the wide kernel), then the reaction
+  // records (16-byte aligned)
+  const size_t cbytes = wide ? sizeof(double) : sizeof(int32_t);
+  const size_t xbytes = ((size_t)smax * kLdsThreads * cbytes + 15) & ~(size_t)15;
   const size_t lds = xbytes + (size_t)rmax * REBOP_REC_WORDS * sizeof(uint32_t);
   if (lds > g_dev.lds_max)
     return fail(REBOP_ERR_UNSUPPORTED,
@@ -472,53 +545,72 @@ int enqueue_lds(const std::vector<rebop_dev_job> &jobs, const std::vector<double
   const std::vector<rebop_dev_item> items = make_items(jobs, cost);
   if (items.empty()) return REBOP_OK;
   if (items.size() > 0xffffffffull) return fail(REBOP_ERR_VALUE, "too many work items");
-  HIP_TRY(djobs.ensure(sizeof(rebop_dev_job) * jobs.size()));
-  HIP_TRY(ditems.ensure(sizeof(rebop_dev_item) * items.size()));
-  HIP_TRY(hipMemcpy(djobs.p, jobs.data(), sizeof(rebop_dev_job) * jobs.size(),
-                    hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(ditems.p, items.data(), sizeof(rebop_dev_item) * items.size(),
-                    hipMemcpyHostToDevice));
+  const size_t jbytes = sizeof(rebop_dev_job) * jobs.size();
+  const size_t ibytes = sizeof(rebop_dev_item) * items.size();
+  void *pj = dst.fixed_jobs, *pi = dst.fixed_items;
+  if (dst.jobs) {
+    HIP_TRY(dst.jobs->ensure(jbytes));
+    HIP_TRY(dst.items->ensure(ibytes));
+    pj = dst.jobs->p;
+    pi = dst.items->p;
+  } else if (jbytes > dst.jobs_cap || ibytes > dst.items_cap) {
+    return fail(REBOP_ERR_VALUE, "the LDS kernel's work items (" + std::to_string(items.size()) +
+                                     ") exceed the scratch limit of " +
+                                     std::to_string(dst.items_cap / sizeof(rebop_dev_item)));
+  }
+  // stream-ordered: a previous launch on this stream may still read the tables
+  HIP_TRY(hipMemcpyAsync(pj, jobs.data(), jbytes, hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(pi, items.data(), ibytes, hipMemcpyHostToDevice, stream));
   int per_cu = 0;
-  HIP_TRY(rebop::lds_occupancy(kLdsThreads, lds, &per_cu));
+  HIP_TRY(rebop::lds_occupancy(kLdsThreads, lds, wide, &per_cu));
   if (per_cu < 1) per_cu = 1;
   const uint64_t waves_per_block = kLdsThreads / 64;
   uint64_t blocks = (uint64_t)per_cu * (uint64_t)g_dev.cus;
   const uint64_t need = (items.size() + waves_per_block - 1) / waves_per_block;
   if (blocks > need) blocks = need;
-  HIP_TRY(rebop::launch_lds((const rebop_dev_job *)djobs.p, (const rebop_dev_item *)ditems.p,
-                            (uint32_t)items.size(), counters, job_events, (int)blocks,
-                            kLdsThreads, lds, (uint32_t)(xbytes / sizeof(uint32_t)), stream));
+  HIP_TRY(rebop::launch_lds((const rebop_dev_job *)pj, (const rebop_dev_item *)pi,
+                            (uint32_t)items.size(), counters, job_events, job_queue, (int)blocks,
+                            kLdsThreads, lds, (uint32_t)(xbytes / sizeof(uint32_t)), wide, stream));
   return REBOP_OK;
 }
 
-// Reads the counters of a finished launch into `stats`.
-int finish_stats(rebop_stats *stats, unsigned long long *counters, hipEvent_t ev0, hipEvent_t ev1,
-                 int kind, hipStream_t stream) {
+// Reads the counters of a finished launch into `stats` (waits for the stream).
+int finish_stats(rebop_stats *stats, unsigned long long *counters, const EventPair &ev, int kind,
+                 hipStream_t stream, uint64_t draw_cap) {
   unsigned long long ctr[REBOP_CTR_COUNT] = {0};
   HIP_TRY(hipMemcpyAsync(ctr, counters, sizeof ctr, hipMemcpyDeviceToHost, stream));
   HIP_TRY(hipStreamSynchronize(stream));
   float ms = 0.f;
-  HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
-  (void)hipEventDestroy(ev0);
-  (void)hipEventDestroy(ev1);
+  if (ev.e0) HIP_TRY(hipEventElapsedTime(&ms, ev.e0, ev.e1));
   stats->events = ctr[REBOP_CTR_EVENTS];
   stats->draws = ctr[REBOP_CTR_DRAWS];
   stats->trajectories = ctr[REBOP_CTR_TRAJ];
   stats->kernel = (uint64_t)kind;
   stats->kernel_ms = ms;
   stats->tail = ctr[REBOP_CTR_SPILL];
+  stats->capped = ctr[REBOP_CTR_CAPPED];
   if (ctr[REBOP_CTR_FLAGS] & REBOP_FLAG_INTERNAL)
     return fail(REBOP_ERR_DEVICE, "internal error: a kernel launch invariant did not hold");
+  if (ctr[REBOP_CTR_FLAGS] & REBOP_FLAG_NARROW) return kRetryWide;
   if (ctr[REBOP_CTR_FLAGS] & REBOP_FLAG_RANGE)
     return fail(REBOP_ERR_RANGE, "a count left the exactly simulated range (|x| <= 2^52), or a "
                                  "total propensity was infinite");
+  if (ctr[REBOP_CTR_FLAGS] & REBOP_FLAG_CAP)
+    return fail(REBOP_ERR_CAP, std::to_string(ctr[REBOP_CTR_CAPPED]) +
+                                   " trajectories reached the cap of " + std::to_string(draw_cap) +
+                                   " draws (their results stop there)");
   return REBOP_OK;
 }
 
 // Enqueues one SSA launch (plus the counter reset) on `stream`.  When
-// `stats` is set, synchronises and fills it.
+// `stats` is set, synchronises and fills it.  With `sc`, the launch counters,
+// sample-time table and LDS work tables live in that caller scratch instead
+// of the handle's buffers, nothing is allocated (the handle must be prepared:
+// network uploaded, kernel compiled), the tail hand-off is off (its results
+// are the same) and the kernel is not timed.
 int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t stream,
-               rebop_stats *stats, int threads = 0, double occ = 1.0) {
+               rebop_stats *stats, int threads = 0, double occ = 1.0,
+               const rebop::Scratch *sc = nullptr) {
   int rc = ensure_device();
   if (rc) return rc;
   if (kernel == REBOP_KERNEL_AUTO && g->tune_kernel >= 0 && threads == 0) {
@@ -537,6 +629,7 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
     return fail(REBOP_ERR_VALUE, "workgroup size must be 64, 128 or 256");
   rc = upload_net(g);
   if (rc) return rc;
+  run.draw_cap = effective_cap(g, run.tmax);
   const uint32_t S = g->S, R = (uint32_t)g->rx.size();
   // kernel choice: LDS when asked; the specialised kernel when asked or (AUTO)
   // when the network is small enough; else the data-driven register kernel
@@ -560,11 +653,11 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
                       ? "network with rate expressions or define_system semantics too large for "
                         "a specialised kernel (S <= 32, R <= 64)"
                       : "network too large for a specialised kernel");
-    rc = prepare_jit(g);
+    rc = prepare_jit(g, run.draw_cap != 0);
     if (rc) return rc;
     spec = true;
   } else if (!use_lds && variant == REBOP_KERNEL_AUTO) {
-    spec = jit_eligible(g) && !jit_disabled() && prepare_jit(g) == REBOP_OK;
+    spec = jit_eligible(g) && !jit_disabled() && prepare_jit(g, run.draw_cap != 0) == REBOP_OK;
     use_lds = !spec && !fits_regs;
   } else if (!use_lds && !fits_regs) {
     return fail(REBOP_ERR_UNSUPPORTED, "network with " + std::to_string(S) + " species and " +
@@ -574,6 +667,9 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
   if (use_lds && run.rates)
     return fail(REBOP_ERR_UNSUPPORTED,
                 "parameter sweeps run on the register and specialised kernels (S <= 32, R <= 64)");
+  unsigned long long *const ctrs =
+      sc ? (unsigned long long *)(sc->base + rebop::kScratchCounters)
+         : (unsigned long long *)g->counters.p;
   if (use_lds) {
     // LDS kernel: one job, items of 64 trajectories
     rebop_dev_job job = g->job;
@@ -583,6 +679,7 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
     job.tmax = run.tmax;
     job.nb_steps = run.nb_steps;
     job.mode = run.mode;
+    job.draw_cap = run.draw_cap;
     job.x0_div = run.x0_div;
     job.x0 = run.x0;
     job.out = run.out;
@@ -594,22 +691,39 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
     job.rec_offsets = run.rec_offsets;
     job.rec_times = run.rec_times;
     job.rec_states = run.rec_states;
-    unsigned long long *ctrs = (unsigned long long *)g->counters.p;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (stats) {
-      HIP_TRY(hipEventCreate(&e0));
-      HIP_TRY(hipEventCreate(&e1));
+    TableDest td;
+    if (sc) {
+      td.fixed_jobs = sc->base + rebop::kScratchJob;
+      td.jobs_cap = rebop::kScratchJobBytes;
+      td.fixed_items = sc->base + rebop::kScratchItems;
+      td.items_cap = rebop::kScratchItemsBytes;
+    } else {
+      td.jobs = &g->djobs;
+      td.items = &g->ditems;
     }
-    HIP_TRY(rebop::launch_reset_counters(ctrs, stream));
-    if (stats) HIP_TRY(hipEventRecord(e0, stream));
-    rc = enqueue_lds(std::vector<rebop_dev_job>{job}, std::vector<double>{1.0}, g->djobs,
-                     g->ditems, ctrs, nullptr, stream);
-    if (rc) return rc;
-    if (stats) {
-      HIP_TRY(hipEventRecord(e1, stream));
-      return finish_stats(stats, ctrs, e0, e1, REBOP_KERNEL_LDS, stream);
+    // int32 counts (narrow) unless asked for, a resumable batch (whose state a
+    // rerun could not restore), or shared initial counts beyond int32; a
+    // narrow run that meets a count beyond int32 is rerun wide (f64 counts)
+    // when its counters are read (stats)
+    bool wide = (kernel & REBOP_KERNEL_WIDE) || run.mode == REBOP_MODE_TARGET;
+    if (!run.x0_div)
+      for (int64_t v : g->x0) wide = wide || v > REBOP_LDS_LIMIT || v < -REBOP_LDS_LIMIT;
+    for (;;) {
+      EventPair ev;
+      if (stats && !sc) HIP_TRY(ev.create());
+      HIP_TRY(rebop::launch_reset_counters(ctrs, stream));
+      if (ev.e0) HIP_TRY(hipEventRecord(ev.e0, stream));
+      rc = enqueue_lds(std::vector<rebop_dev_job>{job}, std::vector<double>{1.0}, td, ctrs,
+                       nullptr, ctrs + REBOP_CTR_JOBQ, wide, stream);
+      if (rc) return rc;
+      if (!stats) return REBOP_OK;
+      if (ev.e1) HIP_TRY(hipEventRecord(ev.e1, stream));
+      rc = finish_stats(stats, ctrs, ev, REBOP_KERNEL_LDS | (wide ? REBOP_KERNEL_WIDE : 0), stream,
+                        run.draw_cap);
+      if (rc != kRetryWide) return rc;
+      if (wide) return fail(REBOP_ERR_DEVICE, "internal error: the wide LDS kernel flagged int32");
+      wide = true;
     }
-    return REBOP_OK;
   }
   // lanes per trajectory: asked for, or (AUTO) when even 4 (2) lanes per
   // trajectory leave at most one wave per SIMD
@@ -635,14 +749,15 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
                                            std::to_string(R) +
                                            " reactions exceeds the register kernels");
   run.persistent = persistent ? 1 : 0;
-  run.counters = (unsigned long long *)g->counters.p;
+  run.counters = ctrs;
+  run.queue = ctrs + REBOP_CTR_QUEUE;
   const size_t lds = spec ? 0 : sizeof(double) * (size_t)R * S;
   const uint64_t need_blocks = (run.n_traj * (uint64_t)group + threads - 1) / threads;
   uint64_t blocks = need_blocks;
   if (persistent) {
     int per_cu = 0;
     if (spec)
-      HIP_TRY(rebop::jit_occupancy(g->jit, threads, &per_cu));
+      HIP_TRY(rebop::jit_occupancy(g->jitk[run.draw_cap ? 1 : 0], threads, &per_cu));
     else
       HIP_TRY(rebop::reg_occupancy(shape, threads, lds, &per_cu));
     if (per_cu < 1) per_cu = 1;
@@ -659,7 +774,7 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
   // trajectories, once, so the arrays need (waves x spill_below) entries
   const uint32_t below = tail_lanes();
   const bool tail = persistent && spec && !run.rates && run.mode != REBOP_MODE_EVENTS &&
-                    !(kernel & REBOP_KERNEL_WHOLE) && below > 0 && run.n_traj > 0;
+                    !(kernel & REBOP_KERNEL_WHOLE) && below > 0 && run.n_traj > 0 && !sc;
   const int tail_k = tail_group();
   uint64_t tail_blocks = 0;
   if (tail) {
@@ -682,21 +797,29 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
     tail_blocks = (cap * (uint64_t)tail_k + threads - 1) / threads;
   }
 
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  if (stats) {
-    HIP_TRY(hipEventCreate(&ev0));
-    HIP_TRY(hipEventCreate(&ev1));
-  }
+  EventPair ev;
+  if (stats && !sc) HIP_TRY(ev.create());
   HIP_TRY(rebop::launch_reset_counters(run.counters, stream));
   // the sample times, read by the kernels at every sample
-  HIP_TRY(g->grid.ensure(sizeof(double) *
-                         (run.mode == REBOP_MODE_GRID ? (size_t)run.nb_steps + 1u : 1u)));
-  run.grid = (const double *)g->grid.p;
-  HIP_TRY(rebop::launch_grid((double *)g->grid.p, run, stream));
-  if (stats) HIP_TRY(hipEventRecord(ev0, stream));
+  const size_t n_grid = run.mode == REBOP_MODE_GRID ? (size_t)run.nb_steps + 1u : 1u;
+  double *grid = nullptr;
+  if (sc) {
+    if (n_grid > rebop::kScratchMaxSamples)
+      return fail(REBOP_ERR_VALUE, "nb_steps + 1 = " + std::to_string(n_grid) +
+                                       " sample times exceed the scratch limit of " +
+                                       std::to_string(rebop::kScratchMaxSamples));
+    grid = (double *)(sc->base + rebop::kScratchGrid);
+  } else {
+    HIP_TRY(g->grid.ensure(sizeof(double) * n_grid));
+    grid = (double *)g->grid.p;
+  }
+  run.grid = grid;
+  HIP_TRY(rebop::launch_grid(grid, run, stream));
+  if (ev.e0) HIP_TRY(hipEventRecord(ev.e0, stream));
   if (run.n_traj > 0) {
     if (spec)
-      HIP_TRY(rebop::jit_launch(g->jit, run, (int)blocks, threads, stream, group));
+      HIP_TRY(rebop::jit_launch(g->jitk[run.draw_cap ? 1 : 0], run, (int)blocks, threads, stream,
+                                group));
     else
       HIP_TRY(rebop::launch_reg(shape, g->view, run, (int)blocks, threads, lds, stream));
   }
@@ -706,15 +829,16 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
     r2.persistent = 0;
     r2.spill_below = 0;
     r2.resume = 1;
-    HIP_TRY(rebop::jit_launch(g->jit, r2, (int)tail_blocks, threads, stream, tail_k));
+    HIP_TRY(rebop::jit_launch(g->jitk[run.draw_cap ? 1 : 0], r2, (int)tail_blocks, threads, stream,
+                              tail_k));
   }
   // the samples were written as f64 bits: convert them to int64 in place
   if (run.out && run.mode == REBOP_MODE_GRID && run.n_traj > 0)
     HIP_TRY(rebop::launch_raw_counts(run.out, (run.nb_steps + 1u) * (uint64_t)S * run.n_traj,
                                      g_dev.cus, stream));
-  if (stats) HIP_TRY(hipEventRecord(ev1, stream));
+  if (ev.e1) HIP_TRY(hipEventRecord(ev.e1, stream));
 
-  if (stats) return finish_stats(stats, run.counters, ev0, ev1, kind, stream);
+  if (stats) return finish_stats(stats, run.counters, ev, kind, stream, run.draw_cap);
   return REBOP_OK;
 }
 
@@ -724,6 +848,61 @@ int check_handle(const void *p, const char *what) {
 }
 
 } // namespace
+
+namespace rebop {
+
+int report_error(int code, const char *msg) { return fail(code, msg); }
+
+int run_device_scratch(const rebop_gillespie *gc, double tmax, uint64_t nb_steps,
+                       uint64_t traj_begin, uint64_t n_traj, const int64_t *d_x0, int64_t *d_out,
+                       int64_t *d_final, int kernel, void *stream, rebop_stats *stats,
+                       const Scratch *sc) {
+  if (int rc = check_handle(gc, "gillespie")) return rc;
+  rebop_gillespie *g = const_cast<rebop_gillespie *>(gc);
+  if (nb_steps == 0)
+    return fail(REBOP_ERR_VALUE, "nb_steps must be > 0 for a grid run");
+  // sample positions are 31-bit in the kernels
+  if (nb_steps >= (1ull << 31)) return fail(REBOP_ERR_VALUE, "nb_steps must be below 2^31");
+  if (!d_x0) {
+    if (int rc = check_x0(g->x0.data(), g->x0.size())) return rc;
+  }
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  int rc = ensure_device();
+  if (rc) return rc;
+  if (sc) {
+    // nothing may be allocated: the network must be on this device already,
+    // and the specialised kernel compiled where the launch would use it
+    int cur = -1;
+    HIP_TRY(hipGetDevice(&cur));
+    if (g->dirty || g->dev_id != cur)
+      return fail(REBOP_ERR_VALUE, "the network is not prepared on this device: call "
+                                   "rebop_gillespie_prepare first");
+    const int variant = kernel & 0xF;
+    const bool wants_jit = specialised_only(g) || (kernel & REBOP_KERNEL_SPECIALIZED) ||
+                           (variant == REBOP_KERNEL_AUTO && jit_eligible(g) && !jit_disabled());
+    if (wants_jit && !jit_ready(g, effective_cap(g, tmax) != 0, cur))
+      return fail(REBOP_ERR_VALUE, "the specialised kernel is not compiled on this device: call "
+                                   "rebop_gillespie_prepare with this kernel kind (after setting "
+                                   "any draw cap) first");
+  } else {
+    rc = upload_net(g);
+    if (rc) return rc;
+  }
+  rebop_dev_run run{};
+  run.seed = g->seed;
+  run.traj_begin = traj_begin;
+  run.n_traj = n_traj;
+  run.tmax = tmax;
+  run.nb_steps = nb_steps;
+  run.mode = REBOP_MODE_GRID;
+  run.x0 = d_x0 ? d_x0 : (const int64_t *)g->dx0.p;
+  run.x0_div = d_x0 ? 1 : 0;
+  run.out = d_out;
+  run.final_x = d_final;
+  return launch_run(g, run, kernel, (hipStream_t)stream, stats, 0, 1.0, sc);
+}
+
+} // namespace rebop
 
 extern "C" {
 
@@ -805,11 +984,17 @@ int rebop_system_add_reaction(rebop_gillespie *g, double rate, const uint32_t *r
   r.diff = std::move(d);
   g->rx.push_back(std::move(r));
   g->dirty = true;
-  g->jit_dirty = true;
+  mark_jit_dirty(g);
   return REBOP_OK;
 }
 
 void rebop_gillespie_free(rebop_gillespie *g) { delete g; }
+
+int rebop_gillespie_set_draw_cap(rebop_gillespie *g, uint64_t max_draws) {
+  if (int rc = check_handle(g, "gillespie")) return rc;
+  g->draw_cap = max_draws;
+  return REBOP_OK;
+}
 
 int rebop_gillespie_seed(rebop_gillespie *g, uint64_t seed) {
   if (int rc = check_handle(g, "gillespie")) return rc;
@@ -871,7 +1056,7 @@ static int add_reaction_common(rebop_gillespie *g, double rate, bool rate_sparse
   r.diff.assign(diffs, diffs + nd);
   g->rx.push_back(std::move(r));
   g->dirty = true;
-  g->jit_dirty = true;
+  mark_jit_dirty(g);
   return REBOP_OK;
 }
 
@@ -950,7 +1135,7 @@ int rebop_gillespie_add_reaction_expr(rebop_gillespie *g, const uint32_t *code, 
   g->rx.push_back(std::move(r));
   g->n_expr += 1;
   g->dirty = true;
-  g->jit_dirty = true;
+  mark_jit_dirty(g);
   return REBOP_OK;
 }
 
@@ -983,57 +1168,46 @@ int rebop_gillespie_get_species(const rebop_gillespie *g, int64_t *species, uint
 int rebop_gillespie_prepare(const rebop_gillespie *gc, int kernel) {
   if (int rc = check_handle(gc, "gillespie")) return rc;
   rebop_gillespie *g = const_cast<rebop_gillespie *>(gc);
-  if (!(kernel & REBOP_KERNEL_SPECIALIZED)) return REBOP_OK;
+  int cur = -1;
+  const bool has_dev = hipGetDevice(&cur) == hipSuccess;
+  // AUTO compiles the specialised kernel where an AUTO run would use it
+  const bool auto_jit = (kernel & 0xF) == REBOP_KERNEL_AUTO && !(kernel & REBOP_KERNEL_SPECIALIZED) &&
+                        jit_eligible(g) && !jit_disabled();
+  if (!(kernel & REBOP_KERNEL_SPECIALIZED) && !auto_jit && !specialised_only(g)) {
+    if (!has_dev) return REBOP_OK;
+    if (int rc = ensure_device()) return rc;
+    return upload_net(g); // the network on the device (the data-driven and LDS kernels read it)
+  }
   if (!jit_eligible(g))
     return fail(REBOP_ERR_UNSUPPORTED, "network too large for a specialised kernel");
-  int cur = -1;
-  if (hipGetDevice(&cur) != hipSuccess) {
+  if (!has_dev) {
     // no device: compile only, to check the generated kernel
     std::string err;
     size_t cs = 0;
-    if (rebop::jit_compile_only(jit_source_of(g), &cs, &err) != 0)
+    if (rebop::jit_compile_only(jit_source_of(g, g->draw_cap != 0), &cs, &err) != 0)
       return fail(REBOP_ERR_DEVICE, "specialised kernel: " + err);
     return REBOP_OK;
   }
   if (int rc = ensure_device()) return rc;
   if (int rc = upload_net(g)) return rc;
+  // with a draw cap set, the variant that checks it too
+  if (g->draw_cap)
+    if (int rc = prepare_jit(g, true)) return rc;
   return prepare_jit(g);
 }
 
 int rebop_run_device(const rebop_gillespie *gc, double tmax, uint64_t nb_steps,
                      uint64_t traj_begin, uint64_t n_traj, const int64_t *d_x0, int64_t *d_out,
                      int64_t *d_final, int kernel, void *stream, rebop_stats *stats) {
-  if (int rc = check_handle(gc, "gillespie")) return rc;
-  rebop_gillespie *g = const_cast<rebop_gillespie *>(gc);
-  if (nb_steps == 0)
-    return fail(REBOP_ERR_VALUE, "nb_steps must be > 0 for a grid run");
-  if (nb_steps > (1ull << 40)) return fail(REBOP_ERR_VALUE, "nb_steps too large");
-  if (!d_x0) {
-    if (int rc = check_x0(g->x0.data(), g->x0.size())) return rc;
-  }
-  if (stats) std::memset(stats, 0, sizeof *stats);
-  int rc = ensure_device();
-  if (rc) return rc;
-  rc = upload_net(g);
-  if (rc) return rc;
-  rebop_dev_run run{};
-  run.seed = g->seed;
-  run.traj_begin = traj_begin;
-  run.n_traj = n_traj;
-  run.tmax = tmax;
-  run.nb_steps = nb_steps;
-  run.mode = REBOP_MODE_GRID;
-  run.x0 = d_x0 ? d_x0 : (const int64_t *)g->dx0.p;
-  run.x0_div = d_x0 ? 1 : 0;
-  run.out = d_out;
-  run.final_x = d_final;
-  return launch_run(g, run, kernel, (hipStream_t)stream, stats);
+  return rebop::run_device_scratch(gc, tmax, nb_steps, traj_begin, n_traj, d_x0, d_out, d_final,
+                                   kernel, stream, stats, nullptr);
 }
 
 int rebop_autotune(rebop_gillespie *g, double tmax, uint64_t nb_steps, uint64_t n_pilot,
                    rebop_tune_result *out) {
   if (int rc = check_handle(g, "gillespie")) return rc;
   if (nb_steps == 0) return fail(REBOP_ERR_VALUE, "nb_steps must be > 0");
+  if (nb_steps >= (1ull << 31)) return fail(REBOP_ERR_VALUE, "nb_steps must be below 2^31");
   if (n_pilot == 0) return fail(REBOP_ERR_VALUE, "n_pilot must be > 0");
   if (int rc = check_x0(g->x0.data(), g->x0.size())) return rc;
   int rc = ensure_device();
@@ -1111,6 +1285,7 @@ int rebop_run_sweep_device(const rebop_gillespie *gc, uint64_t n_points, uint64_
   if (int rc = check_handle(gc, "gillespie")) return rc;
   rebop_gillespie *g = const_cast<rebop_gillespie *>(gc);
   if (nb_steps == 0) return fail(REBOP_ERR_VALUE, "nb_steps must be > 0 for a grid run");
+  if (nb_steps >= (1ull << 31)) return fail(REBOP_ERR_VALUE, "nb_steps must be below 2^31");
   if (traj_per_point == 0) return fail(REBOP_ERR_VALUE, "traj_per_point must be > 0");
   if (d_rates && g->n_expr)
     return fail(REBOP_ERR_UNSUPPORTED,
@@ -1146,6 +1321,7 @@ int rebop_run(const rebop_gillespie *gc, double tmax, uint64_t nb_steps, uint64_
               rebop_stats *stats) {
   if (int rc = check_handle(gc, "gillespie")) return rc;
   if (nb_steps == 0) return fail(REBOP_ERR_VALUE, "nb_steps must be > 0 for a grid run");
+  if (nb_steps >= (1ull << 31)) return fail(REBOP_ERR_VALUE, "nb_steps must be below 2^31");
   const uint32_t S = gc->S;
   if (x0) {
     if (int rc = check_x0(x0, (size_t)n_traj * S)) return rc;
@@ -1182,10 +1358,12 @@ int rebop_run_jobs_device(const rebop_job *jobs, uint32_t n_jobs, void *stream_,
   std::vector<rebop_dev_job> dj;
   std::vector<double> cost;
   dj.reserve(n_jobs);
+  uint64_t cap_max = 0;
   for (uint32_t j = 0; j < n_jobs; ++j) {
     const rebop_job &J = jobs[j];
     if (int e = check_handle(J.net, "job net")) return e;
-    if (J.nb_steps == 0) return fail(REBOP_ERR_VALUE, "job " + std::to_string(j) + ": nb_steps must be > 0");
+    if (J.nb_steps == 0 || J.nb_steps >= (1ull << 31))
+      return fail(REBOP_ERR_VALUE, "job " + std::to_string(j) + ": nb_steps must be in 1 .. 2^31 - 1");
     rebop_gillespie *g = const_cast<rebop_gillespie *>(J.net);
     if (specialised_only(g))
       return fail(REBOP_ERR_UNSUPPORTED, "job " + std::to_string(j) +
@@ -1204,6 +1382,8 @@ int rebop_run_jobs_device(const rebop_job *jobs, uint32_t n_jobs, void *stream_,
     d.tmax = J.tmax;
     d.nb_steps = J.nb_steps;
     d.mode = REBOP_MODE_GRID;
+    d.draw_cap = effective_cap(g, J.tmax);
+    cap_max = std::max(cap_max, d.draw_cap);
     d.x0_div = J.d_x0 ? 1 : 0;
     d.x0 = J.d_x0 ? J.d_x0 : (const int64_t *)g->dx0.p;
     d.out = J.d_out;
@@ -1211,33 +1391,59 @@ int rebop_run_jobs_device(const rebop_job *jobs, uint32_t n_jobs, void *stream_,
     dj.push_back(d);
     cost.push_back(J.cost_hint);
   }
-  DevBuf counters, djobs, ditems, dev_ev;
-  HIP_TRY(counters.ensure(sizeof(unsigned long long) * REBOP_CTR_COUNT));
-  unsigned long long *ctrs = (unsigned long long *)counters.p;
-  unsigned long long *jev = nullptr;
-  if (events_per_job && n_jobs) {
-    HIP_TRY(dev_ev.ensure(sizeof(unsigned long long) * n_jobs));
-    HIP_TRY(hipMemsetAsync(dev_ev.p, 0, sizeof(unsigned long long) * n_jobs, stream));
-    jev = (unsigned long long *)dev_ev.p;
+  // device tables and counters, kept per (device, stream) and reused: calls
+  // on one stream are ordered, so the next call rewrites them only after the
+  // previous launch has read them
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  JobBuffers *jb = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_job_mu);
+    jb = &(*g_job_bufs)[std::make_pair(dev, stream)];
   }
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
-  HIP_TRY(rebop::launch_reset_counters(ctrs, stream));
-  HIP_TRY(hipEventRecord(e0, stream));
-  rc = enqueue_lds(dj, cost, djobs, ditems, ctrs, jev, stream);
-  if (rc) {
-    (void)hipStreamSynchronize(stream);
+  HIP_TRY(jb->counters.ensure(sizeof(unsigned long long) * REBOP_CTR_COUNT));
+  unsigned long long *ctrs = (unsigned long long *)jb->counters.p;
+  if (events_per_job && n_jobs) HIP_TRY(jb->dev_ev.ensure(sizeof(unsigned long long) * n_jobs));
+  unsigned long long *jev = events_per_job && n_jobs ? (unsigned long long *)jb->dev_ev.p : nullptr;
+  // one trajectory queue per job
+  HIP_TRY(jb->dev_q.ensure(sizeof(unsigned long long) * (n_jobs ? n_jobs : 1)));
+  unsigned long long *jq = (unsigned long long *)jb->dev_q.p;
+  TableDest td;
+  td.jobs = &jb->djobs;
+  td.items = &jb->ditems;
+  // int32 counts unless some job's shared initial counts are beyond int32; a
+  // run that meets a count beyond int32 is rerun with f64 counts when its
+  // counters are read (stats or events_per_job)
+  bool wide = false;
+  for (uint32_t j = 0; j < n_jobs; ++j)
+    if (!jobs[j].d_x0)
+      for (int64_t v : jobs[j].net->x0) wide = wide || v > REBOP_LDS_LIMIT || v < -REBOP_LDS_LIMIT;
+  const bool read = stats || jev;
+  rebop_stats local{};
+  for (;;) {
+    if (jev) HIP_TRY(rebop::launch_zero_u64(jev, n_jobs, stream));
+    HIP_TRY(rebop::launch_zero_u64(jq, n_jobs, stream));
+    EventPair ev;
+    if (stats) HIP_TRY(ev.create());
+    HIP_TRY(rebop::launch_reset_counters(ctrs, stream));
+    if (ev.e0) HIP_TRY(hipEventRecord(ev.e0, stream));
+    rc = enqueue_lds(dj, cost, td, ctrs, jev, jq, wide, stream);
+    if (rc) return rc;
+    if (ev.e1) HIP_TRY(hipEventRecord(ev.e1, stream));
+    if (!read) return REBOP_OK;
+    if (jev)
+      HIP_TRY(hipMemcpyAsync(events_per_job, jev, sizeof(unsigned long long) * n_jobs,
+                             hipMemcpyDeviceToHost, stream));
+    rc = finish_stats(&local, ctrs, ev, REBOP_KERNEL_LDS | (wide ? REBOP_KERNEL_WIDE : 0), stream,
+                      cap_max);
+    if (rc == kRetryWide && !wide) {
+      wide = true;
+      continue;
+    }
+    if (rc == kRetryWide) return fail(REBOP_ERR_DEVICE, "internal error: the wide LDS kernel flagged int32");
+    if (stats) *stats = local;
     return rc;
   }
-  HIP_TRY(hipEventRecord(e1, stream));
-  rebop_stats local{};
-  rc = finish_stats(&local, ctrs, e0, e1, REBOP_KERNEL_LDS, stream);
-  if (stats) *stats = local;
-  if (jev)
-    HIP_TRY(hipMemcpy(events_per_job, jev, sizeof(unsigned long long) * n_jobs,
-                      hipMemcpyDeviceToHost));
-  return rc;
 }
 
 int rebop_run_all_events(const rebop_gillespie *gc, double tmax, uint64_t traj_begin,
@@ -1245,8 +1451,6 @@ int rebop_run_all_events(const rebop_gillespie *gc, double tmax, uint64_t traj_b
                          rebop_stats *stats) {
   if (int rc = check_handle(gc, "gillespie")) return rc;
   if (!n_records && n_traj) return fail(REBOP_ERR_VALUE, "n_records is NULL");
-  if (std::isinf(tmax) && tmax > 0)
-    return fail(REBOP_ERR_VALUE, "tmax must be finite: every event up to +inf never ends");
   rebop_gillespie *g = const_cast<rebop_gillespie *>(gc);
   if (int rc = check_x0(g->x0.data(), g->x0.size())) return rc;
   if (stats) std::memset(stats, 0, sizeof *stats);
@@ -1270,13 +1474,14 @@ int rebop_run_all_events(const rebop_gillespie *gc, double tmax, uint64_t traj_b
   const bool want = times != nullptr || states != nullptr;
   rebop_stats local{};
   if (!want) {
-    // pass 1: count the records of every trajectory
+    // pass 1: count the records of every trajectory (also when the draw cap
+    // stopped some: their counts end at the cap)
     rc = launch_run(g, run, REBOP_KERNEL_AUTO, nullptr, &local);
-    if (rc) return rc;
+    if (rc && rc != REBOP_ERR_CAP) return rc;
     if (n_traj)
       HIP_TRY(hipMemcpy(n_records, dcount.p, sizeof(uint64_t) * n_traj, hipMemcpyDeviceToHost));
     if (stats) *stats = local;
-    return REBOP_OK;
+    return rc;
   }
   // pass 2: the caller's counts give the offsets; the kernel checks them
   std::vector<uint64_t> off(n_traj ? n_traj : 1, 0);
@@ -1299,12 +1504,12 @@ int rebop_run_all_events(const rebop_gillespie *gc, double tmax, uint64_t traj_b
   if (rc == REBOP_ERR_RANGE)
     return fail(REBOP_ERR_VALUE, "n_records does not match this network and seed (or a count "
                                  "left the exactly simulated range)");
-  if (rc) return rc;
+  if (rc && rc != REBOP_ERR_CAP) return rc;
   if (times && total) HIP_TRY(hipMemcpy(times, dt.p, sizeof(double) * total, hipMemcpyDeviceToHost));
   if (states && total * S)
     HIP_TRY(hipMemcpy(states, ds.p, sizeof(int64_t) * total * S, hipMemcpyDeviceToHost));
   if (stats) *stats = local;
-  return REBOP_OK;
+  return rc;
 }
 
 int rebop_run_all_events_device(const rebop_gillespie *gc, double tmax, uint64_t traj_begin,
@@ -1313,8 +1518,6 @@ int rebop_run_all_events_device(const rebop_gillespie *gc, double tmax, uint64_t
                                 rebop_stats *stats) {
   if (int rc = check_handle(gc, "gillespie")) return rc;
   if (!d_n_records && n_traj) return fail(REBOP_ERR_VALUE, "d_n_records is NULL");
-  if (std::isinf(tmax) && tmax > 0)
-    return fail(REBOP_ERR_VALUE, "tmax must be finite: every event up to +inf never ends");
   const bool want = d_offsets != nullptr || d_times != nullptr || d_states != nullptr;
   if (want && n_traj && (!d_offsets || !d_times || !d_states))
     return fail(REBOP_ERR_VALUE, "pass 2 needs d_offsets, d_times and d_states");

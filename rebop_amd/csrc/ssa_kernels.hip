@@ -16,10 +16,12 @@
 
 namespace rebop {
 
-template <int MS_, int MR_>
+template <int MS_, int MR_, bool CAP>
 struct GenericNet {
   static constexpr int MS = MS_;
   static constexpr int MR = MR_;
+  static constexpr bool kCap = CAP;
+  static constexpr uint32_t kRangeFlag = REBOP_FLAG_RANGE;
   rebop_dev_net net;
   const double *lds_diff; // [R][S]
   double kr[MR];          // this trajectory's rate constants
@@ -87,14 +89,14 @@ struct GenericNet {
   }
 };
 
-template <int MS, int MR>
+template <int MS, int MR, bool CAP>
 __global__ void __launch_bounds__(256) ssa_reg_kernel(rebop_dev_net net, rebop_dev_run run) {
   extern __shared__ double lds_diff[]; // [R][S]
   __shared__ unsigned long long blk_ctr[4];
   for (uint32_t q = threadIdx.x; q < net.R * net.S; q += blockDim.x) lds_diff[q] = net.diff[q];
   if (threadIdx.x < 4) blk_ctr[threadIdx.x] = 0ull;
   __syncthreads();
-  GenericNet<MS, MR> pol;
+  GenericNet<MS, MR, CAP> pol;
   pol.net = net;
   pol.lds_diff = lds_diff;
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -104,8 +106,6 @@ __global__ void __launch_bounds__(256) ssa_reg_kernel(rebop_dev_net net, rebop_d
   flush_totals(acc, run, blk_ctr);
 }
 
-// Clears the launch counters with a kernel (not hipMemsetAsync), so that the
-// sequence stays valid inside a captured hipGraph.
 // The sample times tmax * i / nb_steps (src/pyo3_gillespie.rs:202: product,
 // then quotient, as grid_time computes them), or tmax for a target run.
 __global__ void grid_kernel(double *grid, double tmax, uint64_t nb, uint64_t n, int is_grid) {
@@ -113,8 +113,16 @@ __global__ void grid_kernel(double *grid, double tmax, uint64_t nb, uint64_t n, 
   if (i < n) grid[i] = is_grid ? grid_time(tmax, i, nb) : tmax;
 }
 
+// Clears the launch counters with a kernel (not hipMemsetAsync), so that the
+// sequence stays valid inside a captured hipGraph.
 __global__ void reset_counters_kernel(unsigned long long *counters) {
   if (threadIdx.x < REBOP_CTR_COUNT) counters[threadIdx.x] = 0ull;
+}
+
+// Clears n 64-bit words (per-job event counters) with a kernel, for the same reason.
+__global__ void zero_u64_kernel(unsigned long long *p, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0ull;
 }
 
 } // namespace rebop
@@ -136,9 +144,9 @@ int reg_shape_mr(int c) { return kRegShapes[c].mr; }
 
 static const void *reg_kernel_ptr(int c) {
   switch (c) {
-  case 0: return (const void *)&ssa_reg_kernel<4, 8>;
-  case 1: return (const void *)&ssa_reg_kernel<8, 16>;
-  case 2: return (const void *)&ssa_reg_kernel<16, 32>;
+  case 0: return (const void *)&ssa_reg_kernel<4, 8, false>;
+  case 1: return (const void *)&ssa_reg_kernel<8, 16, false>;
+  case 2: return (const void *)&ssa_reg_kernel<16, 32, false>;
   default: return nullptr;
   }
 }
@@ -150,6 +158,14 @@ hipError_t reg_occupancy(int c, int threads, size_t lds, int *blocks_per_cu) {
 
 hipError_t launch_reset_counters(unsigned long long *counters, hipStream_t stream) {
   hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, stream, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_u64(unsigned long long *p, uint64_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  uint64_t blocks = (n + 255u) / 256u;
+  if (blocks > 1024u) blocks = 1024u;
+  hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, p, n);
   return hipGetLastError();
 }
 
@@ -196,12 +212,17 @@ hipError_t launch_grid(double *grid, const rebop_dev_run &run, hipStream_t strea
   return hipGetLastError();
 }
 
+// the variant with the draw-cap checks when the run has a cap
 hipError_t launch_reg(int c, const rebop_dev_net &net, const rebop_dev_run &run, int blocks,
                       int threads, size_t lds, hipStream_t stream) {
-  switch (c) {
-  case 0: hipLaunchKernelGGL((ssa_reg_kernel<4, 8>), dim3(blocks), dim3(threads), lds, stream, net, run); break;
-  case 1: hipLaunchKernelGGL((ssa_reg_kernel<8, 16>), dim3(blocks), dim3(threads), lds, stream, net, run); break;
-  case 2: hipLaunchKernelGGL((ssa_reg_kernel<16, 32>), dim3(blocks), dim3(threads), lds, stream, net, run); break;
+  const bool cap = run.draw_cap != 0;
+  switch (c * 2 + (cap ? 1 : 0)) {
+  case 0: hipLaunchKernelGGL((ssa_reg_kernel<4, 8, false>), dim3(blocks), dim3(threads), lds, stream, net, run); break;
+  case 1: hipLaunchKernelGGL((ssa_reg_kernel<4, 8, true>), dim3(blocks), dim3(threads), lds, stream, net, run); break;
+  case 2: hipLaunchKernelGGL((ssa_reg_kernel<8, 16, false>), dim3(blocks), dim3(threads), lds, stream, net, run); break;
+  case 3: hipLaunchKernelGGL((ssa_reg_kernel<8, 16, true>), dim3(blocks), dim3(threads), lds, stream, net, run); break;
+  case 4: hipLaunchKernelGGL((ssa_reg_kernel<16, 32, false>), dim3(blocks), dim3(threads), lds, stream, net, run); break;
+  case 5: hipLaunchKernelGGL((ssa_reg_kernel<16, 32, true>), dim3(blocks), dim3(threads), lds, stream, net, run); break;
   default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -16,12 +16,22 @@
 //                       are recomputed in the reference's order (make_cumrates
 //                       :357-364 over Rate::rate :71-90) right after the jump
 //
+// A watchdog the reference does not have: with run.draw_cap set, a
+// trajectory whose draws in this launch reach the cap is stopped at that
+// event (its state is treated as absorbing from there), so a trajectory whose
+// propensity keeps growing cannot hold its wave, and the launch, forever.
+// The run then reports REBOP_FLAG_CAP.  The check is compiled in only for
+// policies with Net::kCap (the host launches those when a cap is set), so
+// uncapped runs pay nothing for it.
+//
 // The network enters through a policy object `net`.  The lane's register
 // state is two local arrays of lane_loop, double x[Net::MS] (counts, for
 // policies that keep them in VGPRs) and double cum[Net::MR] (what choose
 // needs), handed to every policy call (local arrays, not struct members, so
 // that the compiler keeps them in VGPRs even when x is indexed by a
 // wave-uniform runtime species index):
+//   static constexpr bool kCap;  compile the draw-cap checks in
+//   static constexpr uint32_t kRangeFlag;  raised when in_range fails
 //   uint32_t S() const;
 //   void start(const rebop_dev_run &run, uint64_t j);    per-trajectory setup (rates)
 //   void load_x0(X &x, const int64_t *src);                counts from an int64 row
@@ -134,10 +144,24 @@ __device__ __forceinline__ uint64_t next_trajectory(const rebop_dev_run &run, bo
   const uint32_t lane = __lane_id();
   const int leader = __ffsll((unsigned long long)mask) - 1;
   unsigned long long base = 0ull;
-  if ((int)lane == leader)
-    base = atomicAdd(&run.counters[REBOP_CTR_QUEUE], (unsigned long long)__popcll(mask));
+  if ((int)lane == leader) base = atomicAdd(run.queue, (unsigned long long)__popcll(mask));
   base = __shfl(base, leader);
   return queue_offset + base + (uint64_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// The draw count at which a trajectory that started this launch with d0
+// draws hits the watchdog (never, without a cap).
+__device__ __forceinline__ uint64_t cap_limit(const rebop_dev_run &run, uint64_t d0) {
+  const uint64_t c = run.draw_cap;
+  return (c == 0ull || d0 + c < d0) ? ~0ull : d0 + c;
+}
+
+// A trajectory reached the draw cap: flag the run, count it once (`lead`),
+// and return the total propensity that makes its state absorbing from here.
+__device__ __forceinline__ double hit_cap(const rebop_dev_run &run, LaneTotals &acc, bool lead) {
+  acc.flags |= REBOP_FLAG_CAP;
+  if (lead) atomicAdd(&run.counters[REBOP_CTR_CAPPED], 1ull);
+  return 0.0;
 }
 
 // make_cumrates after a change of the counts.  An infinite total would stop
@@ -170,7 +194,7 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
 #pragma unroll
   for (int r = 0; r < Net::MR; ++r) cum[r] = 0.0;
   double tot = 0.0, t = 0.0, T = 0.0;
-  uint64_t draw = 0, draw0 = 0, mark = 0, jl = 0;
+  uint64_t draw = 0, draw0 = 0, mark = 0, jl = 0, dlim = ~0ull;
   int64_t *orec = nullptr;
   uint32_t i = 0, ov = 0;
   LaneTotals acc;
@@ -196,9 +220,10 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
           t = 0.0;
           draw = 0;
         }
-        if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
+        if (!net.in_range(x)) acc.flags |= Net::kRangeFlag;
         draw0 = draw;
         mark = draw;
+        if constexpr (Net::kCap) dlim = cap_limit(run, draw);
         ov = 0;
         i = 0;
         T = net.grid_time(run, 0);
@@ -234,7 +259,7 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
         net.write_raw(x, orec, n);
         orec += Sn;
       }
-      if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
+      if (!net.in_range(x)) acc.flags |= Net::kRangeFlag;
       if (draw - mark >= REBOP_DRAWS_PER_CHECK) acc.flags |= REBOP_FLAG_RANGE;
       mark = draw;
       t = T;
@@ -257,6 +282,8 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
       t = tn;
       net.affect(x, net.choose(x, cum, tot * u));
       tot = props_checked(net, x, cum, acc);
+      if constexpr (Net::kCap)
+        if (__builtin_expect(draw >= dlim, 0)) tot = hit_cap(run, acc, true);
     }
   }
   if (spill) {
@@ -274,7 +301,9 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
       run.spill_draw[e] = draw;
       run.spill_mark[e] = mark;
       run.spill_t[e] = t;
-      run.spill_i[e] = i;
+      // bit 31: the trajectory was stopped by the draw cap (its zero total
+      // is forced, not recomputed from the counts); grid positions < 2^31
+      run.spill_i[e] = i | ((Net::kCap && tot == 0.0 && draw >= dlim) ? 0x80000000u : 0u);
       net.store_state(x, run.spill_x + e, run.spill_cap);
     } else {
       acc.flags |= REBOP_FLAG_INTERNAL; // cannot happen: one entry per lane
@@ -352,12 +381,14 @@ __device__ __forceinline__ LaneTotals lane_loop_group(Net &net, const rebop_dev_
   double t;
   uint64_t draw, mark;
   uint32_t i = 0, ov = 0;
+  bool capped = false;
   if (run.resume) {
     net.load_state(x, run.spill_x + e, run.spill_cap);
     t = run.spill_t[e];
     draw = run.spill_draw[e];
     mark = run.spill_mark[e];
-    i = run.spill_i[e];
+    i = run.spill_i[e] & 0x7fffffffu;
+    capped = (run.spill_i[e] >> 31) != 0u;
   } else if (run.st_x) {
     net.load_state(x, run.st_x + jl, n);
     t = run.st_t[jl];
@@ -369,13 +400,18 @@ __device__ __forceinline__ LaneTotals lane_loop_group(Net &net, const rebop_dev_
     draw = 0;
     mark = 0;
   }
-  if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
+  if (!net.in_range(x)) acc.flags |= Net::kRangeFlag;
   const uint64_t draw0 = draw;
+  // the cap counts from the trajectory's draws at the start of this call
+  // (a resumed tail entry started it in the persistent launch)
+  const uint64_t dlim =
+      cap_limit(run, run.resume ? (run.st_x ? run.st_draw[jl] : 0ull) : draw);
   // latency-bound launches: computing a sample time beats loading it
   double T = grid_time(run, i), tot = 0.0;
   int64_t *orec = run.out && lead ? run.out + jl + (uint64_t)i * Sn : nullptr;
   bool done = false;
   tot = props_checked(net, x, cum, acc);
+  if (capped) tot = 0.0; // stopped by the cap before the hand-off (already counted)
 
   while (!done) {
     const rebop_u32x4 blk = rebop_draw_block(run.seed, run.traj_begin + jl, draw + sub);
@@ -398,7 +434,7 @@ __device__ __forceinline__ LaneTotals lane_loop_group(Net &net, const rebop_dev_
             net.write_raw(x, orec, n);
             orec += Sn;
           }
-          if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
+          if (!net.in_range(x)) acc.flags |= Net::kRangeFlag;
           if (draw - mark >= REBOP_DRAWS_PER_CHECK) acc.flags |= REBOP_FLAG_RANGE;
           mark = draw;
           t = T;
@@ -423,6 +459,8 @@ __device__ __forceinline__ LaneTotals lane_loop_group(Net &net, const rebop_dev_
           t = tn;
           net.affect(x, net.choose(x, cum, tot * u));
           tot = props_checked(net, x, cum, acc);
+          if constexpr (Net::kCap)
+            if (__builtin_expect(draw >= dlim, 0)) tot = hit_cap(run, acc, lead);
         }
       }
     }
@@ -455,7 +493,7 @@ __device__ __forceinline__ LaneTotals lane_events(Net &net, const rebop_dev_run 
     if (jl >= n) break;
     net.start(run, jl);
     net.load_x0(x, run.x0_div ? run.x0 + (jl / run.x0_div) * S : run.x0);
-    if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
+    if (!net.in_range(x)) acc.flags |= Net::kRangeFlag;
     const bool write = run.rec_offsets != nullptr;
     const uint64_t base = write ? run.rec_offsets[jl] : 0ull;
     // the caller's count bounds the writes: a count that does not match this
@@ -463,6 +501,8 @@ __device__ __forceinline__ LaneTotals lane_events(Net &net, const rebop_dev_run 
     const uint64_t cap = write ? run.n_records[jl] : 0ull;
     double t = 0.0;
     uint64_t draw = 0, rec = 0;
+    const uint64_t dlim = cap_limit(run, 0ull);
+    bool stop = false; // the draw cap was reached
     auto emit = [&]() {
       if (write && rec < cap) {
         run.rec_times[base + rec] = t;
@@ -471,7 +511,7 @@ __device__ __forceinline__ LaneTotals lane_events(Net &net, const rebop_dev_run 
       rec += 1;
     };
     emit();
-    while (t < run.tmax) {
+    while (t < run.tmax && !stop) {
       const double tot = net.props(x, cum);
       if (!(0.0 < tot)) {
         t = __builtin_inf();
@@ -483,7 +523,13 @@ __device__ __forceinline__ LaneTotals lane_events(Net &net, const rebop_dev_run 
         draw += 1;
         t = t + rebop_exp1_of(blk) / tot;
         net.affect(x, net.choose(x, cum, tot * rebop_unif_of(blk)));
-        if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
+        if (!net.in_range(x)) acc.flags |= Net::kRangeFlag;
+        if constexpr (Net::kCap) {
+          if (__builtin_expect(draw >= dlim, 0)) {
+            hit_cap(run, acc, true);
+            stop = true;
+          }
+        }
       }
       emit();
     }
@@ -520,12 +566,14 @@ __device__ __forceinline__ LaneTotals lane_events_group(Net &net, const rebop_de
   for (int r = 0; r < Net::MR; ++r) cum[r] = 0.0;
   net.start(run, jl);
   net.load_x0(x, run.x0_div ? run.x0 + (jl / run.x0_div) * S : run.x0);
-  if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
+  if (!net.in_range(x)) acc.flags |= Net::kRangeFlag;
   const bool write = run.rec_offsets != nullptr;
   const uint64_t base = write ? run.rec_offsets[jl] : 0ull;
   const uint64_t cap = write ? run.n_records[jl] : 0ull;
   double t = 0.0;
   uint64_t draw = 0, rec = 0;
+  const uint64_t dlim = cap_limit(run, 0ull);
+  bool stop = false; // the draw cap was reached
   auto emit = [&]() {
     if (write && lead && rec < cap) {
       run.rec_times[base + rec] = t;
@@ -534,14 +582,14 @@ __device__ __forceinline__ LaneTotals lane_events_group(Net &net, const rebop_de
     rec += 1;
   };
   emit();
-  while (t < run.tmax) {
+  while (t < run.tmax && !stop) {
     const rebop_u32x4 blk = rebop_draw_block(run.seed, run.traj_begin + jl, draw + sub);
     const double e_own = rebop_exp1_of(blk), u_own = rebop_unif_of(blk);
 #pragma unroll
     for (int q = 0; q < K; ++q) {
       const double e = group_read<K>(e_own, q);
       const double u = group_read<K>(u_own, q);
-      if (t < run.tmax) {
+      if (t < run.tmax && !stop) {
         const double tot = net.props(x, cum);
         if (!(0.0 < tot)) {
           t = __builtin_inf();
@@ -552,7 +600,13 @@ __device__ __forceinline__ LaneTotals lane_events_group(Net &net, const rebop_de
           draw += 1;
           t = t + e / tot;
           net.affect(x, net.choose(x, cum, tot * u));
-          if (!net.in_range(x)) acc.flags |= REBOP_FLAG_RANGE;
+          if (!net.in_range(x)) acc.flags |= Net::kRangeFlag;
+          if constexpr (Net::kCap) {
+            if (__builtin_expect(draw >= dlim, 0)) {
+              hit_cap(run, acc, lead);
+              stop = true;
+            }
+          }
         }
         emit();
       }

@@ -37,6 +37,8 @@
 /* error flags raised by kernels */
 #define REBOP_FLAG_RANGE 1u
 #define REBOP_FLAG_INTERNAL 2u /* a launch invariant broke (reported as a device error) */
+#define REBOP_FLAG_CAP 4u      /* a trajectory reached the draw cap (watchdog) */
+#define REBOP_FLAG_NARROW 8u   /* a count left the int32 range of the narrow LDS kernel */
 
 /* counters[] slots */
 #define REBOP_CTR_EVENTS 0
@@ -45,6 +47,8 @@
 #define REBOP_CTR_FLAGS 3
 #define REBOP_CTR_QUEUE 4
 #define REBOP_CTR_SPILL 5 /* trajectories handed from a persistent launch to its tail */
+#define REBOP_CTR_CAPPED 6 /* trajectories stopped by the draw cap */
+#define REBOP_CTR_JOBQ 7   /* trajectory queue of a single-job LDS launch */
 #define REBOP_CTR_COUNT 8
 
 typedef struct {
@@ -69,6 +73,11 @@ typedef struct {
   uint64_t nb_steps;
   int32_t mode;
   int32_t persistent;
+  /* Watchdog: a trajectory that has drawn draw_cap times in this launch (0 =
+   * no cap) stops there as if its state were absorbing (no further draws;
+   * remaining samples repeat its counts), raises REBOP_FLAG_CAP and is
+   * counted in counters[REBOP_CTR_CAPPED]. */
+  uint64_t draw_cap;
   const int64_t *x0;   /* [S] shared (x0_div == 0) or row j / x0_div of [*][S] */
   uint64_t x0_div;
   const double *rates; /* NULL, or row j / rates_div of [*][R]: per-trajectory rate constants */
@@ -79,6 +88,8 @@ typedef struct {
   double *st_t;        /* [n] */
   uint64_t *st_draw;   /* [n] */
   unsigned long long *counters; /* [REBOP_CTR_COUNT] */
+  unsigned long long *queue;    /* persistent launches: the next trajectory index to hand out
+                                   (&counters[REBOP_CTR_QUEUE], or an LDS job's own queue) */
   const double *grid;  /* [nb_steps + 1] sample times tmax * i / nb_steps ([0] = tmax
                           in target mode), written before the launch; the LDS
                           kernel computes them instead */
@@ -102,7 +113,7 @@ typedef struct {
   uint64_t *spill_draw; /* [cap] draws made */
   uint64_t *spill_mark; /* [cap] draw count at the last sample (range check) */
   double *spill_t;      /* [cap] */
-  uint32_t *spill_i;    /* [cap] next grid point */
+  uint32_t *spill_i;    /* [cap] next grid point; bit 31: stopped by the draw cap */
   double *spill_x;      /* [S][cap] counts */
 } rebop_dev_run;
 
@@ -113,7 +124,7 @@ typedef struct {
  */
 #define REBOP_JOB_LINEAR 1u /* cumulative sum may be non-monotone: full linear search */
 #define REBOP_JOB_SLOW 2u   /* some reaction's factors are not all in its record */
-#define REBOP_LDS_LIMIT 1073741824 /* |count| <= 2^30 in the int32 LDS kernel */
+#define REBOP_LDS_LIMIT 1073741824 /* |count| <= 2^30 in the narrow (int32) LDS kernel */
 
 /*
  * Reaction records of the LDS kernel, 4 words per reaction, copied into LDS
@@ -155,6 +166,7 @@ typedef struct {
   uint64_t nb_steps;
   int32_t mode;
   int32_t pad2;
+  uint64_t draw_cap; /* as rebop_dev_run.draw_cap */
   uint64_t x0_div;
   const int64_t *x0;
   int64_t *out;

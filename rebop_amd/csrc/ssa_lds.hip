@@ -1,8 +1,11 @@
 // ssa_lds.hip -- LDS-resident SSA kernel for large and heterogeneous networks.
 //
-// Counts live in LDS as int32, one column per lane (x[s] at lds[s * BL +
-// lane]), so every access -- uniform or lane-varying species index -- maps
-// lane l to bank l mod 32: conflict-free.  The network (a "job") is read at
+// Counts live in LDS, one column per lane (x[s] at lds[s * BL + lane]), so
+// every access -- uniform or lane-varying species index -- maps lane l to
+// the same banks whatever the species: conflict-free.  The narrow kernel
+// keeps them as int32 (|x| <= 2^30, half the LDS, twice the workgroups per
+// CU); the wide one as doubles holding exact integers (|x| <= 2^52, the
+// register kernels' range), for populations beyond int32.  The network (a "job") is read at
 // wave-uniform addresses through the scalar cache, except in the per-lane
 // block scan below.
 //
@@ -27,10 +30,18 @@
 // arithmetic of earlier ones); a missing factor multiplies by exactly 1.0.
 // Reactions whose factors or jump do not fit a record use the global lists.
 //
-// Work items are (job, up to 64 trajectories); a wave takes items from a
-// device-wide queue, so one launch can run thousands of different networks
-// (heterogeneous batch) with the items ordered largest-first by the host.
+// Work items are tickets of a job (ceil(n_traj / 64) per job); a wave takes
+// items from a device-wide queue, so one launch can run thousands of
+// different networks (heterogeneous batch) with the items ordered
+// largest-first by the host.  A wave holding a ticket takes 64 trajectory
+// indices from its job's own queue, and each lane whose trajectory ends takes
+// the job's next index (one atomic per wave per refill), until the job's
+// queue is empty: a lane waits for the wave's slowest trajectory only at the
+// end of a job, not at the end of every group of 64.  Every trajectory is
+// claimed exactly once, so the results do not depend on which wave runs it.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "rebop_rng.h"
 #include "ssa_lane.h"
@@ -40,9 +51,14 @@
 
 namespace rebop {
 
-template <int MB>
+template <int MB, bool WIDE>
 struct LdsNet {
+  using CountT = typename std::conditional<WIDE, double, int32_t>::type;
   static constexpr int MS = 1;  // counts are in LDS, not in x[]
+  static constexpr bool kCap = true; // the check costs nothing next to a pass over R reactions
+  // the narrow kernel's range is the smaller one: the host reruns such a run wide
+  static constexpr uint32_t kRangeFlag = WIDE ? REBOP_FLAG_RANGE : REBOP_FLAG_NARROW;
+  static constexpr double kLimit = WIDE ? REBOP_EXACT_LIMIT : (double)REBOP_LDS_LIMIT;
   static constexpr int MR = MB; // cum[] holds the block checkpoints
   // job fields (wave-uniform)
   uint32_t nS, R, B, nblk;
@@ -56,7 +72,7 @@ struct LdsNet {
   const uint32_t *js;
   const int32_t *jd;
   // this lane's counts
-  int32_t *xl;
+  CountT *xl;
   uint32_t BL;
   // the job's reaction records in LDS, jump records in global memory
   const uint32_t *rec;
@@ -65,7 +81,7 @@ struct LdsNet {
   bool linear;
   bool bad;
 
-  __device__ __forceinline__ int32_t &cnt(uint32_t s) const { return xl[(uint64_t)s * BL]; }
+  __device__ __forceinline__ CountT &cnt(uint32_t s) const { return xl[(uint64_t)s * BL]; }
   __device__ __forceinline__ double fcnt(uint32_t s) const { return (double)xl[(uint64_t)s * BL]; }
 
   __device__ __forceinline__ uint32_t S() const { return nS; }
@@ -76,11 +92,11 @@ struct LdsNet {
   }
 
   __device__ __forceinline__ void set_count(uint32_t s, double v) {
-    if (!(__builtin_fabs(v) <= (double)REBOP_LDS_LIMIT)) {
+    if (!(__builtin_fabs(v) <= kLimit)) {
       bad = true;
       v = 0.0;
     }
-    cnt(s) = (int32_t)v;
+    cnt(s) = (CountT)v;
     if (v < 0.0) linear = true;
   }
   __device__ __forceinline__ void load_x0(double (&)[MS], const int64_t *src) {
@@ -96,7 +112,7 @@ struct LdsNet {
   __device__ __forceinline__ void store_state(const double (&)[MS], double *st, uint64_t stride) const {
     for (uint32_t s = 0; s < nS; ++s) st[(uint64_t)s * stride] = fcnt(s);
   }
-  // int32 counts: written as int64 directly (no conversion pass)
+  // integer counts: written as int64 directly (no conversion pass)
   __device__ __forceinline__ void write_raw(const double (&x)[MS], int64_t *dst, uint64_t stride) const {
     write_counts(x, dst, stride);
   }
@@ -114,14 +130,19 @@ struct LdsNet {
   }
 
   // factor (x[s] - c) of a record word, or exactly 1.0 when absent.  The
-  // difference is taken in int32 (|x| <= 2^30, c < 4096) and converted once:
-  // the integer Rate::rate converts, so the value is the same
-  __device__ __forceinline__ int32_t factor_int(uint32_t w, int32_t x) const {
-    const int32_t v = x - (int32_t)((w >> 16) & 0xfffu);
-    return (w & REBOP_REC_PRESENT) ? v : 1;
+  // narrow kernel takes the difference in int32 (|x| <= 2^30, c < 4096) and
+  // converts once; the wide one subtracts exact integers in f64 (|x| <= 2^52):
+  // either way the value of the reference's integer difference converted
+  __device__ __forceinline__ double factor_val(uint32_t w, CountT x) const {
+    if constexpr (WIDE) {
+      return (w & REBOP_REC_PRESENT) ? x - (double)((w >> 16) & 0xfffu) : 1.0;
+    } else {
+      const int32_t v = x - (int32_t)((w >> 16) & 0xfffu);
+      return (double)((w & REBOP_REC_PRESENT) ? v : 1);
+    }
   }
   __device__ __forceinline__ double factor(uint32_t w) const {
-    return (double)factor_int(w, cnt(w & 0xffffu));
+    return factor_val(w, cnt(w & 0xffffu));
   }
   __device__ __forceinline__ uint4 record(uint32_t r) const {
     return *(const uint4 *)(rec + (size_t)r * REBOP_REC_WORDS);
@@ -147,10 +168,10 @@ struct LdsNet {
   }
 
   // k * f1 * f2 of one record whose factor counts are already read
-  __device__ __forceinline__ double rate_of(const uint4 &q, int32_t x1, int32_t x2) const {
+  __device__ __forceinline__ double rate_of(const uint4 &q, CountT x1, CountT x2) const {
     double a = rebop_f64_from_bits(((uint64_t)q.y << 32) | q.x);
-    a = a * (double)factor_int(q.z, x1);
-    a = a * (double)factor_int(q.w, x2);
+    a = a * factor_val(q.z, x1);
+    a = a * factor_val(q.w, x2);
     return a;
   }
 
@@ -179,10 +200,10 @@ struct LdsNet {
             q1 = record(n1);
             q2 = record(n2);
             q3 = record(n3);
-            const int32_t x0a = cnt(c0.z & 0xffffu), x0b = cnt(c0.w & 0xffffu);
-            const int32_t x1a = cnt(c1.z & 0xffffu), x1b = cnt(c1.w & 0xffffu);
-            const int32_t x2a = cnt(c2.z & 0xffffu), x2b = cnt(c2.w & 0xffffu);
-            const int32_t x3a = cnt(c3.z & 0xffffu), x3b = cnt(c3.w & 0xffffu);
+            const CountT x0a = cnt(c0.z & 0xffffu), x0b = cnt(c0.w & 0xffffu);
+            const CountT x1a = cnt(c1.z & 0xffffu), x1b = cnt(c1.w & 0xffffu);
+            const CountT x2a = cnt(c2.z & 0xffffu), x2b = cnt(c2.w & 0xffffu);
+            const CountT x3a = cnt(c3.z & 0xffffu), x3b = cnt(c3.w & 0xffffu);
             // keep the next records' reads ahead of the wait for these counts
             __builtin_amdgcn_sched_barrier(0);
             const double a0 = rate_of(c0, x0a, x0b), a1 = rate_of(c1, x1a, x1b);
@@ -242,10 +263,17 @@ struct LdsNet {
   }
 
   __device__ __forceinline__ void add_count(uint32_t s, int32_t d) {
-    const int32_t v = (int32_t)((uint32_t)cnt(s) + (uint32_t)d);
-    cnt(s) = v;
-    if (v < 0) linear = true;
-    if (v > REBOP_LDS_LIMIT || v < -REBOP_LDS_LIMIT) bad = true;
+    if constexpr (WIDE) {
+      const double v = cnt(s) + (double)d; // exact: |x| <= 2^52, |d| < 2^23
+      cnt(s) = v;
+      if (v < 0.0) linear = true;
+      if (!(__builtin_fabs(v) <= kLimit)) bad = true;
+    } else {
+      const int32_t v = (int32_t)((uint32_t)cnt(s) + (uint32_t)d);
+      cnt(s) = v;
+      if (v < 0) linear = true;
+      if (v > REBOP_LDS_LIMIT || v < -REBOP_LDS_LIMIT) bad = true;
+    }
   }
 
   // Jump::affect (src/gillespie.rs:142-152), sparse form
@@ -265,16 +293,17 @@ struct LdsNet {
   }
 };
 
-template <int MB>
+template <int MB, bool WIDE>
 __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__restrict__ jobs,
                                                       const rebop_dev_item *__restrict__ items,
                                                       uint32_t n_items,
                                                       unsigned long long *counters,
                                                       unsigned long long *job_events,
+                                                      unsigned long long *job_queue,
                                                       uint32_t rec_word_off) {
   // no static LDS: the dynamic size (counts + records) alone decides how
   // many workgroups share a CU (e.g. 4 x 40 KB for 128 species, 512 reactions)
-  extern __shared__ int32_t lds_x[]; // [S_max][blockDim.x], then the records
+  extern __shared__ int32_t lds_x[]; // counts [S_max][blockDim.x], then the records
 
   const uint32_t lane = __lane_id();
   LaneTotals total;
@@ -288,6 +317,11 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
     if (it >= n_items) break;
     const rebop_dev_item item = items[it];
     const rebop_dev_job &job = jobs[item.job];
+    // the wave's first 64 trajectories of the job; none left: next ticket
+    unsigned long long base = 0ull;
+    if (lane == 0) base = atomicAdd(&job_queue[item.job], 64ull);
+    base = __shfl(base, 0);
+    if (base >= job.n_traj) continue;
     // the job's reaction records into LDS (every item: a wave's consecutive
     // items may belong to different jobs)
     uint32_t *lrec = (uint32_t *)lds_x + rec_word_off;
@@ -296,7 +330,7 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
       *(uint4 *)(lrec + w) = *(const uint4 *)(job.rec + w);
     __syncthreads();
 
-    LdsNet<MB> net;
+    LdsNet<MB, WIDE> net;
     net.nS = job.S;
     net.R = job.R;
     net.B = job.block;
@@ -310,7 +344,7 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
     net.jb = job.jump_begin;
     net.js = job.jump_s;
     net.jd = job.jump_d;
-    net.xl = lds_x + threadIdx.x;
+    net.xl = reinterpret_cast<typename LdsNet<MB, WIDE>::CountT *>(lds_x) + threadIdx.x;
     net.BL = blockDim.x;
     net.rec = lrec;
     net.jrec = job.jrec;
@@ -324,7 +358,8 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
     run.tmax = job.tmax;
     run.nb_steps = job.nb_steps;
     run.mode = job.mode;
-    run.persistent = 0;
+    run.persistent = 1; // lanes refill from the job's queue
+    run.draw_cap = job.draw_cap;
     run.x0 = job.x0;
     run.x0_div = job.x0_div;
     run.out = job.out;
@@ -333,12 +368,13 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
     run.st_t = job.st_t;
     run.st_draw = job.st_draw;
     run.counters = counters;
+    run.queue = &job_queue[item.job];
     run.n_records = job.n_records;
     run.rec_offsets = job.rec_offsets;
     run.rec_times = job.rec_times;
     run.rec_states = job.rec_states;
 
-    const uint64_t first = lane < item.count ? item.first + lane : job.n_traj;
+    const uint64_t first = base + lane < job.n_traj ? base + lane : job.n_traj;
     const LaneTotals acc = run.mode == REBOP_MODE_EVENTS ? lane_events(net, run, first, 0ull)
                                                          : lane_loop(net, run, first, 0ull);
     if (job_events) {
@@ -363,21 +399,31 @@ constexpr int kLdsCheckpoints = 32;
 
 int lds_checkpoints() { return kLdsCheckpoints; }
 
-hipError_t lds_occupancy(int threads, size_t lds, int *blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      blocks_per_cu, (const void *)&ssa_lds_kernel<kLdsCheckpoints>, threads, lds);
+static const void *lds_kernel_ptr(bool wide) {
+  return wide ? (const void *)&ssa_lds_kernel<kLdsCheckpoints, true>
+              : (const void *)&ssa_lds_kernel<kLdsCheckpoints, false>;
+}
+
+hipError_t lds_occupancy(int threads, size_t lds, bool wide, int *blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, lds_kernel_ptr(wide), threads,
+                                                      lds);
 }
 
 hipError_t launch_lds(const rebop_dev_job *jobs, const rebop_dev_item *items, uint32_t n_items,
-                      unsigned long long *counters, unsigned long long *job_events, int blocks,
-                      int threads, size_t lds, uint32_t rec_word_off, hipStream_t stream) {
+                      unsigned long long *counters, unsigned long long *job_events,
+                      unsigned long long *job_queue, int blocks, int threads, size_t lds,
+                      uint32_t rec_word_off, bool wide, hipStream_t stream) {
   if (lds > 64 * 1024) {
-    const hipError_t e = hipFuncSetAttribute((const void *)&ssa_lds_kernel<kLdsCheckpoints>,
+    const hipError_t e = hipFuncSetAttribute(lds_kernel_ptr(wide),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((ssa_lds_kernel<kLdsCheckpoints>), dim3(blocks), dim3(threads), lds, stream,
-                     jobs, items, n_items, counters, job_events, rec_word_off);
+  if (wide)
+    hipLaunchKernelGGL((ssa_lds_kernel<kLdsCheckpoints, true>), dim3(blocks), dim3(threads), lds,
+                       stream, jobs, items, n_items, counters, job_events, job_queue, rec_word_off);
+  else
+    hipLaunchKernelGGL((ssa_lds_kernel<kLdsCheckpoints, false>), dim3(blocks), dim3(threads), lds,
+                       stream, jobs, items, n_items, counters, job_events, job_queue, rec_word_off);
   return hipGetLastError();
 }
 

@@ -186,6 +186,13 @@ class GillespieNet:
         """``Gillespie::seed`` -- src/gillespie.rs:189-191."""
         check(self._lib.rebop_gillespie_seed(self._h, int(seed) & (2**64 - 1)))
 
+    def set_draw_cap(self, max_draws: int) -> None:
+        """Watchdog (no counterpart in the reference): later runs stop a
+        trajectory after ``max_draws`` random draws in one call and raise
+        ``RebopCapError``; 0 removes the cap (runs to a horizon that is not
+        finite then use 2^20)."""
+        check(self._lib.rebop_gillespie_set_draw_cap(self._h, int(max_draws)))
+
     def nb_species(self) -> int:
         return int(self._lib.rebop_gillespie_nb_species(self._h))
 

@@ -8,11 +8,13 @@ launch over every parameter point) by the project's own device kernels
 (include/rebop_surrogate.h): BF16 matrix-core GEMMs with FP32 accumulation
 for the dense layers, an FP64 mixture-likelihood head, Adam.
 
-The SSA stays the reference and the fallback: ``TrustMap`` validates the
-surrogate against fresh SSA histograms (10^4 trajectories per point) on a
-grid of parameter cells; a query is answered by the surrogate only for the
-(cell, sample time, species) where every validated Wasserstein-1 distance
-was <= 2 % of the species mean, and by the SSA elsewhere.
+The SSA stays the reference and the fallback: ``validate`` compares the
+surrogate with fresh SSA histograms (10^4 trajectories per point) at every
+sample time; ``TrustMap`` admits a (parameter cell, sample time, species)
+only where the cell held enough validated points and every one had a
+Wasserstein-1 distance <= 2 % of the species mean; ``route`` sends each
+query to the surrogate only where the map admits all its species, and
+``answer`` runs the SSA for the rest.
 """
 
 from __future__ import annotations
@@ -489,16 +491,58 @@ def w1_to_samples(params: np.ndarray, samples: np.ndarray, bound: float | None =
     return float(np.abs(F - emp).sum())
 
 
+def mixture_cdf_batch(params: np.ndarray, xmax: int, bounds: np.ndarray | None) -> np.ndarray:
+    """mixture_cdf for many queries at once: params [P][3][K], bounds [P]
+    (or None) -> CDF [P][xmax + 1]."""
+    from scipy import stats
+
+    logits, lmu, lr = params[:, 0], params[:, 1], params[:, 2]
+    w = np.exp(logits - logits.max(axis=1, keepdims=True))
+    w /= w.sum(axis=1, keepdims=True)
+    xs = np.arange(xmax + 1)[None, None, :]
+    nbk = K // 2 if bounds is not None else K
+    mu = np.exp(np.clip(lmu[:, :nbk], -8, 9))[..., None]
+    r = np.exp(np.clip(lr[:, :nbk], -6, 12))[..., None]
+    pmf = (w[:, :nbk, None] * stats.nbinom.pmf(xs, r, r / (r + mu))).sum(axis=1)
+    if bounds is not None:
+        pb = 1.0 / (1.0 + np.exp(-np.clip(lmu[:, nbk:], -30, 30)))[..., None]
+        n = np.round(bounds).astype(np.int64)[:, None, None]
+        pmf += (w[:, nbk:, None] * stats.binom.pmf(xs, n, pb)).sum(axis=1)
+    return np.minimum(np.cumsum(pmf, axis=1), 1.0)
+
+
+def w1_batch(params: np.ndarray, samples: np.ndarray, bounds: np.ndarray | None) -> np.ndarray:
+    """w1_to_samples for P queries: params [P][3][K], integer samples [P][n],
+    bounds [P] (or None) -> W1 [P]."""
+    P, n = samples.shape
+    xmax = int(max(samples.max(), 1) * 2 + 50)
+    if bounds is not None:
+        xmax = max(xmax, int(bounds.max()) + 1)
+    out = np.zeros(P)
+    for a in range(0, P, 128):  # bounded memory: 128 x K x (xmax + 1) doubles
+        b = min(P, a + 128)
+        F = mixture_cdf_batch(params[a:b], xmax, None if bounds is None else bounds[a:b])
+        for q in range(b - a):
+            cnt = np.bincount(np.clip(samples[a + q], 0, xmax), minlength=xmax + 1)
+            out[a + q] = np.abs(F[q] - np.cumsum(cnt) / n).sum()
+    return out
+
+
+SAMPLE_TIMES = tuple(range(1, NB + 1))  # every sample index a query can ask for
+
+
 @dataclass
 class TrustMap:
     """Where the surrogate may answer: per cell of (log kf, log kb, log kcat,
-    E0, S0), validated sample time and species, whether every validated point
-    of the cell passed there."""
+    E0, S0), sample time and species, whether the cell held at least
+    ``min_points`` validated points and every one of them passed there (W1 <=
+    ``tol`` of the species mean).  A sample time that was not validated is
+    never trusted; the SSA answers everything that is not."""
 
     bins: int
     times: tuple  # the validated sample indices
     passed: np.ndarray  # bool [bins]^3 x 2 x 2 x len(times) x S
-    seen: np.ndarray  # bool [bins]^3 x 2 x 2: cells holding a validated point
+    seen: np.ndarray  # int [bins]^3 x 2 x 2: validated points per cell
 
     @staticmethod
     def cell(rates: np.ndarray, x0: np.ndarray, bins: int) -> tuple:
@@ -507,76 +551,136 @@ class TrustMap:
         s = (x0[:, 1] > 525).astype(int)
         return lk[:, 0], lk[:, 1], lk[:, 2], e, s
 
-    def trusted(self, rates: np.ndarray, x0: np.ndarray, sample: int, species: int) -> np.ndarray:
-        """Per query point: may the surrogate answer for this species at the
-        validated sample time nearest to `sample` (at or after it)?"""
-        ti = min(int(np.searchsorted(np.asarray(self.times), sample)), len(self.times) - 1)
-        return self.passed[self.cell(rates, x0, self.bins) + (ti, species)]
+    @staticmethod
+    def build(rel: np.ndarray, rates: np.ndarray, x0: np.ndarray, times, bins: int, tol: float,
+              min_points: int) -> "TrustMap":
+        """From validated relative W1 [P][len(times)][S] of points (rates, x0)."""
+        ok = rel <= tol
+        passed = np.ones((bins,) * 3 + (2, 2, len(times), S), dtype=bool)
+        seen = np.zeros((bins,) * 3 + (2, 2), dtype=np.int64)
+        cells = TrustMap.cell(rates, x0, bins)
+        for p in range(len(rates)):
+            c = tuple(ci[p] for ci in cells)
+            seen[c] += 1
+            passed[c] &= ok[p]
+        passed &= (seen >= min_points)[..., None, None]
+        return TrustMap(bins, tuple(int(t) for t in times), passed, seen)
+
+    def trusted(self, rates: np.ndarray, x0: np.ndarray, sample, species) -> np.ndarray:
+        """Per query (rows of rates/x0; sample indices 1..32 and species,
+        scalars or per-query arrays): may the surrogate answer?"""
+        sample = np.broadcast_to(np.asarray(sample, dtype=np.int64), (len(rates),))
+        species = np.broadcast_to(np.asarray(species, dtype=np.int64), (len(rates),))
+        lut = np.full(NB + 2, -1, dtype=np.int64)  # sample index -> validated position
+        lut[np.asarray(self.times, dtype=np.int64)] = np.arange(len(self.times))
+        ti = lut[np.clip(sample, 0, NB + 1)]
+        out = np.zeros(len(rates), dtype=bool)
+        v = ti >= 0
+        if v.any():
+            c = tuple(ci[v] for ci in self.cell(rates[v], x0[v], self.bins))
+            out[v] = self.passed[c + (ti[v], species[v])]
+        return out
 
 
 def validate(mlp: MLP, rates: np.ndarray, x0: np.ndarray, n_ref: int, seed: int, tol: float = 0.02,
-             bins: int = 2, times=(4, 8, 16, 32)):
-    """W1 of the surrogate against fresh n_ref-trajectory SSA histograms.
+             bins: int = 2, times=SAMPLE_TIMES, min_points: int = 1, trust_tol: float | None = None,
+             traj_begin: int = 1 << 40):
+    """W1 of the surrogate against fresh n_ref-trajectory SSA histograms at
+    every sample time in ``times``.
 
-    Returns (relative W1 [P][len(times)][S], TrustMap): (cell, time,
-    species) is trusted if every validated point of the cell has W1 <= tol *
-    mean there.
+    Returns (relative W1 [P][len(times)][S], TrustMap, SSA stats): (cell,
+    time, species) is trusted if the cell holds at least ``min_points``
+    validated points and each has W1 <= trust_tol (default tol) * mean there.
     """
-    data = generate(rates, x0, n_ref, seed=seed, traj_begin=1 << 40)
+    data = generate(rates, x0, n_ref, seed=seed, traj_begin=traj_begin)
     samp = data.samples.cpu().numpy()
     P = len(rates)
     rel = np.zeros((P, len(times), S))
     for ti, i in enumerate(times):
         params, bounds = predict(mlp, rates, x0, np.full(P, i))
-        for p in range(P):
-            for s in range(S):
-                xs = samp[i, s, p * n_ref:(p + 1) * n_ref]
-                w1 = w1_to_samples(params[p, s], xs, bounds[p, s])
-                mean = xs.mean()
-                rel[p, ti, s] = w1 / mean if mean > 0 else (0.0 if w1 <= tol else np.inf)
-    ok = rel <= tol  # [P][T][S]
-    passed = np.ones((bins,) * 3 + (2, 2, len(times), S), dtype=bool)
-    seen = np.zeros((bins,) * 3 + (2, 2), dtype=bool)
-    cells = TrustMap.cell(rates, x0, bins)
-    for p in range(P):
-        c = tuple(ci[p] for ci in cells)
-        seen[c] = True
-        passed[c] &= ok[p]
-    passed &= seen[..., None, None]
-    return rel, TrustMap(bins, tuple(times), passed, seen), data.stats
+        for s in range(S):
+            xs = samp[i, s].reshape(P, n_ref)
+            w1 = w1_batch(params[:, s], xs, bounds[:, s])
+            mean = xs.mean(axis=1)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                rel[:, ti, s] = np.where(mean > 0, w1 / np.where(mean > 0, mean, 1.0),
+                                         np.where(w1 <= tol, 0.0, np.inf))
+    trust = TrustMap.build(rel, rates, x0, times, bins, tol if trust_tol is None else trust_tol,
+                           min_points)
+    return rel, trust, data.stats
 
 
 def ssa_noise_floor(rates: np.ndarray, x0: np.ndarray, n_ref: int, seed: int,
-                    times=(4, 8, 16, 32)) -> np.ndarray:
-    """Relative W1 between validate's reference histograms and a second,
-    independent n_ref-trajectory SSA run of the same points [P][times][S]:
-    the spread the 2 % gate has to sit above.  Where a species' mean is near
-    zero, 2 % of the mean is a small fraction of one molecule and two SSA runs
-    already differ by more."""
-    a = generate(rates, x0, n_ref, seed=seed, traj_begin=1 << 40).samples.cpu().numpy()
-    b = generate(rates, x0, n_ref, seed=seed + 7, traj_begin=1 << 41).samples.cpu().numpy()
+                    times=SAMPLE_TIMES, traj_begin: int = 1 << 40) -> np.ndarray:
+    """Relative W1 between validate's reference histograms (same seed and
+    traj_begin) and a second, independent n_ref-trajectory SSA run of the
+    same points [P][times][S]: the spread the 2 % gate has to sit above.
+    Where a species' mean is near zero, 2 % of the mean is a small fraction
+    of one molecule and two SSA runs already differ by more."""
+    a = generate(rates, x0, n_ref, seed=seed, traj_begin=traj_begin).samples.cpu().numpy()
+    b = generate(rates, x0, n_ref, seed=seed + 7, traj_begin=traj_begin + (1 << 39)).samples.cpu().numpy()
     P = len(rates)
     out = np.zeros((P, len(times), S))
     for ti, i in enumerate(times):
-        for p in range(P):
-            for s in range(S):
-                xa = np.sort(a[i, s, p * n_ref:(p + 1) * n_ref])
-                xb = np.sort(b[i, s, p * n_ref:(p + 1) * n_ref])
-                m = xa.mean()
-                w1 = float(np.abs(xa - xb).mean())  # equal sizes: W1 of the two empirical laws
-                out[p, ti, s] = w1 / m if m > 0 else (0.0 if w1 == 0 else np.inf)
+        for s in range(S):
+            xa = np.sort(a[i, s].reshape(P, n_ref), axis=1)
+            xb = np.sort(b[i, s].reshape(P, n_ref), axis=1)
+            m = xa.mean(axis=1)
+            w1 = np.abs(xa - xb).mean(axis=1)  # equal sizes: W1 of the two empirical laws
+            with np.errstate(divide="ignore", invalid="ignore"):
+                out[:, ti, s] = np.where(m > 0, w1 / np.where(m > 0, m, 1.0),
+                                         np.where(w1 == 0, 0.0, np.inf))
     return out
 
 
-def benchmark(n_train: int = 100_000, steps: int = 12_000, batch: int = 8192, n_queries: int = 1_000_000,
-              n_valid: int = 128, n_ref: int = 10_000, seed: int = 2026, lr: float = 1e-3,
-              log=print) -> dict:
-    """End-to-end: SSA training data -> training -> W1 validation -> query timing.
+def ssa_ms_per_point(stats: _native.Stats, n_points: int) -> float:
+    """Device time of one parameter point's SSA (all its sample times) in a
+    sweep launch."""
+    return stats.kernel_ms / max(n_points, 1)
 
-    Returns a dict with the surrogate's and the SSA's queries/s (a query is
-    the distribution of every species at one (parameter point, sample time));
-    the SSA's cost per query is that of n_ref trajectories of the point,
-    shared by the point's 32 sample times.
+
+def route(trust: TrustMap, rates: np.ndarray, x0: np.ndarray, sample_idx: np.ndarray) -> np.ndarray:
+    """Which (point, sample time) queries the surrogate answers: those whose
+    every species is trusted there; the SSA answers the others.  bool [Q]."""
+    ok = np.ones(len(rates), dtype=bool)
+    for s in range(S):
+        ok &= trust.trusted(rates, x0, sample_idx, s)
+    return ok
+
+
+def answer(mlp: MLP, trust: TrustMap, rates: np.ndarray, x0: np.ndarray, sample_idx: np.ndarray,
+           n_ref: int = 10_000, seed: int = 0, traj_begin: int = 1 << 42):
+    """A sweep query answered the gated way: mixture parameters from the
+    surrogate where the trust map admits the query, and for every other query
+    the n_ref-trajectory SSA histogram of its point (one sweep launch over
+    those points).  Returns (surrogate mask [Q], params [Q][S][3][K] (valid
+    where the mask is set), bounds [Q][S], {query index: samples [S][n_ref]})."""
+    use = route(trust, rates, x0, sample_idx)
+    params, bounds = predict(mlp, rates, x0, sample_idx)
+    ssa = {}
+    rest = np.flatnonzero(~use)
+    if len(rest):
+        data = generate(rates[rest], x0[rest], n_ref, seed=seed, traj_begin=traj_begin)
+        samp = data.samples.cpu().numpy()
+        for k, q in enumerate(rest):
+            ssa[int(q)] = samp[int(sample_idx[q]), :, k * n_ref:(k + 1) * n_ref]
+    return use, params, bounds, ssa
+
+
+def benchmark(n_train: int = 100_000, steps: int = 12_000, batch: int = 8192, n_queries: int = 1_000_000,
+              n_valid: int = 256, n_holdout: int = 256, n_ref: int = 10_000, seed: int = 2026,
+              lr: float = 1e-3, bins: int = 2, min_points: int = 4, trust_tol: float = 0.02,
+              log=print) -> dict:
+    """End-to-end: SSA training data -> training -> W1 validation at every
+    sample time -> trust map -> check of the map on held-out points -> query
+    timing and the gated sweep's effective speedup.
+
+    A query is the distribution of every species at one (parameter point,
+    sample time).  The plain GPU SSA's cost per query is that of n_ref
+    trajectories of the point shared by its 32 sample times; the gated
+    surrogate answers the queries the trust map admits and sends the rest
+    to the SSA at that same cost per query, so its effective speedup is
+    SSA time of all queries / (surrogate time + SSA time of the rest).
     """
     import time
 
@@ -597,23 +701,34 @@ def benchmark(n_train: int = 100_000, steps: int = 12_000, batch: int = 8192, n_
     log(f"training: {steps} steps of {batch}, {train_s:.2f} s, NLL/sample {hist[0][1]:.3f} -> {hist[-1][1]:.3f}")
     del data
     torch.cuda.empty_cache()
-    # validation against fresh n_ref-trajectory histograms
+    # validation against fresh n_ref-trajectory histograms at all 32 times
     vr, vx = sample_points(n_valid, seed + 1)
     t0 = time.perf_counter()
-    rel, trust, vstats = validate(mlp, vr, vx, n_ref=n_ref, seed=seed + 2)
+    rel, trust, vstats = validate(mlp, vr, vx, n_ref=n_ref, seed=seed + 2, bins=bins,
+                                  min_points=min_points, trust_tol=trust_tol)
     val_s = time.perf_counter() - t0
-    ssa_ms = vstats.kernel_ms
-    ssa_queries = n_valid * NB
-    ssa_qps = ssa_queries / (ssa_ms / 1e3)
+    ssa_point_ms = ssa_ms_per_point(vstats, n_valid)
+    ssa_qps = NB / (ssa_point_ms / 1e3)
     pass_frac = float((rel <= 0.02).mean())
     point_pass = float((rel <= 0.02).all(axis=(1, 2)).mean())
     floor = ssa_noise_floor(vr, vx, n_ref, seed + 2)
-    repro = floor <= 0.02  # cells where two SSA runs agree to 2 % of the mean
+    repro = floor <= 0.02  # entries where two SSA runs agree to 2 % of the mean
     pass_repro = float((rel <= 0.02)[repro].mean()) if repro.any() else float("nan")
-    log(f"validation: {n_valid} points x {n_ref} trajectories ({ssa_ms:.1f} ms kernel); "
-        f"W1 <= 2% of mean for {pass_frac:.3f} of (point, time, species), {point_pass:.3f} of points; "
-        f"two SSA runs differ by more than 2% in {1 - repro.mean():.3f} of the cells, the surrogate "
-        f"passes {pass_repro:.3f} of the others")
+    log(f"validation: {n_valid} points x {n_ref} trajectories x {NB} times; W1 <= 2% of mean for "
+        f"{pass_frac:.3f} of (point, time, species), {point_pass:.3f} of points; two SSA runs differ "
+        f"by more than 2% in {1 - repro.mean():.3f} of the entries, the surrogate passes "
+        f"{pass_repro:.3f} of the others; {val_s:.1f} s")
+    # the trust map on held-out points and a fresh SSA
+    hr, hx = sample_points(n_holdout, seed + 5)
+    rel_h, _, _ = validate(mlp, hr, hx, n_ref=n_ref, seed=seed + 6, bins=bins, traj_begin=1 << 41)
+    adm = np.stack([np.stack([trust.trusted(hr, hx, t, s_) for s_ in range(S)], axis=1)
+                    for t in SAMPLE_TIMES], axis=1)  # [P][T][S]
+    admitted = float(adm.mean())
+    admitted_pass = float((rel_h[adm] <= 0.02).mean()) if adm.any() else float("nan")
+    q_adm = adm.all(axis=2)
+    log(f"held-out: {n_holdout} points; the trust map admits {admitted:.3f} of (point, time, "
+        f"species) and {q_adm.mean():.3f} of (point, time) queries; W1 <= 2% holds for "
+        f"{admitted_pass:.3f} of the admitted entries")
     # query timing (device-resident inputs, like the SSA's)
     qr, qx = sample_points(n_queries, seed + 3)
     d_r = torch.as_tensor(qr, device="cuda")
@@ -637,12 +752,27 @@ def benchmark(n_train: int = 100_000, steps: int = 12_000, batch: int = 8192, n_
     sg_ms = float(np.median(times))
     sg_qps = n_queries / (sg_ms / 1e3)
     flops = 2.0 * n_queries * sum(mlp.dims[l] * mlp.dims[l + 1] for l in range(mlp.depth))
+    # the gated sweep: routing of the same queries, SSA for the rest
+    t0 = time.perf_counter()
+    use = route(trust, qr, qx, qs)
+    route_s = time.perf_counter() - t0
+    q_trusted = float(use.mean())
+    ssa_all_ms = n_queries * ssa_point_ms / NB
+    ssa_rest_ms = float((~use).sum()) * ssa_point_ms / NB
+    effective = ssa_all_ms / (sg_ms + ssa_rest_ms)
     log(f"surrogate: {n_queries} queries in {sg_ms:.2f} ms ({sg_qps:.3e} q/s, {flops / sg_ms / 1e9:.1f} TFLOP/s); "
-        f"SSA: {ssa_qps:.3e} q/s; speedup {sg_qps / ssa_qps:.1f}x")
+        f"SSA: {ssa_qps:.3e} q/s; raw speedup {sg_qps / ssa_qps:.1f}x; the trust map admits "
+        f"{q_trusted:.3f} of the queries, effective speedup of the gated sweep {effective:.2f}x")
     return dict(surrogate_qps=sg_qps, ssa_qps=ssa_qps, speedup=sg_qps / ssa_qps, surrogate_ms=sg_ms,
-                ssa_ms=ssa_ms, w1_pass_fraction=pass_frac, point_pass_fraction=point_pass,
-                w1_pass_fraction_reproducible=pass_repro, ssa_self_w1_over_2pct=float(1 - repro.mean()),
+                ssa_ms=vstats.kernel_ms, ssa_point_ms=ssa_point_ms, w1_pass_fraction=pass_frac,
+                point_pass_fraction=point_pass, w1_pass_fraction_reproducible=pass_repro,
+                ssa_self_w1_over_2pct=float(1 - repro.mean()),
                 trusted_cells=float(trust.passed.mean()),
-                trusted_validated_cells=float(trust.passed[trust.seen].mean()), train_s=train_s, gen_s=gen_s, val_s=val_s,
-                nll_first=hist[0][1], nll_last=hist[-1][1], gemm_tflops=flops / sg_ms / 1e9,
-                train_events=int(vstats.events), rel_w1_median=float(np.median(rel)))
+                trusted_validated_cells=float(trust.passed[trust.seen >= min_points].mean())
+                if (trust.seen >= min_points).any() else 0.0,
+                holdout_admitted_entries=admitted, holdout_admitted_queries=float(q_adm.mean()),
+                holdout_admitted_pass_fraction=admitted_pass, trusted_query_fraction=q_trusted,
+                effective_speedup=effective, route_s=route_s, train_s=train_s, gen_s=gen_s,
+                val_s=val_s, nll_first=hist[0][1], nll_last=hist[-1][1],
+                gemm_tflops=flops / sg_ms / 1e9, train_events=int(vstats.events),
+                rel_w1_median=float(np.median(rel)))

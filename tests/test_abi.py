@@ -82,6 +82,17 @@ def test_issue85_oob_rejected_at_the_boundary():
     assert g.nb_reactions() == 0
 
 
+def test_draw_cap_setter_at_the_boundary():
+    """The watchdog setter validates its handle and needs no device."""
+    lib = _native.load()
+    assert lib.rebop_gillespie_set_draw_cap(None, 5) == _native.REBOP_ERR_VALUE
+    g = GillespieNet([1], seed=0)
+    g.set_draw_cap(1000)
+    g.set_draw_cap(0)
+    st = _native.Stats()
+    assert hasattr(st, "capped") and ctypes.sizeof(st) == 7 * 8
+
+
 def test_shape_asserts_at_the_boundary():
     g = GillespieNet([1, 2, 3], seed=0)
     with pytest.raises(rebop_amd.RebopValueError) as e:

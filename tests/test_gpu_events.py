@@ -88,10 +88,13 @@ def test_all_events_absorbing_records_inf():
     assert t[0] == 0 and np.isinf(t[-1]) and x[-1, 0] == 0 and len(t) == 5
 
 
-def test_all_events_refuses_infinite_tmax():
+def test_all_events_infinite_tmax_without_absorbing_state_stops_at_the_cap():
+    """A pure inflow never absorbs: every event up to tmax = +inf would never
+    end (the reference's loop spins), so the default draw cap stops it and the
+    call reports RebopCapError."""
     g = GillespieNet([3], seed=2)
     g.add_reaction(Rate.lma(1.0, [0]), [1])
-    with pytest.raises(_native.RebopValueError):
+    with pytest.raises(_native.RebopCapError):
         g.run_all_events(float("inf"))
 
 

@@ -108,11 +108,77 @@ def test_lds_per_trajectory_x0_and_range_guard(nthreads):
     o = build_oracle(spec, sparse=True)
     _, oout, _, _, _ = o.run_grid(4, x0, 0.5, 8, n_traj=200, x0_per_traj=True, nthreads=nthreads)
     assert np.array_equal(out, oout)
-    # counts beyond the int32 LDS range are refused
-    big = GillespieNet([2**31, 0] + [0] * 38, seed=1)
+    # counts beyond the exactly simulated range (2^52) are refused
+    big = GillespieNet([2**53, 0] + [0] * 38, seed=1)
     big.add_reaction(Rate.lma(1.0, [1] + [0] * 39), [-1, 1] + [0] * 38)
     with pytest.raises(_native.RebopValueError):
         big.run(1.0, 2, n_traj=4)
+
+
+def _wide_spec(x0a):
+    """40 species (beyond the register kernels): a pool A with an inflow in
+    jumps of 2^20 that feeds B in jumps of 2^20, B decaying, plus a small
+    network on the other species."""
+    S = 40
+    rx = [(2.0, [0] * 40, [2**20] + [0] * 39),
+          (2e-9, [1] + [0] * 39, [-(2**20), 2**20] + [0] * 38),
+          (0.05, [0, 1] + [0] * 38, [0, -(2**10)] + [0] * 38)]
+    for i in range(2, S - 1):
+        o = [0] * S
+        o[i] = 1
+        d = [0] * S
+        d[i] = -1
+        d[i + 1] = 1
+        rx.append((0.3, o, d))
+    return S, rx, [x0a, 0] + [20] * 38
+
+
+@pytest.mark.parametrize("x0a", [2**31 + 5, 2**30 - 3 * 2**20])
+def test_lds_counts_beyond_int32(x0a, nthreads):
+    """Populations beyond int32 run on the LDS kernel with f64 counts: chosen
+    up front when the initial counts are beyond int32 (2^31 + 5), or by a rerun
+    when a narrow run's count leaves int32 during the run (A starts at
+    2^30 - 3 * 2^20 and its inflow carries it past 2^30).  Both equal the
+    oracle (int64)."""
+    spec = _wide_spec(x0a)
+    g = build_device(spec, 8)
+    out, st = _dev(g, 6.0, 12, 256, LDS)
+    assert st.kernel == LDS | _native.REBOP_KERNEL_WIDE
+    o = build_oracle(spec, sparse=True)
+    _, oout, _, oev, _ = o.run_grid(8, spec[2], 6.0, 12, n_traj=256, nthreads=nthreads)
+    assert np.array_equal(out, oout)
+    assert st.events == oev
+    assert np.abs(oout).max() > 2**30
+
+
+def test_lds_wide_per_trajectory_x0_and_jobs(nthreads):
+    """Device-side initial counts (not visible to the host) start narrow and
+    are rerun wide; so are fused job batches; the batch state runs wide."""
+    import torch
+
+    spec = _wide_spec(2**30 - 3 * 2**20)
+    x0 = np.tile(np.asarray(spec[2], dtype=np.int64), (128, 1))
+    g = build_device(spec, 9)
+    out, st = _dev(g, 6.0, 12, 128, LDS, x0=x0)
+    assert st.kernel == LDS | _native.REBOP_KERNEL_WIDE
+    o = build_oracle(spec, sparse=True)
+    _, oout, _, oev, _ = o.run_grid(9, x0, 6.0, 12, n_traj=128, x0_per_traj=True, nthreads=nthreads)
+    assert np.array_equal(out, oout) and st.events == oev
+    buf = torch.zeros((13, spec[0], 128), dtype=torch.int64, device="cuda")
+    st2, ev = run_jobs([dict(net=g, tmax=6.0, nb_steps=12, n_traj=128, out_ptr=buf.data_ptr())],
+                       per_job_events=True)
+    torch.cuda.synchronize()
+    assert st2.kernel == LDS | _native.REBOP_KERNEL_WIDE
+    assert np.array_equal(buf.cpu().numpy(), oout) and int(ev[0]) == oev
+    b = g.batch(32)
+    b.advance_until(3.0)
+    b.advance_until(6.0)
+    xs = b.get_species()
+    for j in (0, 31):
+        tr = o.state(spec[2], 9, traj=j)
+        tr.advance_until(3.0)
+        tr.advance_until(6.0)
+        assert xs[j].tolist() == tr.x.tolist()
 
 
 def test_fused_heterogeneous_jobs_bit_exact(nthreads):

@@ -382,3 +382,38 @@ def test_training_learns_and_beats_untrained(sg):
     rel_t, _, _ = sg.validate(mlp, vr, vx, n_ref=4000, seed=14, times=(8, 32))
     rel_u, _, _ = sg.validate(untrained, vr, vx, n_ref=4000, seed=14, times=(8, 32))
     assert np.median(rel_t) < 0.5 * np.median(rel_u), (np.median(rel_t), np.median(rel_u))
+
+
+def test_trust_map_holds_on_held_out_points(sg):
+    """The gate as the sweep uses it: validate a trained surrogate against
+    fresh 10^4-trajectory SSA histograms at every sample time, build the trust
+    map, then on held-out points and a fresh SSA require W1 <= 2 % of the
+    species mean for >= 95 % of the (point, time, species) entries the map
+    admits.  The map must admit a useful share; route() and answer() send the
+    rest to the SSA."""
+    P = 20000
+    rates, x0 = sg.sample_points(P, seed=21)
+    data = sg.generate(rates, x0, sg.J, seed=22)
+    mlp = sg.MLP(seed=3)
+    sg.train(mlp, data, steps=3000, batch=8192, lr=1e-3)
+    del data
+    vr, vx = sg.sample_points(256, seed=23)
+    rel, trust, _ = sg.validate(mlp, vr, vx, n_ref=10_000, seed=24, min_points=4)
+    hr, hx = sg.sample_points(256, seed=25)
+    rel_h, _, _ = sg.validate(mlp, hr, hx, n_ref=10_000, seed=26, traj_begin=1 << 41)
+    adm = np.stack([np.stack([trust.trusted(hr, hx, t, s) for s in range(sg.S)], axis=1)
+                    for t in sg.SAMPLE_TIMES], axis=1)
+    share = float(adm.mean())
+    ok = float((rel_h[adm] <= 0.02).mean()) if adm.any() else float("nan")
+    print(f"validated pass {float((rel <= 0.02).mean()):.3f}; held-out: admitted {share:.3f} of "
+          f"entries, W1 <= 2% on {ok:.4f} of them (vs {float((rel_h <= 0.02).mean()):.3f} overall)")
+    assert share >= 0.05, share
+    assert ok >= 0.95, ok
+    # routing: the admitted queries go to the surrogate, the rest to the SSA
+    samp = np.random.default_rng(27).integers(1, sg.NB + 1, 12)
+    use, params, bounds, ssa = sg.answer(mlp, trust, hr[:12], hx[:12], samp, n_ref=2000, seed=28)
+    assert use.tolist() == sg.route(trust, hr[:12], hx[:12], samp).tolist()
+    assert sorted(ssa) == np.flatnonzero(~use).tolist()
+    for q, xs in ssa.items():
+        assert xs.shape == (sg.S, 2000)
+        assert (xs[0] + xs[2] == hx[q, 0]).all()  # E + ES = E0 in every SSA sample

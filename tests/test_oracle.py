@@ -223,6 +223,50 @@ def test_draw_accounting(oracle):
     assert tr.draws == 1 and tr.t == 0.0  # advance_until(0.0) consumes one draw
 
 
+def test_draw_cap_closed_forms(oracle):
+    """The device's watchdog as the oracle states it (no reference
+    counterpart).  Pure birth A -> 2A from A = 1 over a horizon it never
+    reaches: in a grid run the first draw is the overshoot of sample 0 (t =
+    0), every later draw is an event, so a cap of c >= 2 draws stops at
+    A = c after c - 1 events (the cap is checked after events only, so c = 1
+    stops after the first event: A = 2); in the all-events loop every draw
+    is an event (A = c + 1, c + 1 records); advance_until with a cap set
+    midway counts from there.  Without a cap nothing changes."""
+    n = oracle.OracleNet(1)
+    n.add_lma(1.0, [1], [1])
+    for cap in (1, 2, 7, 100):
+        _, out, final, ev, dr = n.run_grid(5, [1], 1e9, 4, n_traj=3, draw_cap=cap)
+        a = max(cap, 2)
+        assert final[:, 0].tolist() == [a] * 3
+        assert ev == 3 * (a - 1) and dr == 3 * a
+        assert (out[0, 0] == 1).all() and (out[1:, 0] == a).all()
+        times, xs = n.run_all_events(5, 0, [1], float("inf"), draw_cap=cap)
+        assert len(times) == cap + 1 and xs[-1, 0] == cap + 1
+        assert np.isfinite(times).all() and (np.diff(times) > 0).all()
+    tr = n.state([1], 9)
+    tr.advance_until(0.5)
+    d0, a0 = tr.draws, int(tr.x[0])
+    tr.set_cap(10)
+    tr.advance_until(1e9)
+    assert tr.capped and tr.draws == d0 + 10 and tr.t == 1e9
+    assert int(tr.x[0]) == a0 + 10  # every capped draw was an event
+    # a cap that is never reached changes nothing, draw for draw
+    m = build_oracle(sir())
+    a = m.run_grid(3, [999, 1, 0], 250.0, 50, n_traj=20)
+    b = m.run_grid(3, [999, 1, 0], 250.0, 50, n_traj=20, draw_cap=10**6)
+    assert np.array_equal(a[1], b[1]) and a[3:] == b[3:]
+
+
+def test_all_events_infinite_horizon_absorbs(oracle):
+    """pyo3_gillespie.rs:215 with tmax = +inf ends when the state absorbs:
+    pure decay from A = 30 gives 30 events and a final t = +inf record."""
+    n = oracle.OracleNet(1)
+    n.add_lma(0.5, [1], [-1])
+    times, xs = n.run_all_events(2, 0, [30], float("inf"))
+    assert len(times) == 32 and xs[-1, 0] == 0 and xs[-2, 0] == 0 and np.isinf(times[-1])
+    assert np.isfinite(times[:-1]).all()
+
+
 def test_run_empty(oracle):
     """tests/test_rebop.py:148-152: no species, no reactions, times linspace(0, 10, 11)."""
     n = oracle.OracleNet(0)

@@ -159,3 +159,55 @@ def test_features_are_the_stated_groups():
     assert np.allclose(10 ** (4.05 * X[:, 15] - 0.75), kc * e0 * t / s0)
     assert np.all(X[:, 16:] == 0)
     assert np.array_equal(bnd, np.stack([e0, s0, e0, s0], axis=1))
+
+
+def test_w1_batch_equals_single_query_w1():
+    """The gate's vectorised W1 equals the one-query definition (mixture CDF
+    against the empirical CDF of integer samples), bounded and unbounded."""
+    from rebop_amd import surrogate as sg
+
+    rng = np.random.default_rng(5)
+    P = 40
+    params = np.zeros((P, 3, K))
+    params[:, 0] = rng.normal(size=(P, K))
+    params[:, 1] = rng.normal(np.log(30.0), 0.8, size=(P, K))
+    params[:, 2] = rng.normal(1.0, 1.0, size=(P, K))
+    samples = rng.poisson(rng.uniform(5, 60, size=(P, 1)), size=(P, 500))
+    # one common largest sample, so that both forms sum the CDFs over the same range
+    samples = np.minimum(samples, 80)
+    samples[:, 0] = 80
+    bounds = rng.integers(60, 120, size=P).astype(np.float64)
+    for b in (None, bounds):
+        got = sg.w1_batch(params, samples, b)
+        want = [sg.w1_to_samples(params[q], samples[q], None if b is None else b[q]) for q in range(P)]
+        assert np.allclose(got, want, rtol=1e-9, atol=1e-9)
+    # a mixture at 0 (means clipped to e^-8) against samples at 0: W1 = its
+    # mean; against samples at 1: W1 = 1 - its mean
+    pm = np.zeros((1, 3, K))
+    pm[0, 1] = -30.0
+    pm[0, 2] = 12.0  # dispersion large: Poisson-like
+    m = np.exp(-8.0)
+    assert abs(sg.w1_batch(pm, np.zeros((1, 10), dtype=np.int64), None)[0] - m) < 1e-6
+    assert abs(sg.w1_batch(pm, np.ones((1, 10), dtype=np.int64), None)[0] - (1.0 - m)) < 1e-6
+
+
+def test_trust_map_admits_only_validated_passing_cells():
+    """A (cell, time, species) is trusted only if the cell held min_points
+    validated points and all passed there; unvalidated sample times are never
+    trusted (the map does not extrapolate in time)."""
+    from rebop_amd import surrogate as sg
+
+    rates = np.array([[1e-2, 1e-2, 1e-2]] * 3 + [[5.0, 5.0, 5.0]] * 2)
+    x0 = np.array([[50, 300, 0, 0]] * 5)
+    times = (4, 8)
+    rel = np.full((5, 2, sg.S), 0.01)
+    rel[1, 1, 2] = 0.03  # one failing point in the first cell at t-index 8, species 2
+    tm = sg.TrustMap.build(rel, rates, x0, times, bins=2, tol=0.02, min_points=3)
+    q = np.array([[1e-2, 1e-2, 1e-2], [5.0, 5.0, 5.0]])
+    qx = np.array([[50, 300, 0, 0]] * 2)
+    assert tm.trusted(q, qx, 4, 0).tolist() == [True, False]  # the second cell has 2 < 3 points
+    assert tm.trusted(q, qx, 8, 2).tolist() == [False, False]
+    assert tm.trusted(q, qx, 8, 1).tolist() == [True, False]
+    assert tm.trusted(q, qx, 5, 0).tolist() == [False, False]  # time 5 was not validated
+    assert sg.route(tm, q, qx, np.array([4, 4])).tolist() == [True, False]
+    assert sg.route(tm, q, qx, np.array([8, 8])).tolist() == [False, False]

@@ -262,6 +262,27 @@ int rebop_run_device(const rebop_gillespie *g, double tmax, uint64_t nb_steps,
                      int64_t *d_final, int kernel, void *stream, rebop_stats *stats);
 
 /*
+ * rebop_run_device without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory -- code
+ * being captured into a HIP graph, tight serving loops, harnesses that forbid
+ * hipMalloc.  Same arguments, results and errors as rebop_run_device, except:
+ *   - the handle must be prepared on the current device first
+ *     (rebop_gillespie_prepare with the kernel kind to be run; AUTO prepares
+ *     what an AUTO run uses), else REBOP_ERR_VALUE;
+ *   - the launch counters, sample-time table and work tables live in one
+ *     fixed ~0.8 MB device region per GPU, which limits nb_steps to 65536
+ *     and, for networks on the LDS kernel, n_traj to 2^20 (REBOP_ERR_VALUE
+ *     beyond);
+ *   - no tail hand-off (identical results) and stats->kernel_ms = 0.
+ * All calls on one GPU share that region and it keeps the last call's
+ * contents: calls that overlap on one GPU race, so issue them on one stream.
+ */
+int rebop_run_device_noalloc(const rebop_gillespie *g, double tmax, uint64_t nb_steps,
+                             uint64_t traj_begin, uint64_t n_traj, const int64_t *d_x0,
+                             int64_t *d_out, int64_t *d_final, int kernel, void *stream,
+                             rebop_stats *stats);
+
+/*
  * The nb_steps == 0 mode of rebop's Python `Gillespie.run`
  * (src/pyo3_gillespie.rs:209-223 over Gillespie::_advance_one_reaction,
  * src/gillespie.rs:275-297) for n_traj trajectories: the state before the

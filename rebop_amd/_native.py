@@ -86,6 +86,11 @@ SIGNATURES: dict[str, tuple] = {
         [c_void_p, c_double, c_uint64, c_uint64, c_uint64, c_void_p, c_void_p, c_void_p, c_int,
          c_void_p, c_void_p],
     ),
+    "rebop_run_device_noalloc": (
+        c_int,
+        [c_void_p, c_double, c_uint64, c_uint64, c_uint64, c_void_p, c_void_p, c_void_p, c_int,
+         c_void_p, c_void_p],
+    ),
     "rebop_run_all_events": (
         c_int,
         [c_void_p, c_double, c_uint64, c_uint64, POINTER(c_uint64), POINTER(c_double), POINTER(c_int64), c_void_p],

@@ -20,7 +20,7 @@ SG_LIB = HERE / "librebop_surrogate.so"
 SG_SOURCES = ["surrogate.hip"]
 ARCH = "gfx950"
 
-SOURCES = ["ssa_kernels.hip", "ssa_lds.hip", "rebop_gpu.cpp", "jit.cpp"]
+SOURCES = ["ssa_kernels.hip", "ssa_lds.hip", "rebop_gpu.cpp", "jit.cpp", "noalloc.hip"]
 # headers whose text is embedded in the library for run-time compilation
 EMBEDDED_HEADERS = ["rebop_rng.h", "rebop_expr.h", "ssa_layout.h", "ssa_lane.h"]
 DELIM = "REBOPSRC"

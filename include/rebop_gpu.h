@@ -221,7 +221,12 @@ typedef struct rebop_tune_result {
  * persistent / plain, workgroup size 64/128/256, full or half resident grid,
  * LDS) on a pilot of n_pilot trajectories of this network (no output), and
  * makes REBOP_KERNEL_AUTO runs of this handle use the fastest.  All variants
- * give identical results, so the choice only affects speed.  No counterpart
+ * give identical results, so the choice only affects speed.  The choice is
+ * kept in an on-disk cache keyed by the network's shape (species, reactions,
+ * total reactant order), nb_steps, the pilot size (power of two) and the
+ * device; a later call for the same key reuses it without timing (n_tried =
+ * 0).  REBOP_TUNE_CACHE names the file (empty: no cache; default under
+ * $XDG_CACHE_HOME or ~/.cache), REBOP_TUNE_FORCE=1 re-times.  No counterpart
  * in the reference. */
 int rebop_autotune(rebop_gillespie *g, double tmax, uint64_t nb_steps, uint64_t n_pilot,
                    rebop_tune_result *out);

@@ -7,6 +7,7 @@
 #include "../../include/rebop_gpu.h"
 
 #include <hip/hip_runtime.h>
+#include <sys/stat.h>
 
 #include <algorithm>
 #include <cmath>
@@ -430,6 +431,76 @@ uint32_t tail_lanes() {
 int tail_group() {
   const char *v = std::getenv("REBOP_TAIL_GROUP");
   return v && std::strcmp(v, "2") == 0 ? 2 : 4;
+}
+
+// The autotuner's on-disk cache: one line per tuned shape,
+//   <key> <kernel> <threads> <occupancy> <ms>
+// keyed by the network shape (species, reactions, total reactant order,
+// specialised-only arithmetic), nb_steps, the run size (power of two) and the
+// device.  A cached choice only changes speed: every variant gives the same
+// results.  REBOP_TUNE_CACHE names the file (default
+// $XDG_CACHE_HOME/rebop_amd/tune_cache.txt or ~/.cache/...); an empty value
+// turns the cache off; REBOP_TUNE_FORCE=1 re-times and rewrites the entry.
+std::string tune_cache_path() {
+  if (const char *v = std::getenv("REBOP_TUNE_CACHE")) return v;
+  std::string base;
+  if (const char *x = std::getenv("XDG_CACHE_HOME"); x && *x)
+    base = x;
+  else if (const char *h = std::getenv("HOME"); h && *h)
+    base = std::string(h) + "/.cache";
+  else
+    return "";
+  return base + "/rebop_amd/tune_cache.txt";
+}
+
+std::string tune_key(const rebop_gillespie *g, uint64_t nb_steps, uint64_t n_pilot,
+                     const std::string &device) {
+  uint64_t order = 0;
+  for (auto &r : g->rx)
+    for (auto &e : r.entries) order += e.second;
+  int lg = 0;
+  while ((1ull << (lg + 1)) <= n_pilot && lg < 62) ++lg;
+  char buf[256];
+  std::snprintf(buf, sizeof buf, "S%u-R%zu-O%llu-E%u-Y%d-nb%llu-n2^%d-%s", g->S, g->rx.size(),
+                (unsigned long long)order, g->n_expr, g->system ? 1 : 0,
+                (unsigned long long)nb_steps, lg, device.c_str());
+  return buf;
+}
+
+bool tune_cache_get(const std::string &path, const std::string &key, int *kernel, int *threads,
+                    double *occ, double *ms) {
+  if (path.empty()) return false;
+  FILE *f = std::fopen(path.c_str(), "r");
+  if (!f) return false;
+  char k[256];
+  int kk = 0, th = 0;
+  double oc = 0, t = 0;
+  bool found = false;
+  while (std::fscanf(f, "%255s %d %d %lf %lf", k, &kk, &th, &oc, &t) == 5)
+    if (key == k) { // the last entry for the key wins
+      *kernel = kk;
+      *threads = th;
+      *occ = oc;
+      *ms = t;
+      found = true;
+    }
+  std::fclose(f);
+  return found;
+}
+
+void tune_cache_put(const std::string &path, const std::string &key, int kernel, int threads,
+                    double occ, double ms) {
+  if (path.empty()) return;
+  const size_t slash = path.rfind('/');
+  if (slash != std::string::npos) {
+    // create the directory chain (failures are ignored: the cache is optional)
+    for (size_t i = 1; i <= slash; ++i)
+      if (path[i] == '/' || i == slash) (void)mkdir(path.substr(0, i == slash ? slash : i).c_str(), 0755);
+  }
+  if (FILE *f = std::fopen(path.c_str(), "a")) {
+    std::fprintf(f, "%s %d %d %.6f %.6f\n", key.c_str(), kernel, threads, occ, ms);
+    std::fclose(f);
+  }
 }
 
 bool jit_disabled() {
@@ -1215,6 +1286,26 @@ int rebop_autotune(rebop_gillespie *g, double tmax, uint64_t nb_steps, uint64_t 
   rc = upload_net(g);
   if (rc) return rc;
   const uint32_t S = g->S, R = (uint32_t)g->rx.size();
+  const std::string cache = tune_cache_path(), key = tune_key(g, nb_steps, n_pilot, g_dev.name);
+  const char *force = std::getenv("REBOP_TUNE_FORCE");
+  {
+    int k = 0, th = 0;
+    double oc = 0, ms = 0;
+    if (!(force && *force && std::strcmp(force, "0") != 0) &&
+        tune_cache_get(cache, key, &k, &th, &oc, &ms)) {
+      g->tune_kernel = k;
+      g->tune_threads = th;
+      g->tune_occ = oc;
+      if (out) {
+        out->kernel = k;
+        out->threads = th;
+        out->occupancy = oc;
+        out->ms = ms;
+        out->n_tried = 0; // from the cache
+      }
+      return REBOP_OK;
+    }
+  }
   struct Cand {
     int kernel, threads;
     double occ;
@@ -1268,6 +1359,7 @@ int rebop_autotune(rebop_gillespie *g, double tmax, uint64_t nb_steps, uint64_t 
   g->tune_kernel = pick.kernel;
   g->tune_threads = pick.threads;
   g->tune_occ = pick.occ;
+  tune_cache_put(cache, key, pick.kernel, pick.threads, pick.occ, best);
   if (out) {
     out->kernel = pick.kernel;
     out->threads = pick.threads;

@@ -131,7 +131,8 @@ typedef struct {
  * when a wave starts an item of the job (read R times per event):
  *   w0, w1  rate constant k (f64 bits, low word first)
  *   w2, w3  the first two propensity factors (x[s] - c) in Rate::rate order:
- *           s in bits 0-15, c in bits 16-27, bit 31 = present
+ *           s in bits 0-15, c in bits 16-27, bit 31 = present (an absent
+ *           factor multiplies by exactly 1.0)
  *   w3      bit 30: the factors are not all here (more than two, or c >=
  *           4096): use fac_begin/fac_s/fac_c for this reaction
  *           bit 29: the jump is not all in the jump record (more than four
@@ -141,8 +142,8 @@ typedef struct {
  * 8-31 (zero difference = unused).
  */
 #define REBOP_REC_WORDS 4u
-#define REBOP_JREC_WORDS 4u
 #define REBOP_REC_PRESENT 0x80000000u
+#define REBOP_JREC_WORDS 4u
 #define REBOP_REC_SLOW_FACTORS 0x40000000u
 #define REBOP_REC_SLOW_JUMP 0x20000000u
 

@@ -1,6 +1,6 @@
 // ssa_lds.hip -- LDS-resident SSA kernel for large and heterogeneous networks.
 //
-// Counts live in LDS, one column per lane (x[s] at lds[s * BL + lane]), so
+// Counts live in LDS, one column per lane (x[s] at lds[s * 64 + lane]), so
 // every access -- uniform or lane-varying species index -- maps lane l to
 // the same banks whatever the species: conflict-free.  The narrow kernel
 // keeps them as int32 (|x| <= 2^30, half the LDS, twice the workgroups per
@@ -71,9 +71,9 @@ struct LdsNet {
   const uint32_t *jb;
   const uint32_t *js;
   const int32_t *jd;
-  // this lane's counts
+  // this lane's column of counts
+  static constexpr uint32_t kBL = 64; // lanes per workgroup (one wave)
   CountT *xl;
-  uint32_t BL;
   // the job's reaction records in LDS, jump records in global memory
   const uint32_t *rec;
   const uint32_t *jrec;
@@ -81,8 +81,8 @@ struct LdsNet {
   bool linear;
   bool bad;
 
-  __device__ __forceinline__ CountT &cnt(uint32_t s) const { return xl[(uint64_t)s * BL]; }
-  __device__ __forceinline__ double fcnt(uint32_t s) const { return (double)xl[(uint64_t)s * BL]; }
+  __device__ __forceinline__ CountT &cnt(uint32_t s) const { return xl[s * kBL]; }
+  __device__ __forceinline__ double fcnt(uint32_t s) const { return (double)cnt(s); }
 
   __device__ __forceinline__ uint32_t S() const { return nS; }
   __device__ __forceinline__ void start(const rebop_dev_run &, uint64_t) {}
@@ -129,10 +129,11 @@ struct LdsNet {
     return a;
   }
 
-  // factor (x[s] - c) of a record word, or exactly 1.0 when absent.  The
-  // narrow kernel takes the difference in int32 (|x| <= 2^30, c < 4096) and
-  // converts once; the wide one subtracts exact integers in f64 (|x| <= 2^52):
-  // either way the value of the reference's integer difference converted
+  // factor (x[s] - c) of a record word whose count x is read, or exactly
+  // 1.0 when absent.  The narrow kernel takes the difference in int32
+  // (|x| <= 2^30, c < 4096) and converts once; the wide one subtracts exact
+  // integers in f64 (|x| <= 2^52): either way the reference's integer
+  // difference, converted
   __device__ __forceinline__ double factor_val(uint32_t w, CountT x) const {
     if constexpr (WIDE) {
       return (w & REBOP_REC_PRESENT) ? x - (double)((w >> 16) & 0xfffu) : 1.0;
@@ -345,7 +346,6 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
     net.js = job.jump_s;
     net.jd = job.jump_d;
     net.xl = reinterpret_cast<typename LdsNet<MB, WIDE>::CountT *>(lds_x) + threadIdx.x;
-    net.BL = blockDim.x;
     net.rec = lrec;
     net.jrec = job.jrec;
     net.linear = net.job_linear;

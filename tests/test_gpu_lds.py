@@ -256,9 +256,10 @@ def test_lds_jumps_on_species_beyond_255(nthreads):
     assert st.events == oev and oev > 128 * 50
 
 
-def test_autotune_picks_a_variant_that_stays_bit_exact(nthreads):
+def test_autotune_picks_a_variant_that_stays_bit_exact(nthreads, tmp_path, monkeypatch):
     """rebop_autotune times the applicable variants; the AUTO runs that follow
     use the winner and still equal the oracle bit for bit."""
+    monkeypatch.setenv("REBOP_TUNE_CACHE", str(tmp_path / "tune.txt"))  # time, do not reuse
     spec = dimers()
     g = build_device(spec, 31)
     res = g.autotune(0.5, 50, n_pilot=4096)
@@ -269,3 +270,4 @@ def test_autotune_picks_a_variant_that_stays_bit_exact(nthreads):
     o = build_oracle(spec)
     _, oout, _, oev, _ = o.run_grid(31, spec[2], 0.5, 50, n_traj=4096, nthreads=nthreads)
     assert np.array_equal(out, oout) and st.events == oev
+

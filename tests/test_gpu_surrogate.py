@@ -397,8 +397,9 @@ def test_trust_map_holds_on_held_out_points(sg):
     mlp = sg.MLP(seed=3)
     sg.train(mlp, data, steps=3000, batch=8192, lr=1e-3)
     del data
-    vr, vx = sg.sample_points(256, seed=23)
-    rel, trust, _ = sg.validate(mlp, vr, vx, n_ref=10_000, seed=24, min_points=4)
+    vr, vx = sg.sample_points(1024, seed=23)
+    rel, trust, _ = sg.validate(mlp, vr, vx, n_ref=10_000, seed=24, bins=3, min_points=4,
+                                trust_tol=0.015)
     hr, hx = sg.sample_points(256, seed=25)
     rel_h, _, _ = sg.validate(mlp, hr, hx, n_ref=10_000, seed=26, traj_begin=1 << 41)
     adm = np.stack([np.stack([trust.trusted(hr, hx, t, s) for s in range(sg.S)], axis=1)

@@ -43,7 +43,9 @@ std::string jit_source(uint32_t S, const std::vector<JitReaction> &rx, bool syst
 int jit_get(const std::string &src, JitKernel *out, std::string *err);
 // Compiles for gfx950 without loading (no device needed).
 int jit_compile_only(const std::string &src, size_t *code_size, std::string *err);
-hipError_t jit_occupancy(const JitKernel &k, int threads, int *blocks_per_cu);
+// Dynamic LDS bytes a launch of `run` takes (the LDS copy of the sample times).
+size_t jit_dynamic_lds(const rebop_dev_run &run);
+hipError_t jit_occupancy(const JitKernel &k, int threads, size_t lds, int *blocks_per_cu);
 // group = lanes per trajectory (1, 2 or 4; 2 and 4 only for plain, non-sweep
 // grid/target runs)
 hipError_t jit_launch(const JitKernel &k, const rebop_dev_run &run, int blocks, int threads,

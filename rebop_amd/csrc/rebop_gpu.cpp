@@ -828,7 +828,8 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
   if (persistent) {
     int per_cu = 0;
     if (spec)
-      HIP_TRY(rebop::jit_occupancy(g->jitk[run.draw_cap ? 1 : 0], threads, &per_cu));
+      HIP_TRY(rebop::jit_occupancy(g->jitk[run.draw_cap ? 1 : 0], threads,
+                                   rebop::jit_dynamic_lds(run), &per_cu));
     else
       HIP_TRY(rebop::reg_occupancy(shape, threads, lds, &per_cu));
     if (per_cu < 1) per_cu = 1;

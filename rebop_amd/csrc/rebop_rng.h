@@ -276,9 +276,19 @@ static const double rebop_ln_table[128][4] = {
 /*
  * Natural logarithm for positive, normal, finite x (the SSA only calls it on
  * U in [2^-53, 1]).  No special-value handling: zero, subnormals, negative
- * numbers, inf and NaN are outside the contract.
+ * numbers, inf and NaN are outside the contract.  `tab` is rebop_ln_table or
+ * a copy of it (device code may pass one held in LDS).
+ *
+ * In two halves, so that a caller can issue the table read long before it
+ * needs the value: rebop_ln64_begin reduces x and reads its table row,
+ * rebop_ln64_finish evaluates the rest.  rebop_ln64_tab is the two in a row.
  */
-REBOP_HD double rebop_ln64(double x) {
+typedef struct {
+  double m, dk;       /* x = 2^k m */
+  double t0, t1, t2;  /* the bucket's table row */
+} rebop_ln_pre;
+
+REBOP_HD rebop_ln_pre rebop_ln64_begin(double x, const double (*tab)[4]) {
   const uint64_t u = rebop_f64_bits(x);
   int32_t k = (int32_t)(u >> 52) - 1023;
   const uint64_t mant = u & 0x000fffffffffffffull;
@@ -288,9 +298,18 @@ REBOP_HD double rebop_ln64(double x) {
     k += 1;
   }
   const uint64_t mb = mexp | mant;
-  const double m = rebop_f64_from_bits(mb);
-  const double *t = rebop_ln_table[(uint32_t)((mb - REBOP_LN_BASE) >> 45)];
-  const double r = __builtin_fma(m, t[0], -1.0);
+  const double *t = tab[(uint32_t)((mb - REBOP_LN_BASE) >> 45)];
+  rebop_ln_pre p;
+  p.m = rebop_f64_from_bits(mb);
+  p.dk = (double)k;
+  p.t0 = t[0];
+  p.t1 = t[1];
+  p.t2 = t[2];
+  return p;
+}
+
+REBOP_HD double rebop_ln64_finish(rebop_ln_pre q) {
+  const double r = __builtin_fma(q.m, q.t0, -1.0);
   double p = -0x1p-3;                            /* -1/8 */
   p = __builtin_fma(p, r, 0x1.2492492492492p-3);  /*  1/7 */
   p = __builtin_fma(p, r, -0x1.5555555555555p-3); /* -1/6 */
@@ -298,18 +317,29 @@ REBOP_HD double rebop_ln64(double x) {
   p = __builtin_fma(p, r, -0x1p-2);               /* -1/4 */
   p = __builtin_fma(p, r, 0x1.5555555555555p-2);  /*  1/3 */
   const double r2 = r * r;
-  const double dk = (double)k;
+  const double dk = q.dk;
   const double a = dk * REBOP_LN2_HI; /* exact: 32-bit ln2_hi times |k| <= 1075 */
-  const double hi = a + t[1];         /* |a| >= |t[1]| or a == 0: Fast2Sum */
-  const double err = (a - hi) + t[1];
-  const double lo = __builtin_fma(dk, REBOP_LN2_LO, t[2]) + err;
+  const double hi = a + q.t1;         /* |a| >= |t1| or a == 0: Fast2Sum */
+  const double err = (a - hi) + q.t1;
+  const double lo = __builtin_fma(dk, REBOP_LN2_LO, q.t2) + err;
   const double tail = __builtin_fma(r2, __builtin_fma(r, p, -0.5), lo);
   return hi + (r + tail);
 }
 
+REBOP_HD double rebop_ln64_tab(double x, const double (*tab)[4]) {
+  return rebop_ln64_finish(rebop_ln64_begin(x, tab));
+}
+
+REBOP_HD double rebop_ln64(double x) { return rebop_ln64_tab(x, rebop_ln_table); }
+
 /* The exponential E of draw block b: E = -ln(U1), U1 in (0, 1]. */
 REBOP_HD double rebop_exp1_of(rebop_u32x4 b) {
   return -rebop_ln64(rebop_u01_open_closed(b.w[0], b.w[1]));
+}
+
+/* The same E with the ln table at `tab` (a copy of rebop_ln_table). */
+REBOP_HD double rebop_exp1_of_tab(rebop_u32x4 b, const double (*tab)[4]) {
+  return -rebop_ln64_tab(rebop_u01_open_closed(b.w[0], b.w[1]), tab);
 }
 
 /* The uniform U of draw block b, in [0, 1). */

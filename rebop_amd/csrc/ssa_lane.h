@@ -180,9 +180,11 @@ __device__ __forceinline__ double props_checked(Net &net, X &x, C &cum, LaneTota
 // Runs trajectories on this lane until none is left.  `first` is the lane's
 // first local trajectory index (>= run.n_traj for none); a persistent launch
 // then takes indices queue_offset + atomicAdd(queue) from the device queue.
+// `ln_tab`: the ln table, or the workgroup's copy of it in LDS.
 template <class Net>
 __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &run, uint64_t first,
-                                                uint64_t queue_offset) {
+                                                uint64_t queue_offset,
+                                                const double (*ln_tab)[4] = rebop_ln_table) {
   const uint64_t n = run.n_traj;
   const uint32_t nb = run.mode == REBOP_MODE_TARGET ? 0u : (uint32_t)run.nb_steps;
   const uint64_t Sn = (uint64_t)net.S() * n;
@@ -248,7 +250,7 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
     if (0.0 < tot) {
       const rebop_u32x4 blk = rebop_draw_block(run.seed, run.traj_begin + jl, draw);
       draw += 1;
-      tn = t + rebop_exp1_of(blk) / tot;
+      tn = t + rebop_exp1_of_tab(blk, ln_tab) / tot;
       u = rebop_unif_of(blk);
       rec = tn > T; // overshoot: the draw is discarded
       ov += rec ? 1u : 0u;
@@ -351,9 +353,21 @@ __device__ __forceinline__ double group_read(double v, int q) {
 // and counts.  Plain launches only: trajectory j is group j, or with
 // run.resume the trajectory of spill entry j (the tail of a persistent
 // launch, continued from its written-out state).
-template <class Net, int K>
+//
+// One wave per SIMD issues every instruction of the chain itself, so the
+// loop keeps memory latency off that chain: `ln_tab` is the workgroup's LDS
+// copy of the ln table, whose rows for the next round's draws are read while
+// this round's events run (the next round's blocks are (draw + K + sub),
+// since a round either consumes K draws or ends the drawing for good), and
+// `grid_lds` its LDS copy of the sample times, read at every iteration while
+// the division runs.  No vector-memory wait is left in the loop, so the
+// sample stores are never waited for.  GridLds: grid_lds holds the sample
+// times (else the next one is carried, computed at each sample, for grids too
+// long for LDS).
+template <class Net, int K, bool GridLds>
 __device__ __forceinline__ LaneTotals lane_loop_group(Net &net, const rebop_dev_run &run,
-                                                      uint64_t gtid) {
+                                                      uint64_t gtid, const double (*ln_tab)[4],
+                                                      const double *grid_lds) {
   LaneTotals acc;
   const uint64_t n = run.n_traj;
   const uint64_t e = gtid / K; // group index
@@ -406,29 +420,38 @@ __device__ __forceinline__ LaneTotals lane_loop_group(Net &net, const rebop_dev_
   // (a resumed tail entry started it in the persistent launch)
   const uint64_t dlim =
       cap_limit(run, run.resume ? (run.st_x ? run.st_draw[jl] : 0ull) : draw);
-  // latency-bound launches: computing a sample time beats loading it
-  double T = grid_time(run, i), tot = 0.0;
+  double Tc = GridLds ? 0.0 : grid_time(run, i), tot = 0.0;
   int64_t *orec = run.out && lead ? run.out + jl + (uint64_t)i * Sn : nullptr;
   bool done = false;
   tot = props_checked(net, x, cum, acc);
   if (capped) tot = 0.0; // stopped by the cap before the hand-off (already counted)
 
+  // this lane's draw of the next round: ln reduced and its table row read,
+  // the rest of the ln evaluated when the round starts
+  rebop_ln_pre pre;
+  double u_next;
+  auto fetch = [&](uint64_t d) {
+    const rebop_u32x4 blk = rebop_draw_block(run.seed, run.traj_begin + jl, d + sub);
+    pre = rebop_ln64_begin(rebop_u01_open_closed(blk.w[0], blk.w[1]), ln_tab);
+    u_next = rebop_unif_of(blk);
+  };
+  fetch(draw);
   while (!done) {
-    const rebop_u32x4 blk = rebop_draw_block(run.seed, run.traj_begin + jl, draw + sub);
-    const double e_own = rebop_exp1_of(blk), u_own = rebop_unif_of(blk);
+    const double e_own = -rebop_ln64_finish(pre), u_own = u_next;
+    const uint64_t d0 = draw;
 #pragma unroll
     for (int q = 0; q < K; ++q) {
       const double e = group_read<K>(e_own, q);
       const double u = group_read<K>(u_own, q);
       if (!done) {
-        bool rec = true;
-        double tn = 0.0;
-        if (0.0 < tot) {
-          draw += 1;
-          tn = t + e / tot;
-          rec = tn > T;
-          ov += rec ? 1u : 0u;
-        }
+        const double T = GridLds ? grid_lds[i] : Tc; // i <= nb until done
+        // an absorbing (or capped) state takes no draw and records; the
+        // quotient is then unused (E / 0 is not trapped)
+        const bool live = 0.0 < tot;
+        const double tn = t + e / tot;
+        const bool rec = !live || tn > T;
+        draw += live ? 1u : 0u;
+        ov += (live && rec) ? 1u : 0u;
         if (rec) {
           if (orec) {
             net.write_raw(x, orec, n);
@@ -439,6 +462,7 @@ __device__ __forceinline__ LaneTotals lane_loop_group(Net &net, const rebop_dev_
           mark = draw;
           t = T;
           i += 1;
+          if constexpr (!GridLds) Tc = grid_time(run, i);
           if (i > nb) {
             if (lead && run.final_x) net.write_counts(x, run.final_x + jl, n);
             if (lead && run.st_x) {
@@ -452,8 +476,6 @@ __device__ __forceinline__ LaneTotals lane_loop_group(Net &net, const rebop_dev_
               acc.trajectories += 1;
             }
             done = true;
-          } else {
-            T = grid_time(run, i);
           }
         } else {
           t = tn;
@@ -463,6 +485,9 @@ __device__ __forceinline__ LaneTotals lane_loop_group(Net &net, const rebop_dev_
             if (__builtin_expect(draw >= dlim, 0)) tot = hit_cap(run, acc, lead);
         }
       }
+      // after the first event's sample-time read, so that waiting for that
+      // read does not wait for these
+      if (q == 0) fetch(d0 + K);
     }
   }
   return acc;
@@ -477,7 +502,8 @@ __device__ __forceinline__ LaneTotals lane_loop_group(Net &net, const rebop_dev_
 // rec_offsets[j] + r (the stream is deterministic, so both passes agree).
 template <class Net>
 __device__ __forceinline__ LaneTotals lane_events(Net &net, const rebop_dev_run &run,
-                                                  uint64_t first, uint64_t queue_offset) {
+                                                  uint64_t first, uint64_t queue_offset,
+                                                  const double (*ln_tab)[4] = rebop_ln_table) {
   const uint64_t n = run.n_traj;
   const uint32_t S = net.S();
   double x[Net::MS];
@@ -521,7 +547,7 @@ __device__ __forceinline__ LaneTotals lane_events(Net &net, const rebop_dev_run 
       } else {
         const rebop_u32x4 blk = rebop_draw_block(run.seed, run.traj_begin + jl, draw);
         draw += 1;
-        t = t + rebop_exp1_of(blk) / tot;
+        t = t + rebop_exp1_of_tab(blk, ln_tab) / tot;
         net.affect(x, net.choose(x, cum, tot * rebop_unif_of(blk)));
         if (!net.in_range(x)) acc.flags |= Net::kRangeFlag;
         if constexpr (Net::kCap) {
@@ -551,7 +577,8 @@ __device__ __forceinline__ LaneTotals lane_events(Net &net, const rebop_dev_run 
 // round are used in order or not at all (the argument of lane_loop_group).
 // All lanes hold the same state; lane 0 of the group writes and counts.
 template <class Net, int K>
-__device__ __forceinline__ LaneTotals lane_events_group(Net &net, const rebop_dev_run &run, uint64_t gtid) {
+__device__ __forceinline__ LaneTotals lane_events_group(Net &net, const rebop_dev_run &run, uint64_t gtid,
+                                                        const double (*ln_tab)[4]) {
   LaneTotals acc;
   const uint64_t jl = gtid / K;
   if (jl >= run.n_traj) return acc;
@@ -584,7 +611,7 @@ __device__ __forceinline__ LaneTotals lane_events_group(Net &net, const rebop_de
   emit();
   while (t < run.tmax && !stop) {
     const rebop_u32x4 blk = rebop_draw_block(run.seed, run.traj_begin + jl, draw + sub);
-    const double e_own = rebop_exp1_of(blk), u_own = rebop_unif_of(blk);
+    const double e_own = rebop_exp1_of_tab(blk, ln_tab), u_own = rebop_unif_of(blk);
 #pragma unroll
     for (int q = 0; q < K; ++q) {
       const double e = group_read<K>(e_own, q);

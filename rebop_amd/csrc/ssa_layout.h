@@ -33,6 +33,10 @@
 #define REBOP_EXACT_LIMIT 4503599627370496.0 /* 2^52 */
 #define REBOP_DIFF_LIMIT (1ll << 20)
 #define REBOP_DRAWS_PER_CHECK (1ull << 31)
+/* the specialised kernels copy the sample times into LDS when nb_steps is
+ * below this (at most 16 KB per workgroup); above it they read the table in
+ * global memory (grouped kernels: compute them) */
+#define REBOP_GRID_LDS 2048u
 
 /* error flags raised by kernels */
 #define REBOP_FLAG_RANGE 1u

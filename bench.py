@@ -73,7 +73,7 @@ def parse_issue_counters(path: Path):
     keys = {
         "VALU wave-instructions per event": "valu_wave_instructions_per_event",
         "active-lane fraction of VALU issue": "active_lane_fraction",
-        "VALU issue cycles per CU-cycle": "valu_issue_per_cu_cycle",
+        "VALU issue per CU-cycle": "valu_issue_per_cu_cycle",
     }
     for line in path.read_text().splitlines():
         for k, name in keys.items():
@@ -84,7 +84,8 @@ def parse_issue_counters(path: Path):
                     pass
     if not out:
         return None
-    out["source"] = str(path.relative_to(ROOT))
+    out["source"] = (f"{path.relative_to(ROOT)} (rocprofv3 --pmc of the same workload, a committed "
+                     "measurement, not this run's)")
     return out
 
 

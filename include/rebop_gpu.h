@@ -163,11 +163,14 @@ int rebop_gillespie_add_reaction_lma_sparse(rebop_gillespie *g, double rate,
  * Rate::expr at :33-35.  The propensity is the expression's value (no
  * mass-action factor).  The program must be well formed (one value left, at
  * most 32 operands deep, at most 4096 tokens; REBOP_ERR_VALUE) and refer to
- * existing constants and species (REBOP_ERR_INDEX).  Networks with rate
- * expressions run on the network-specialised kernels (S <= 32, R <= 64);
- * other kernels, fused job batches and rate tables return
- * REBOP_ERR_UNSUPPORTED.  An expression that evaluates to +inf makes the run
- * return REBOP_ERR_RANGE (time would not advance). */
+ * existing constants and species (REBOP_ERR_INDEX).  A network with rate
+ * expressions runs on a network-specialised kernel, with the expressions
+ * compiled in, when it fits one (S <= 32, R <= 64), and otherwise (or when
+ * REBOP_KERNEL_LDS is asked for) on the LDS kernel, which interprets them;
+ * fused job batches take such networks too.  The data-driven register
+ * kernels never run them, and rate tables (rebop_run_sweep_device with
+ * d_rates) return REBOP_ERR_UNSUPPORTED.  An expression that evaluates to
+ * +inf makes the run return REBOP_ERR_RANGE (time would not advance). */
 int rebop_gillespie_add_reaction_expr(rebop_gillespie *g, const uint32_t *code, uint32_t n_code,
                                       const double *consts, uint32_t n_consts,
                                       const int64_t *differences, uint32_t n_differences);
@@ -184,8 +187,10 @@ int rebop_gillespie_add_reaction_expr(rebop_gillespie *g, const uint32_t *code, 
  *
  * rebop_system_new makes such a handle (rebop_gillespie_add_reaction_* then
  * return REBOP_ERR_UNSUPPORTED); every run entry point, batch and parameter
- * sweep accepts it and uses the network-specialised kernels (S <= 32,
- * R <= 64); the LDS kernel and fused job batches return REBOP_ERR_UNSUPPORTED.
+ * sweep accepts it.  It runs on a network-specialised kernel when the
+ * network fits one (S <= 32, R <= 64), otherwise (or when REBOP_KERNEL_LDS
+ * is asked for) on the LDS kernel, which also takes it in fused job batches;
+ * never on the data-driven register kernels.
  */
 #define REBOP_SYSTEM_MAX_COUNT 64u /* largest stoichiometric coefficient */
 int rebop_system_new(const int64_t *species, uint32_t nb_species, uint64_t seed,

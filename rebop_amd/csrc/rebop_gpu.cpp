@@ -39,11 +39,11 @@ hipError_t launch_reg(int c, const rebop_dev_net &net, const rebop_dev_run &run,
                       int threads, size_t lds, hipStream_t stream);
 
 int lds_checkpoints();
-hipError_t lds_occupancy(int threads, size_t lds, bool wide, int *blocks_per_cu);
+hipError_t lds_occupancy(int threads, size_t lds, bool wide, bool expr, int *blocks_per_cu);
 hipError_t launch_lds(const rebop_dev_job *jobs, const rebop_dev_item *items, uint32_t n_items,
                       unsigned long long *counters, unsigned long long *job_events,
                       unsigned long long *job_queue, int blocks, int threads, size_t lds,
-                      uint32_t rec_word_off, bool wide, hipStream_t stream);
+                      uint32_t rec_word_off, bool wide, bool expr, hipStream_t stream);
 
 } // namespace rebop
 
@@ -291,6 +291,40 @@ int upload_net(rebop_gillespie *g) {
     jb[r + 1] = (uint32_t)js.size();
   }
   const uint32_t J = (uint32_t)js.size();
+  // Rate::Expr programs for the LDS kernel's interpreter: one token array with
+  // per-reaction offsets, constant tokens rebased onto one constant array
+  std::vector<uint32_t> eb(R + 1, 0), ecode;
+  std::vector<double> econst;
+  for (uint32_t r = 0; r < R; ++r) {
+    const Reaction &x = g->rx[r];
+    const uint32_t cbase = (uint32_t)econst.size();
+    for (uint32_t t : x.expr) {
+      const uint32_t op = REBOP_EXPR_OPCODE(t), arg = REBOP_EXPR_ARG(t);
+      ecode.push_back(op == REBOP_OP_CONST ? REBOP_EXPR_TOKEN(op, cbase + arg) : t);
+    }
+    econst.insert(econst.end(), x.consts.begin(), x.consts.end());
+    if (econst.size() >= (1u << 24))
+      return fail(REBOP_ERR_UNSUPPORTED, "more than 2^24 rate-expression constants in total");
+    eb[r + 1] = (uint32_t)ecode.size();
+    if (!x.expr.empty()) linear = true; // an expression may be negative or not monotone
+  }
+  const uint32_t EC = (uint32_t)ecode.size(), EK = (uint32_t)econst.size();
+  // define_system reactant terms (species, multiplicity) in written order
+  std::vector<uint32_t> tbeg(R + 1, 0), tsp, tmul;
+  if (g->system) {
+    for (uint32_t r = 0; r < R; ++r) {
+      const Reaction &x = g->rx[r];
+      for (auto &e : x.entries) {
+        tsp.push_back(e.first);
+        tmul.push_back(e.second);
+      }
+      tbeg[r + 1] = (uint32_t)tsp.size();
+    }
+    // a NaN or negative rate, or a term product that wraps, can make the sum
+    // non-monotone: the choice scans linearly (it is exact either way)
+    linear = true;
+  }
+  const uint32_t TS = (uint32_t)tsp.size();
   // LDS-kernel reaction records (layout in ssa_layout.h)
   std::vector<uint32_t> rec((size_t)REBOP_REC_WORDS * R, 0u);
   std::vector<uint32_t> jrec((size_t)REBOP_JREC_WORDS * R, 0u);
@@ -302,7 +336,8 @@ int upload_net(rebop_gillespie *g) {
     w[0] = (uint32_t)kb;
     w[1] = (uint32_t)(kb >> 32);
     const uint32_t nf = fb[r + 1] - fb[r];
-    bool slow_f = nf > 2;
+    // an expression (the interpreter) or a define_system product: the slow path
+    bool slow_f = nf > 2 || !g->rx[r].expr.empty() || g->system;
     for (uint32_t q = 0; q < nf && q < 2; ++q) {
       const uint32_t f = fb[r] + q;
       const uint32_t c = (uint32_t)fc[f];
@@ -338,7 +373,13 @@ int upload_net(rebop_gillespie *g) {
   const size_t ojd = ojs + al(sizeof(uint32_t) * J);
   const size_t orc = ojd + al(sizeof(int32_t) * J);
   const size_t ojr = orc + al(sizeof(uint32_t) * rec.size());
-  const size_t total = ojr + al(sizeof(uint32_t) * jrec.size());
+  const size_t oeb = ojr + al(sizeof(uint32_t) * jrec.size());
+  const size_t oec = oeb + al(sizeof(uint32_t) * (R + 1));
+  const size_t oek = oec + al(sizeof(uint32_t) * EC);
+  const size_t otb = oek + al(sizeof(double) * EK);
+  const size_t ots = otb + al(sizeof(uint32_t) * (R + 1));
+  const size_t otm = ots + al(sizeof(uint32_t) * TS);
+  const size_t total = otm + al(sizeof(uint32_t) * TS);
   std::vector<unsigned char> host(total, 0);
   std::memcpy(host.data() + ojb, jb.data(), sizeof(uint32_t) * (R + 1));
   if (J) std::memcpy(host.data() + ojs, js.data(), sizeof(uint32_t) * J);
@@ -350,6 +391,18 @@ int upload_net(rebop_gillespie *g) {
   if (F) std::memcpy(host.data() + ofs, fs.data(), sizeof(uint32_t) * F);
   if (F) std::memcpy(host.data() + ofc, fc.data(), sizeof(double) * F);
   if (R * S) std::memcpy(host.data() + odf, diff.data(), sizeof(double) * R * S);
+  if (EC) {
+    std::memcpy(host.data() + oeb, eb.data(), sizeof(uint32_t) * (R + 1));
+    std::memcpy(host.data() + oec, ecode.data(), sizeof(uint32_t) * EC);
+  }
+  if (EK) std::memcpy(host.data() + oek, econst.data(), sizeof(double) * EK);
+  if (g->system) {
+    std::memcpy(host.data() + otb, tbeg.data(), sizeof(uint32_t) * (R + 1));
+    if (TS) {
+      std::memcpy(host.data() + ots, tsp.data(), sizeof(uint32_t) * TS);
+      std::memcpy(host.data() + otm, tmul.data(), sizeof(uint32_t) * TS);
+    }
+  }
   HIP_TRY(g->dnet.ensure(total));
   HIP_TRY(hipMemcpy(g->dnet.p, host.data(), total, hipMemcpyHostToDevice));
   HIP_TRY(g->dx0.ensure(sizeof(int64_t) * (S ? S : 1)));
@@ -373,7 +426,8 @@ int upload_net(rebop_gillespie *g) {
   // handles reactions in groups of 4); any block size gives the same sums
   jt.block = R ? ((R + MB - 1) / MB + 3u) & ~3u : 4;
   jt.nblocks = R ? (R + jt.block - 1) / jt.block : 0;
-  jt.flags = (linear ? REBOP_JOB_LINEAR : 0u) | (any_slow ? REBOP_JOB_SLOW : 0u);
+  jt.flags = (linear ? REBOP_JOB_LINEAR : 0u) | (any_slow ? REBOP_JOB_SLOW : 0u) |
+             (g->system ? REBOP_JOB_SYSTEM : 0u);
   jt.k = g->view.k;
   jt.fac_begin = g->view.fac_begin;
   jt.fac_s = g->view.fac_s;
@@ -383,6 +437,16 @@ int upload_net(rebop_gillespie *g) {
   jt.jump_d = (const int32_t *)(base + ojd);
   jt.rec = (const uint32_t *)(base + orc);
   jt.jrec = (const uint32_t *)(base + ojr);
+  if (EC) {
+    jt.expr_begin = (const uint32_t *)(base + oeb);
+    jt.expr_code = (const uint32_t *)(base + oec);
+    jt.expr_const = (const double *)(base + oek);
+  }
+  if (g->system) {
+    jt.sys_begin = (const uint32_t *)(base + otb);
+    jt.sys_species = (const uint32_t *)(base + ots);
+    jt.sys_mult = (const uint32_t *)(base + otm);
+  }
   g->job = jt;
   g->dirty = false;
   g->dev_id = cur;
@@ -414,8 +478,13 @@ bool jit_eligible(const rebop_gillespie *g) {
   return F <= 4096;
 }
 
-// Networks whose arithmetic exists only in the specialised kernels.
-bool specialised_only(const rebop_gillespie *g) { return g->n_expr || g->system; }
+// Networks that a run puts on a specialised kernel whatever the kernel kind
+// (other than LDS): define_system arithmetic and rate expressions are
+// compiled into it when the network fits one (larger ones run on the LDS
+// kernel); the data-driven register kernels take neither.
+bool specialised_only(const rebop_gillespie *g) {
+  return (g->system || g->n_expr) && jit_eligible(g);
+}
 
 // Lanes at or below which a wave of a persistent specialised launch hands
 // its trajectories to the grouped tail launch once the queue is empty
@@ -599,9 +668,11 @@ int enqueue_lds(const std::vector<rebop_dev_job> &jobs, const std::vector<double
                 unsigned long long *job_events, unsigned long long *job_queue, bool wide,
                 hipStream_t stream) {
   uint32_t smax = 1, rmax = 1;
+  bool expr = false; // some job has rate expressions: the interpreting kernel
   for (auto &j : jobs) {
     smax = std::max(smax, j.S);
     rmax = std::max(rmax, j.R);
+    expr = expr || j.expr_begin != nullptr;
   }
   // counts [S_max][64] (int32, or f64 in the wide kernel), then the reaction
   // records (16-byte aligned)
@@ -633,7 +704,7 @@ int enqueue_lds(const std::vector<rebop_dev_job> &jobs, const std::vector<double
   HIP_TRY(hipMemcpyAsync(pj, jobs.data(), jbytes, hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(pi, items.data(), ibytes, hipMemcpyHostToDevice, stream));
   int per_cu = 0;
-  HIP_TRY(rebop::lds_occupancy(kLdsThreads, lds, wide, &per_cu));
+  HIP_TRY(rebop::lds_occupancy(kLdsThreads, lds, wide, expr, &per_cu));
   if (per_cu < 1) per_cu = 1;
   const uint64_t waves_per_block = kLdsThreads / 64;
   uint64_t blocks = (uint64_t)per_cu * (uint64_t)g_dev.cus;
@@ -641,7 +712,8 @@ int enqueue_lds(const std::vector<rebop_dev_job> &jobs, const std::vector<double
   if (blocks > need) blocks = need;
   HIP_TRY(rebop::launch_lds((const rebop_dev_job *)pj, (const rebop_dev_item *)pi,
                             (uint32_t)items.size(), counters, job_events, job_queue, (int)blocks,
-                            kLdsThreads, lds, (uint32_t)(xbytes / sizeof(uint32_t)), wide, stream));
+                            kLdsThreads, lds, (uint32_t)(xbytes / sizeof(uint32_t)), wide, expr,
+                            stream));
   return REBOP_OK;
 }
 
@@ -705,25 +777,25 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
   // kernel choice: LDS when asked; the specialised kernel when asked or (AUTO)
   // when the network is small enough; else the data-driven register kernel
   // when it fits, else the LDS kernel
-  const int variant = kernel & 0xF;
-  if (specialised_only(g)) {
-    // rate expressions and define_system arithmetic are compiled into the
-    // specialised kernel only
-    if (variant == REBOP_KERNEL_LDS)
-      return fail(REBOP_ERR_UNSUPPORTED,
-                  g->system ? "define_system networks run on the specialised kernels only"
-                            : "networks with rate expressions run on the specialised kernels only");
-    kernel |= REBOP_KERNEL_SPECIALIZED;
+  int variant = kernel & 0xF;
+  if ((g->system || g->n_expr) && variant != REBOP_KERNEL_LDS &&
+      !(kernel & REBOP_KERNEL_SPECIALIZED)) {
+    // define_system arithmetic and rate expressions: compiled into the
+    // specialised kernel when the network fits one, else run by the LDS
+    // kernel (never the data-driven register kernels)
+    if (jit_eligible(g)) {
+      kernel |= REBOP_KERNEL_SPECIALIZED;
+    } else {
+      kernel = (kernel & ~0xF) | REBOP_KERNEL_LDS;
+      variant = REBOP_KERNEL_LDS;
+    }
   }
-  const bool fits_regs = pick_reg_shape(S, R) >= 0;
+  const bool fits_regs = !g->n_expr && !g->system && pick_reg_shape(S, R) >= 0;
   bool spec = false, use_lds = variant == REBOP_KERNEL_LDS;
   if (!use_lds && (kernel & REBOP_KERNEL_SPECIALIZED)) {
     if (!jit_eligible(g))
       return fail(REBOP_ERR_UNSUPPORTED,
-                  specialised_only(g)
-                      ? "network with rate expressions or define_system semantics too large for "
-                        "a specialised kernel (S <= 32, R <= 64)"
-                      : "network too large for a specialised kernel");
+                  "network too large for a specialised kernel (S <= 32, R <= 64)");
     rc = prepare_jit(g, run.draw_cap != 0);
     if (rc) return rc;
     spec = true;
@@ -1458,11 +1530,6 @@ int rebop_run_jobs_device(const rebop_job *jobs, uint32_t n_jobs, void *stream_,
     if (J.nb_steps == 0 || J.nb_steps >= (1ull << 31))
       return fail(REBOP_ERR_VALUE, "job " + std::to_string(j) + ": nb_steps must be in 1 .. 2^31 - 1");
     rebop_gillespie *g = const_cast<rebop_gillespie *>(J.net);
-    if (specialised_only(g))
-      return fail(REBOP_ERR_UNSUPPORTED, "job " + std::to_string(j) +
-                                             ": networks with rate expressions or define_system "
-                                             "semantics run on the specialised kernels, not in "
-                                             "fused batches");
     if (!J.d_x0) {
       if (int e = check_x0(g->x0.data(), g->x0.size())) return e;
     }

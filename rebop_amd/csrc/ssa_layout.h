@@ -128,6 +128,7 @@ typedef struct {
  */
 #define REBOP_JOB_LINEAR 1u /* cumulative sum may be non-monotone: full linear search */
 #define REBOP_JOB_SLOW 2u   /* some reaction's factors are not all in its record */
+#define REBOP_JOB_SYSTEM 4u /* define_system semantics: term products, first-match choice */
 #define REBOP_LDS_LIMIT 1073741824 /* |count| <= 2^30 in the narrow (int32) LDS kernel */
 
 /*
@@ -183,6 +184,19 @@ typedef struct {
   const uint64_t *rec_offsets;
   double *rec_times;
   int64_t *rec_states;
+  /* Rate::Expr reactions (NULL: the job has none): reaction r's postfix
+   * program is expr_code[expr_begin[r] .. expr_begin[r + 1]) (rebop_expr.h;
+   * empty for a mass-action reaction), its constant tokens index expr_const */
+  const uint32_t *expr_begin; /* [R+1] */
+  const uint32_t *expr_code;
+  const double *expr_const;
+  /* define_system reactions (NULL: not a define_system network): reaction
+   * r's propensity is k[r] times the products lma(m, x[s]) over its reactant
+   * terms sys_species / sys_mult [sys_begin[r] .. sys_begin[r + 1]) in
+   * written order */
+  const uint32_t *sys_begin; /* [R+1] */
+  const uint32_t *sys_species;
+  const uint32_t *sys_mult;
 } rebop_dev_job;
 
 /* a wave's unit of work: trajectories first .. first+count-1 of one job */

@@ -43,6 +43,7 @@
 
 #include <type_traits>
 
+#include "rebop_expr.h"
 #include "rebop_rng.h"
 #include "ssa_lane.h"
 #include "ssa_layout.h"
@@ -51,7 +52,7 @@
 
 namespace rebop {
 
-template <int MB, bool WIDE>
+template <int MB, bool WIDE, bool EXPR>
 struct LdsNet {
   using CountT = typename std::conditional<WIDE, double, int32_t>::type;
   static constexpr int MS = 1;  // counts are in LDS, not in x[]
@@ -71,6 +72,15 @@ struct LdsNet {
   const uint32_t *jb;
   const uint32_t *js;
   const int32_t *jd;
+  // Rate::Expr programs (EXPR kernels only)
+  const uint32_t *eb;
+  const uint32_t *ecode;
+  const double *econst;
+  // define_system reactant terms (NULL: Gillespie semantics) and its choice rule
+  const uint32_t *sb;
+  const uint32_t *ssp;
+  const uint32_t *smul;
+  bool job_system;
   // this lane's column of counts
   static constexpr uint32_t kBL = 64; // lanes per workgroup (one wave)
   CountT *xl;
@@ -121,8 +131,66 @@ struct LdsNet {
   }
   __device__ __forceinline__ bool in_range(const double (&)[MS]) const { return !bad; }
 
-  // Rate::rate of reaction r from the global factor lists (src/gillespie.rs:71-90)
+  // Expr::eval (src/expr.rs:24-38) of a postfix program (rebop_expr.h):
+  // counts enter as f64, the operations in the program's order.  The host
+  // checked that the program never underflows its stack nor exceeds
+  // REBOP_EXPR_MAX_DEPTH values and leaves exactly one.
+  __device__ double expr_rate(uint32_t b, uint32_t e) const {
+    double st[REBOP_EXPR_MAX_DEPTH];
+    uint32_t sp = 0;
+    for (uint32_t i = b; i < e; ++i) {
+      const uint32_t tok = ecode[i];
+      const uint32_t op = REBOP_EXPR_OPCODE(tok), arg = REBOP_EXPR_ARG(tok);
+      if (op == REBOP_OP_CONST) {
+        st[sp++] = econst[arg];
+      } else if (op == REBOP_OP_SPECIES) {
+        st[sp++] = fcnt(arg);
+      } else if (op == REBOP_OP_NEG) {
+        st[sp - 1] = -st[sp - 1];
+      } else if (op == REBOP_OP_EXP) {
+        st[sp - 1] = rebop_exp64(st[sp - 1]);
+      } else {
+        const double y = st[--sp], x = st[sp - 1];
+        double v;
+        switch (op) {
+        case REBOP_OP_ADD: v = x + y; break;
+        case REBOP_OP_SUB: v = x - y; break;
+        case REBOP_OP_MUL: v = x * y; break;
+        case REBOP_OP_DIV: v = x / y; break;
+        case REBOP_OP_POW: v = rebop_pow64(x, y); break;
+        case REBOP_OP_MAX: v = rebop_fmax64(x, y); break;
+        default: v = rebop_fmin64(x, y); break;
+        }
+        st[sp - 1] = v;
+      }
+    }
+    return st[0];
+  }
+
+  // _rate_lma! of a define_system term (src/gillespie_macro.rs:128-142):
+  // x (x-1) ... (x-m+1) as a wrapping isize product, converted once; the
+  // multiplicity 0 of `0 A` still gives x (the macro's loop 1..0 is empty)
+  __device__ __forceinline__ double system_lma(uint32_t s, uint32_t m) const {
+    const int64_t x = (int64_t)cnt(s);
+    uint64_t p = (uint64_t)x;
+    for (uint32_t i = 1; i < m; ++i) p = p * (uint64_t)(x - (int64_t)i);
+    return (double)(int64_t)p;
+  }
+
+  // Rate::rate of reaction r from the global factor lists (src/gillespie.rs:71-90),
+  // its rate expression, or its define_system product (gillespie_macro.rs:108:
+  // the rate times the terms' products, left to right)
   __device__ __forceinline__ double rate_slow(uint32_t r) const {
+    if constexpr (EXPR) {
+      const uint32_t b = eb[r], e = eb[r + 1];
+      if (b != e) return expr_rate(b, e);
+    }
+    if (sb) {
+      double a = k[r];
+      const uint32_t te = sb[r + 1];
+      for (uint32_t i = sb[r]; i < te; ++i) a = a * system_lma(ssp[i], smul[i]);
+      return a;
+    }
     double a = k[r];
     const uint32_t fe = fb[r + 1];
     for (uint32_t f = fb[r]; f < fe; ++f) a = a * (fcnt(fs[f]) - fc[f]);
@@ -232,6 +300,7 @@ struct LdsNet {
   }
 
   __device__ __forceinline__ uint32_t choose(double (&)[MS], double (&cum)[MR], double chosen) const {
+    if (job_system) return choose_first_above(cum, chosen);
     if (linear) {
       // choose_cumrate_sum as defined: count every r with cum_r < chosen
       double acc = 0.0;
@@ -263,6 +332,38 @@ struct LdsNet {
     return r < re ? r : re - 1u;
   }
 
+  // _choice! (src/gillespie_macro.rs:146-170): the first r with chosen <
+  // cum_r, strict, and R (no reaction) when there is none
+  __device__ uint32_t choose_first_above(const double (&cum)[MR], double chosen) const {
+    if (linear) {
+      double acc = 0.0;
+      for (uint32_t r = 0; r < R; ++r) {
+        acc = acc + rate(r);
+        if (chosen < acc) return r;
+      }
+      return R;
+    }
+    // non-decreasing sums: the first block whose end sum exceeds chosen
+    uint32_t b = nblk;
+    double start = 0.0;
+#pragma unroll
+    for (int q = MB - 1; q >= 0; --q) {
+      if ((uint32_t)q < nblk && chosen < cum[q]) {
+        b = (uint32_t)q;
+        start = q > 0 ? cum[q - 1] : 0.0;
+      }
+    }
+    if (b >= nblk) return R;
+    uint32_t r = b * B;
+    const uint32_t re = (r + B < R) ? r + B : R;
+    double acc = start;
+    for (; r < re; ++r) {
+      acc = acc + rate(r);
+      if (chosen < acc) return r;
+    }
+    return R; // unreachable: the block's end sum exceeds chosen
+  }
+
   __device__ __forceinline__ void add_count(uint32_t s, int32_t d) {
     if constexpr (WIDE) {
       const double v = cnt(s) + (double)d; // exact: |x| <= 2^52, |d| < 2^23
@@ -277,8 +378,10 @@ struct LdsNet {
     }
   }
 
-  // Jump::affect (src/gillespie.rs:142-152), sparse form
+  // Jump::affect (src/gillespie.rs:142-152), sparse form; kk >= R (a
+  // define_system choice that matched no reaction) changes nothing
   __device__ __forceinline__ void affect(double (&)[MS], uint32_t kk) {
+    if (kk >= R) return;
     const uint4 j = *(const uint4 *)(jrec + (size_t)kk * REBOP_JREC_WORDS);
     if (rec[(size_t)kk * REBOP_REC_WORDS + 3] & REBOP_REC_SLOW_JUMP) {
       const uint32_t e1 = jb[kk + 1];
@@ -294,7 +397,7 @@ struct LdsNet {
   }
 };
 
-template <int MB, bool WIDE>
+template <int MB, bool WIDE, bool EXPR>
 __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__restrict__ jobs,
                                                       const rebop_dev_item *__restrict__ items,
                                                       uint32_t n_items,
@@ -331,7 +434,7 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
       *(uint4 *)(lrec + w) = *(const uint4 *)(job.rec + w);
     __syncthreads();
 
-    LdsNet<MB, WIDE> net;
+    LdsNet<MB, WIDE, EXPR> net;
     net.nS = job.S;
     net.R = job.R;
     net.B = job.block;
@@ -345,7 +448,14 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
     net.jb = job.jump_begin;
     net.js = job.jump_s;
     net.jd = job.jump_d;
-    net.xl = reinterpret_cast<typename LdsNet<MB, WIDE>::CountT *>(lds_x) + threadIdx.x;
+    net.eb = job.expr_begin;
+    net.ecode = job.expr_code;
+    net.econst = job.expr_const;
+    net.sb = job.sys_begin;
+    net.ssp = job.sys_species;
+    net.smul = job.sys_mult;
+    net.job_system = (job.flags & REBOP_JOB_SYSTEM) != 0u;
+    net.xl = reinterpret_cast<typename LdsNet<MB, WIDE, EXPR>::CountT *>(lds_x) + threadIdx.x;
     net.rec = lrec;
     net.jrec = job.jrec;
     net.linear = net.job_linear;
@@ -399,31 +509,41 @@ constexpr int kLdsCheckpoints = 32;
 
 int lds_checkpoints() { return kLdsCheckpoints; }
 
-static const void *lds_kernel_ptr(bool wide) {
-  return wide ? (const void *)&ssa_lds_kernel<kLdsCheckpoints, true>
-              : (const void *)&ssa_lds_kernel<kLdsCheckpoints, false>;
+// EXPR: the variant that also interprets rate expressions (jobs with Rate::Expr
+// reactions; the others keep a kernel without the interpreter's stack)
+static const void *lds_kernel_ptr(bool wide, bool expr) {
+  if (expr)
+    return wide ? (const void *)&ssa_lds_kernel<kLdsCheckpoints, true, true>
+                : (const void *)&ssa_lds_kernel<kLdsCheckpoints, false, true>;
+  return wide ? (const void *)&ssa_lds_kernel<kLdsCheckpoints, true, false>
+              : (const void *)&ssa_lds_kernel<kLdsCheckpoints, false, false>;
 }
 
-hipError_t lds_occupancy(int threads, size_t lds, bool wide, int *blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, lds_kernel_ptr(wide), threads,
-                                                      lds);
+hipError_t lds_occupancy(int threads, size_t lds, bool wide, bool expr, int *blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, lds_kernel_ptr(wide, expr),
+                                                      threads, lds);
 }
 
 hipError_t launch_lds(const rebop_dev_job *jobs, const rebop_dev_item *items, uint32_t n_items,
                       unsigned long long *counters, unsigned long long *job_events,
                       unsigned long long *job_queue, int blocks, int threads, size_t lds,
-                      uint32_t rec_word_off, bool wide, hipStream_t stream) {
+                      uint32_t rec_word_off, bool wide, bool expr, hipStream_t stream) {
   if (lds > 64 * 1024) {
-    const hipError_t e = hipFuncSetAttribute(lds_kernel_ptr(wide),
+    const hipError_t e = hipFuncSetAttribute(lds_kernel_ptr(wide, expr),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  if (wide)
-    hipLaunchKernelGGL((ssa_lds_kernel<kLdsCheckpoints, true>), dim3(blocks), dim3(threads), lds,
-                       stream, jobs, items, n_items, counters, job_events, job_queue, rec_word_off);
-  else
-    hipLaunchKernelGGL((ssa_lds_kernel<kLdsCheckpoints, false>), dim3(blocks), dim3(threads), lds,
-                       stream, jobs, items, n_items, counters, job_events, job_queue, rec_word_off);
+#define REBOP_LDS_LAUNCH(W, E)                                                                  \
+  hipLaunchKernelGGL((ssa_lds_kernel<kLdsCheckpoints, W, E>), dim3(blocks), dim3(threads), lds, \
+                     stream, jobs, items, n_items, counters, job_events, job_queue, rec_word_off)
+  if (expr) {
+    if (wide) REBOP_LDS_LAUNCH(true, true);
+    else REBOP_LDS_LAUNCH(false, true);
+  } else {
+    if (wide) REBOP_LDS_LAUNCH(true, false);
+    else REBOP_LDS_LAUNCH(false, false);
+  }
+#undef REBOP_LDS_LAUNCH
   return hipGetLastError();
 }
 

@@ -1,7 +1,8 @@
 """Rate expressions (Rate::Expr) on the device: network-specialised kernels
-with compiled expressions, bit for bit against the oracle's interpreter, the
-reference's arbitrary-rate tests through the Python API, and the routes that
-refuse expression networks."""
+with compiled expressions and the LDS kernel's interpreter (networks beyond
+the specialised kernels' size, fused batches), bit for bit against the
+oracle's interpreter, the reference's arbitrary-rate tests through the Python
+API, and the route that refuses expression networks (rate tables)."""
 
 from __future__ import annotations
 
@@ -102,20 +103,111 @@ def test_expression_equal_to_lma_is_bit_identical():
     assert np.array_equal(oa, ob)
 
 
-def test_expression_networks_refuse_other_routes():
+@pytest.mark.parametrize("name,net,tmax,nb,n", [
+    ("hill", HILL, 20.0, 40, 256),
+    ("ref2", REF2, 100.0, 100, 128),
+])
+def test_expression_networks_on_the_lds_kernel_bit_exact(name, net, tmax, nb, n, nthreads):
+    """The LDS kernel's interpreter (asked for explicitly on networks the
+    specialised kernel also takes) equals the oracle bit for bit."""
+    g, o = _build(net, 41)
+    out, st = _device_run(g, tmax, nb, n, K.REBOP_KERNEL_LDS, traj_begin=3)
+    assert st.kernel & K.REBOP_KERNEL_LDS
+    _, oout, _, oev, odr = o.run_grid(41, net[1], tmax, nb, n_traj=n, traj_begin=3, nthreads=nthreads)
+    assert np.array_equal(out, oout), np.argwhere(out != oout)[:5].tolist()
+    assert st.events == oev and st.draws == odr
+
+
+def _large_expression_network(S=40, R=80, seed=12):
+    """S species, R reactions (beyond the specialised kernels' 32 / 64): a
+    ring of conversions and decays, every fifth reaction with a Hill- or
+    exp-type rate expression."""
+    rng = np.random.default_rng(seed)
+    names = {f"X{i}": i for i in range(S)}
+    x0 = [int(v) for v in rng.integers(10, 60, S)]
+    rx = []
+    for r in range(R):
+        a, b = int(rng.integers(0, S)), int(rng.integers(0, S))
+        d = [0] * S
+        if a == b:
+            d[a] = -1
+        else:
+            d[a], d[b] = -1, 1
+        if r % 5 == 0:
+            text = [f"2 * X{a} / (1 + (X{b} / 30)^2)", f"0.05 * X{a} * exp(-X{b} / 80)",
+                    f"max(X{a} - 20, 0) * 0.1", f"min(X{a}, X{b}) * 0.03 + 0.01"][(r // 5) % 4]
+            rx.append((text, None, d))
+        else:
+            orders = [0] * S
+            orders[a] = 1
+            if r % 3 == 0 and a != b:
+                orders[b] = 1
+            rx.append((float(rng.uniform(0.01, 0.5)) / (10.0 if r % 3 == 0 else 1.0), orders, d))
+    # inflow so that the network does not run dry
+    for i in range(0, S, 8):
+        d = [0] * S
+        d[i] = 1
+        rx.append((5.0, [0] * S, d))
+    return (S, x0, names, rx)
+
+
+def test_large_expression_network_runs_on_the_lds_kernel(nthreads):
+    """A network with rate expressions beyond the specialised kernels' size
+    runs (AUTO) on the LDS kernel, bit for bit against the oracle."""
+    net = _large_expression_network()
+    g, o = _build(net, 77)
+    assert g.nb_species() == 40
+    n, tmax, nb = 64, 2.0, 20
+    out, st = _device_run(g, tmax, nb, n, K.REBOP_KERNEL_AUTO, traj_begin=11)
+    assert st.kernel & K.REBOP_KERNEL_LDS
+    _, oout, _, oev, odr = o.run_grid(77, net[1], tmax, nb, n_traj=n, traj_begin=11, nthreads=nthreads)
+    assert np.array_equal(out, oout), np.argwhere(out != oout)[:5].tolist()
+    assert st.events == oev and st.draws == odr
+    assert oev > 50 * n
+    # asking for the specialised kernel is refused: it does not fit one
+    import torch
+    buf = torch.zeros((nb + 1, 40, n), dtype=torch.int64, device="cuda")
+    with pytest.raises(_native.RebopError) as e:
+        g.run_device(tmax, nb, n, buf.data_ptr(), kernel=K.REBOP_KERNEL_SPECIALIZED)
+    assert e.value.code == K.REBOP_ERR_UNSUPPORTED
+
+
+def test_expression_networks_in_a_fused_batch(nthreads):
+    """Expression networks in one fused launch with a mass-action network:
+    each job equals its own oracle run."""
+    import torch
+
+    hill, _ = _build(HILL, 5)
+    big, obig = _build(_large_expression_network(seed=4), 6)
+    _, ohill = _build(HILL, 5)
+    lma = GillespieNet(np.array([300, 0]), seed=9)
+    lma.add_reaction(Rate.lma(0.1, [1, 0]), [-1, 1])
+    from oracle.pyoracle import OracleNet
+    olma = OracleNet(2)
+    olma.add_lma(0.1, [1, 0], [-1, 1])
+    specs = [(hill, ohill, HILL[1], 5, 10.0, 20, 96), (big, obig, _large_expression_network(seed=4)[1], 6, 1.0, 10, 64),
+             (lma, olma, [300, 0], 9, 5.0, 10, 64)]
+    outs = [torch.zeros((nb + 1, g.nb_species(), n), dtype=torch.int64, device="cuda")
+            for g, _, _, _, _, nb, n in specs]
+    st = run_jobs([dict(net=g, tmax=tm, nb_steps=nb, n_traj=n, out_ptr=o.data_ptr())
+                   for (g, _, _, _, tm, nb, n), o in zip(specs, outs)])
+    torch.cuda.synchronize()
+    tot_ev = 0
+    for (g, og, x0, seed, tm, nb, n), o in zip(specs, outs):
+        _, oout, _, oev, _ = og.run_grid(seed, x0, tm, nb, n_traj=n, nthreads=nthreads)
+        assert np.array_equal(o.cpu().numpy(), oout)
+        tot_ev += oev
+    assert st.events == tot_ev
+
+
+def test_expression_networks_refuse_rate_tables():
     import torch
 
     g, _ = _build(REF2, 1)
     out = torch.zeros((11, 4, 64), dtype=torch.int64, device="cuda")
-    with pytest.raises(_native.RebopError) as e:
-        g.run_device(1.0, 10, 64, out.data_ptr(), kernel=K.REBOP_KERNEL_LDS)
-    assert e.value.code == K.REBOP_ERR_UNSUPPORTED
     rates = torch.ones((2, 4), dtype=torch.float64, device="cuda")
     with pytest.raises(_native.RebopError) as e:
         g.run_sweep_device(2, 32, 1.0, 10, out.data_ptr(), rates_ptr=rates.data_ptr())
-    assert e.value.code == K.REBOP_ERR_UNSUPPORTED
-    with pytest.raises(_native.RebopError) as e:
-        run_jobs([dict(net=g, tmax=1.0, nb_steps=10, n_traj=64, out_ptr=out.data_ptr())])
     assert e.value.code == K.REBOP_ERR_UNSUPPORTED
     # an x0-only sweep is fine: expressions need no rate table
     x0 = torch.as_tensor(np.array([[0, 500, 0, 0], [0, 800, 0, 0]]), device="cuda")

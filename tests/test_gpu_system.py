@@ -1,8 +1,9 @@
 """define_system networks on the device (src/gillespie_macro.rs semantics:
 integer mass-action products, first-match choice): bit for bit against the
 oracle's restatement over every specialised kernel variant, all-events runs,
-resumable single trajectories, sweeps; the macro's own unit tests and the
-reference's Rust examples through the Python class; the refused routes."""
+resumable single trajectories, sweeps, the LDS kernel (networks beyond the
+specialised kernels' size, fused batches); the macro's own unit tests and
+the reference's Rust examples through the Python class."""
 
 from __future__ import annotations
 
@@ -261,18 +262,108 @@ def test_same_law_as_run_time_semantics_at_scale():
         assert ks_2samp(a[-1, s], b[-1, s]).pvalue > 1e-3, s
 
 
-def test_system_refused_routes():
+@pytest.mark.parametrize("name", list(CASES))
+def test_system_on_the_lds_kernel_bit_exact(name, nthreads):
+    """The LDS kernel's define_system arithmetic (term products as wrapping
+    integers, first-match choice, `0 A`) equals the oracle bit for bit."""
     import torch
 
+    make, tmax, nb, n = CASES[name]
+    obj = make()
+    obj.seed(31)
+    g = obj.network()
+    out = torch.zeros((nb + 1, len(obj._spec.species), n), dtype=torch.int64, device="cuda")
+    st = g.run_device(tmax, nb, n, out.data_ptr(), traj_begin=2, kernel=K.REBOP_KERNEL_LDS)
+    torch.cuda.synchronize()
+    assert st.kernel & K.REBOP_KERNEL_LDS
+    _, oout, _, oev, odr = _oracle_of(obj).run_grid(31, _x0(obj), tmax, nb, n_traj=n, traj_begin=2,
+                                                    nthreads=nthreads)
+    got = out.cpu().numpy()
+    assert np.array_equal(got, oout), np.argwhere(got != oout)[:5].tolist()
+    assert st.events == oev and st.draws == odr
+
+
+def _large_system(S=36, R=72, seed=3):
+    """A define_system-form network beyond the specialised kernels' 32 / 64:
+    conversions, dimerisations (A + A), third-order terms, `0 A` catalysts
+    and sources, as (GillespieNet, OracleNet, x0)."""
+    rng = np.random.default_rng(seed)
+    x0 = [int(v) for v in rng.integers(5, 40, S)]
+    g = GillespieNet_system(x0, 13)
+    o = OracleNet.system(S)
+    for r in range(R):
+        a, b = int(rng.integers(0, S)), int(rng.integers(0, S))
+        kind = r % 6
+        if kind == 0:
+            reac, prod, k = [(a, 1)], [(b, 1)], 0.3
+        elif kind == 1:
+            reac, prod, k = [(a, 1), (a, 1)], [(b, 1)], 0.002
+        elif kind == 2:
+            reac, prod, k = [(a, 2)], [(b, 1)], 0.004
+        elif kind == 3:
+            reac, prod, k = [(a, 0), (b, 1)], [(a, 1)], 0.05
+        elif kind == 4:
+            reac, prod, k = [(a, 3)], [(b, 2)], 1e-4
+        else:
+            reac, prod, k = [], [(a, 1)], 2.0
+        g.add_system_reaction(k, reac, prod)
+        o.add_system(k, reac, prod)
+    return g, o, x0
+
+
+def GillespieNet_system(x0, seed):
+    from rebop_amd.engine import GillespieNet
+
+    return GillespieNet.system(x0, seed=seed)
+
+
+def test_large_system_network_runs_on_the_lds_kernel(nthreads):
+    import torch
+
+    g, o, x0 = _large_system()
+    n, tmax, nb = 64, 2.0, 20
+    out = torch.zeros((nb + 1, len(x0), n), dtype=torch.int64, device="cuda")
+    st = g.run_device(tmax, nb, n, out.data_ptr(), traj_begin=9)
+    torch.cuda.synchronize()
+    assert st.kernel & K.REBOP_KERNEL_LDS
+    _, oout, _, oev, odr = o.run_grid(13, x0, tmax, nb, n_traj=n, traj_begin=9, nthreads=nthreads)
+    got = out.cpu().numpy()
+    assert np.array_equal(got, oout), np.argwhere(got != oout)[:5].tolist()
+    assert st.events == oev and st.draws == odr and oev > 50 * n
+    with pytest.raises(_native.RebopError) as e:
+        g.run_device(tmax, nb, n, out.data_ptr(), kernel=K.REBOP_KERNEL_PLAIN | SPEC)
+    assert e.value.code == K.REBOP_ERR_UNSUPPORTED
+
+
+def test_system_networks_in_a_fused_batch(nthreads):
+    """define_system networks in one fused launch beside a run-time network:
+    each job equals its own oracle run."""
+    import torch
+    from rebop_amd.engine import GillespieNet, Rate
+
     d = _dimers()
-    g = d.network()
-    out = torch.zeros((11, 4, 64), dtype=torch.int64, device="cuda")
-    with pytest.raises(_native.RebopError) as e:
-        g.run_device(1.0, 10, 64, out.data_ptr(), kernel=K.REBOP_KERNEL_LDS)
-    assert e.value.code == K.REBOP_ERR_UNSUPPORTED
-    with pytest.raises(_native.RebopError) as e:
-        run_jobs([dict(net=g, tmax=1.0, nb_steps=10, n_traj=64, out_ptr=out.data_ptr())])
-    assert e.value.code == K.REBOP_ERR_UNSUPPORTED
+    d.seed(8)
+    big, obig, bx0 = _large_system(seed=5)
+    lma = GillespieNet(np.array([300, 0]), seed=9)
+    lma.add_reaction(Rate.lma(0.1, [1, 0]), [-1, 1])
+    olma = OracleNet(2)
+    olma.add_lma(0.1, [1, 0], [-1, 1])
+    specs = [(d.network(), _oracle_of(d), _x0(d), 8, 0.5, 10, 96), (big, obig, bx0, 13, 1.0, 10, 64),
+             (lma, olma, [300, 0], 9, 5.0, 10, 64)]
+    outs = [torch.zeros((nb + 1, len(x0), n), dtype=torch.int64, device="cuda")
+            for _, _, x0, _, _, nb, n in specs]
+    st = run_jobs([dict(net=g, tmax=tm, nb_steps=nb, n_traj=n, out_ptr=o.data_ptr())
+                   for (g, _, _, _, tm, nb, n), o in zip(specs, outs)])
+    torch.cuda.synchronize()
+    tot = 0
+    for (_, og, x0, seed, tm, nb, n), o in zip(specs, outs):
+        _, oout, _, oev, _ = og.run_grid(seed, x0, tm, nb, n_traj=n, nthreads=nthreads)
+        assert np.array_equal(o.cpu().numpy(), oout)
+        tot += oev
+    assert st.events == tot
+
+
+def test_system_nan_rate_absorbs():
     # NaN rate: every trajectory absorbs at once, no draw
     f = DIMERS.new()
     f.gene = 1

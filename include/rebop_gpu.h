@@ -44,7 +44,9 @@ enum rebop_status {
   REBOP_ERR_VALUE = 3,  /* invalid argument value (e.g. nb_steps == 0, counts too large) */
   REBOP_ERR_DEVICE = 4, /* HIP runtime failure, or no usable gfx950 device */
   REBOP_ERR_ALLOC = 5,  /* host or device allocation failed */
-  REBOP_ERR_RANGE = 6,  /* a count left the exactly simulated range during a run */
+  REBOP_ERR_RANGE = 6,  /* a count overflowed int64 (the reference's isize), or left
+                           |x| <= 2^52 in a resumable batch or a run with rate tables,
+                           or a total propensity was infinite */
   REBOP_ERR_UNSUPPORTED = 7, /* network shape beyond the compiled kernels */
   REBOP_ERR_CAP = 8          /* a trajectory reached the draw cap (rebop_gillespie_set_draw_cap) */
 };
@@ -81,12 +83,19 @@ enum rebop_kernel_kind {
    * (state written to device memory) to a second launch that runs them with
    * 4 lanes each, which shortens the end of the run; results are identical */
   REBOP_KERNEL_WHOLE = 128,
-  /* flag for the LDS kernel: counts as f64 (exact integers, |x| <= 2^52, the
-   * register kernels' range) instead of int32 (|x| <= 2^30; half the LDS).
-   * Without it an LDS run starts narrow unless the handle's initial counts
-   * are beyond int32, and a run whose counters are read (stats non-NULL, or
-   * events_per_job) that meets a count beyond int32 is rerun wide; resumable
-   * batches always run wide. */
+  /* flag for the LDS kernel: counts as int64 (the reference's isize range)
+   * instead of int32 (|x| <= 2^30; half the LDS).  Without it an LDS run
+   * starts narrow unless the handle's initial counts are beyond int32, and a
+   * run whose counters are read (stats non-NULL, or events_per_job) that
+   * meets a count beyond int32 is rerun wide; resumable batches always run
+   * wide (their state arrays are f64: |x| <= 2^52).
+   *
+   * The register and specialised kernels hold counts as f64, exact for
+   * |x| <= 2^52.  A run with shared initial counts beyond that goes to the
+   * wide LDS kernel up front; a run whose counts leave it (per-trajectory
+   * initial counts, or growth) is rerun there when its counters are read
+   * (stats non-NULL), with every output rewritten; with stats NULL the
+   * launch is asynchronous and such a run's outputs are not exact. */
   REBOP_KERNEL_WIDE = 256
 };
 

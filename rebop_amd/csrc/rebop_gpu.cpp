@@ -170,6 +170,9 @@ std::map<std::pair<int, hipStream_t>, JobBuffers> *g_job_bufs =
 
 // finish_stats: a narrow LDS launch saw a count beyond int32 (rerun it wide)
 constexpr int kRetryWide = -1;
+// finish_stats: a register or specialised launch saw a count beyond the f64
+// counts' exact range (rerun it on the LDS kernel's int64 counts)
+constexpr int kRetryExact = -2;
 
 // Draws per trajectory after which a run whose horizon is not finite (tmax
 // = +inf or NaN) stops a trajectory that never reaches an absorbing state,
@@ -735,9 +738,10 @@ int finish_stats(rebop_stats *stats, unsigned long long *counters, const EventPa
   if (ctr[REBOP_CTR_FLAGS] & REBOP_FLAG_INTERNAL)
     return fail(REBOP_ERR_DEVICE, "internal error: a kernel launch invariant did not hold");
   if (ctr[REBOP_CTR_FLAGS] & REBOP_FLAG_NARROW) return kRetryWide;
+  if (ctr[REBOP_CTR_FLAGS] & REBOP_FLAG_EXACT) return kRetryExact;
   if (ctr[REBOP_CTR_FLAGS] & REBOP_FLAG_RANGE)
-    return fail(REBOP_ERR_RANGE, "a count left the exactly simulated range (|x| <= 2^52), or a "
-                                 "total propensity was infinite");
+    return fail(REBOP_ERR_RANGE, "a count overflowed int64 (or, in a resumable batch, left "
+                                 "|x| <= 2^52), or a total propensity was infinite");
   if (ctr[REBOP_CTR_FLAGS] & REBOP_FLAG_CAP)
     return fail(REBOP_ERR_CAP, std::to_string(ctr[REBOP_CTR_CAPPED]) +
                                    " trajectories reached the cap of " + std::to_string(draw_cap) +
@@ -754,6 +758,7 @@ int finish_stats(rebop_stats *stats, unsigned long long *counters, const EventPa
 int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t stream,
                rebop_stats *stats, int threads = 0, double occ = 1.0,
                const rebop::Scratch *sc = nullptr) {
+  const rebop_dev_run run_in = run; // for a rerun on the LDS kernel's int64 counts
   int rc = ensure_device();
   if (rc) return rc;
   if (kernel == REBOP_KERNEL_AUTO && g->tune_kernel >= 0 && threads == 0) {
@@ -790,6 +795,15 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
       variant = REBOP_KERNEL_LDS;
     }
   }
+  // shared initial counts beyond the f64 counts' exact range (|x| <= 2^52):
+  // only the LDS kernel's int64 counts hold them
+  if (!run.x0_div && !run.st_x)
+    for (int64_t v : g->x0)
+      if (v > (int64_t)REBOP_EXACT_LIMIT || v < -(int64_t)REBOP_EXACT_LIMIT) {
+        kernel = REBOP_KERNEL_LDS | REBOP_KERNEL_WIDE;
+        variant = REBOP_KERNEL_LDS;
+        break;
+      }
   const bool fits_regs = !g->n_expr && !g->system && pick_reg_shape(S, R) >= 0;
   bool spec = false, use_lds = variant == REBOP_KERNEL_LDS;
   if (!use_lds && (kernel & REBOP_KERNEL_SPECIALIZED)) {
@@ -846,7 +860,7 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
     }
     // int32 counts (narrow) unless asked for, a resumable batch (whose state a
     // rerun could not restore), or shared initial counts beyond int32; a
-    // narrow run that meets a count beyond int32 is rerun wide (f64 counts)
+    // narrow run that meets a count beyond int32 is rerun wide (int64 counts)
     // when its counters are read (stats)
     bool wide = (kernel & REBOP_KERNEL_WIDE) || run.mode == REBOP_MODE_TARGET;
     if (!run.x0_div)
@@ -863,6 +877,7 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
       if (ev.e1) HIP_TRY(hipEventRecord(ev.e1, stream));
       rc = finish_stats(stats, ctrs, ev, REBOP_KERNEL_LDS | (wide ? REBOP_KERNEL_WIDE : 0), stream,
                         run.draw_cap);
+      if (rc == kRetryExact) return fail(REBOP_ERR_DEVICE, "internal error: the LDS kernel flagged f64");
       if (rc != kRetryWide) return rc;
       if (wide) return fail(REBOP_ERR_DEVICE, "internal error: the wide LDS kernel flagged int32");
       wide = true;
@@ -982,7 +997,19 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
                                      g_dev.cus, stream));
   if (ev.e1) HIP_TRY(hipEventRecord(ev.e1, stream));
 
-  if (stats) return finish_stats(stats, run.counters, ev, kind, stream, run.draw_cap);
+  if (stats) {
+    rc = finish_stats(stats, run.counters, ev, kind, stream, run.draw_cap);
+    if (rc != kRetryExact) return rc;
+    // a count left |x| <= 2^52: the f64 counts of these kernels no longer
+    // hold it exactly, the LDS kernel's int64 counts do (the rerun rewrites
+    // every output).  Rate tables have no LDS kernel, and a resumable batch
+    // has already advanced its state in place.
+    if (run.rates || run.st_x)
+      return fail(REBOP_ERR_RANGE, std::string("a count left the exactly simulated range |x| <= "
+                                               "2^52 of ") +
+                                       (run.rates ? "a run with rate tables" : "a resumable batch"));
+    return launch_run(g, run_in, REBOP_KERNEL_LDS | REBOP_KERNEL_WIDE, stream, stats, 0, 1.0, sc);
+  }
   return REBOP_OK;
 }
 
@@ -1007,9 +1034,6 @@ int run_device_scratch(const rebop_gillespie *gc, double tmax, uint64_t nb_steps
     return fail(REBOP_ERR_VALUE, "nb_steps must be > 0 for a grid run");
   // sample positions are 31-bit in the kernels
   if (nb_steps >= (1ull << 31)) return fail(REBOP_ERR_VALUE, "nb_steps must be below 2^31");
-  if (!d_x0) {
-    if (int rc = check_x0(g->x0.data(), g->x0.size())) return rc;
-  }
   if (stats) std::memset(stats, 0, sizeof *stats);
   int rc = ensure_device();
   if (rc) return rc;
@@ -1488,9 +1512,6 @@ int rebop_run(const rebop_gillespie *gc, double tmax, uint64_t nb_steps, uint64_
   if (nb_steps == 0) return fail(REBOP_ERR_VALUE, "nb_steps must be > 0 for a grid run");
   if (nb_steps >= (1ull << 31)) return fail(REBOP_ERR_VALUE, "nb_steps must be below 2^31");
   const uint32_t S = gc->S;
-  if (x0) {
-    if (int rc = check_x0(x0, (size_t)n_traj * S)) return rc;
-  }
   if (times)
     for (uint64_t i = 0; i <= nb_steps; ++i) times[i] = tmax * (double)i / (double)nb_steps;
   int rc = ensure_device();
@@ -1530,9 +1551,6 @@ int rebop_run_jobs_device(const rebop_job *jobs, uint32_t n_jobs, void *stream_,
     if (J.nb_steps == 0 || J.nb_steps >= (1ull << 31))
       return fail(REBOP_ERR_VALUE, "job " + std::to_string(j) + ": nb_steps must be in 1 .. 2^31 - 1");
     rebop_gillespie *g = const_cast<rebop_gillespie *>(J.net);
-    if (!J.d_x0) {
-      if (int e = check_x0(g->x0.data(), g->x0.size())) return e;
-    }
     rc = upload_net(g);
     if (rc) return rc;
     rebop_dev_job d = g->job;
@@ -1572,7 +1590,7 @@ int rebop_run_jobs_device(const rebop_job *jobs, uint32_t n_jobs, void *stream_,
   td.jobs = &jb->djobs;
   td.items = &jb->ditems;
   // int32 counts unless some job's shared initial counts are beyond int32; a
-  // run that meets a count beyond int32 is rerun with f64 counts when its
+  // run that meets a count beyond int32 is rerun with int64 counts when its
   // counters are read (stats or events_per_job)
   bool wide = false;
   for (uint32_t j = 0; j < n_jobs; ++j)
@@ -1612,7 +1630,6 @@ int rebop_run_all_events(const rebop_gillespie *gc, double tmax, uint64_t traj_b
   if (int rc = check_handle(gc, "gillespie")) return rc;
   if (!n_records && n_traj) return fail(REBOP_ERR_VALUE, "n_records is NULL");
   rebop_gillespie *g = const_cast<rebop_gillespie *>(gc);
-  if (int rc = check_x0(g->x0.data(), g->x0.size())) return rc;
   if (stats) std::memset(stats, 0, sizeof *stats);
   int rc = ensure_device();
   if (rc) return rc;
@@ -1682,7 +1699,6 @@ int rebop_run_all_events_device(const rebop_gillespie *gc, double tmax, uint64_t
   if (want && n_traj && (!d_offsets || !d_times || !d_states))
     return fail(REBOP_ERR_VALUE, "pass 2 needs d_offsets, d_times and d_states");
   rebop_gillespie *g = const_cast<rebop_gillespie *>(gc);
-  if (int rc = check_x0(g->x0.data(), g->x0.size())) return rc;
   if (stats) std::memset(stats, 0, sizeof *stats);
   int rc = ensure_device();
   if (rc) return rc;

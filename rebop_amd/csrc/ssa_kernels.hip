@@ -21,7 +21,8 @@ struct GenericNet {
   static constexpr int MS = MS_;
   static constexpr int MR = MR_;
   static constexpr bool kCap = CAP;
-  static constexpr uint32_t kRangeFlag = REBOP_FLAG_RANGE;
+  // f64 counts: a run leaving |x| <= 2^52 is rerun on the int64 LDS kernel
+  static constexpr uint32_t kRangeFlag = REBOP_FLAG_EXACT;
   rebop_dev_net net;
   const double *lds_diff; // [R][S]
   double kr[MR];          // this trajectory's rate constants

@@ -31,7 +31,8 @@
 // that the compiler keeps them in VGPRs even when x is indexed by a
 // wave-uniform runtime species index):
 //   static constexpr bool kCap;  compile the draw-cap checks in
-//   static constexpr uint32_t kRangeFlag;  raised when in_range fails
+//   static constexpr uint32_t kRangeFlag;  raised when in_range fails (REBOP_FLAG_EXACT
+//                                          marks f64 counts, checked every 2^31 draws)
 //   uint32_t S() const;
 //   void start(const rebop_dev_run &run, uint64_t j);    per-trajectory setup (rates)
 //   void load_x0(X &x, const int64_t *src);                counts from an int64 row
@@ -262,7 +263,8 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
         orec += Sn;
       }
       if (!net.in_range(x)) acc.flags |= Net::kRangeFlag;
-      if (draw - mark >= REBOP_DRAWS_PER_CHECK) acc.flags |= REBOP_FLAG_RANGE;
+      if (Net::kRangeFlag == REBOP_FLAG_EXACT && draw - mark >= REBOP_DRAWS_PER_CHECK)
+        acc.flags |= REBOP_FLAG_EXACT; // f64 counts: too many draws between two checks
       mark = draw;
       t = T;
       i += 1;
@@ -458,7 +460,8 @@ __device__ __forceinline__ LaneTotals lane_loop_group(Net &net, const rebop_dev_
             orec += Sn;
           }
           if (!net.in_range(x)) acc.flags |= Net::kRangeFlag;
-          if (draw - mark >= REBOP_DRAWS_PER_CHECK) acc.flags |= REBOP_FLAG_RANGE;
+          if (Net::kRangeFlag == REBOP_FLAG_EXACT && draw - mark >= REBOP_DRAWS_PER_CHECK)
+            acc.flags |= REBOP_FLAG_EXACT;
           mark = draw;
           t = T;
           i += 1;

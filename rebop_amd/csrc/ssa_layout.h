@@ -15,15 +15,18 @@
  * (n+1-e), ..., n, i.e. c = e-1, ..., 0.  Evaluation order is ascending
  * species for a dense Rate::LMA and the stored pair order for Rate::LMASparse.
  *
- * Counts are carried in FP64 registers.  Integers up to 2^53 are exact in
- * binary64, so x - c, x + diff and the products see exactly the values the
- * reference's int64 counts produce (its `x as f64` conversions are exact in
- * that range).  The host rejects |x0| > 2^52 and |diff| > 2^20; the kernel
- * checks |x| <= 2^52 at every sample and that fewer than 2^31 draws happened
- * since the previous check (so no count can have passed 2^53 in between),
- * and raises REBOP_FLAG_RANGE otherwise.  An infinite total propensity (an
- * overflowing product or rate expression) raises it too: time would not
- * advance, so that trajectory is run out as an absorbing state instead.
+ * The register and specialised kernels carry counts in FP64 registers.
+ * Integers up to 2^53 are exact in binary64, so x - c, x + diff and the
+ * products see exactly the values the reference's int64 counts produce (its
+ * `x as f64` conversions are exact in that range).  The host rejects
+ * |diff| > 2^20; these kernels check |x| <= 2^52 at every sample and that
+ * fewer than 2^31 draws happened since the previous check (so no count can
+ * have passed 2^53 in between), and raise REBOP_FLAG_EXACT otherwise: the
+ * host then reruns the run on the LDS kernel's int64 counts, which check
+ * every addition for int64 overflow (REBOP_FLAG_RANGE).  An infinite total
+ * propensity (an overflowing product or rate expression) raises
+ * REBOP_FLAG_RANGE too: time would not advance, so that trajectory is run
+ * out as an absorbing state instead.
  */
 #ifndef REBOP_SSA_LAYOUT_H
 #define REBOP_SSA_LAYOUT_H
@@ -43,6 +46,9 @@
 #define REBOP_FLAG_INTERNAL 2u /* a launch invariant broke (reported as a device error) */
 #define REBOP_FLAG_CAP 4u      /* a trajectory reached the draw cap (watchdog) */
 #define REBOP_FLAG_NARROW 8u   /* a count left the int32 range of the narrow LDS kernel */
+#define REBOP_FLAG_EXACT 16u   /* a count left the f64-exact range (|x| <= 2^52) of the
+                                  register or specialised kernels: the host reruns such a
+                                  run on the LDS kernel's int64 counts */
 
 /* counters[] slots */
 #define REBOP_CTR_EVENTS 0

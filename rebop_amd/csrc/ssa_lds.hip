@@ -4,8 +4,9 @@
 // every access -- uniform or lane-varying species index -- maps lane l to
 // the same banks whatever the species: conflict-free.  The narrow kernel
 // keeps them as int32 (|x| <= 2^30, half the LDS, twice the workgroups per
-// CU); the wide one as doubles holding exact integers (|x| <= 2^52, the
-// register kernels' range), for populations beyond int32.  The network (a "job") is read at
+// CU); the wide one as int64, the reference's isize, for populations beyond
+// int32 and beyond the f64 counts of the register and specialised kernels
+// (|x| <= 2^52), with every addition checked for overflow.  The network (a "job") is read at
 // wave-uniform addresses through the scalar cache, except in the per-lane
 // block scan below.
 //
@@ -54,12 +55,15 @@ namespace rebop {
 
 template <int MB, bool WIDE, bool EXPR>
 struct LdsNet {
-  using CountT = typename std::conditional<WIDE, double, int32_t>::type;
+  // counts: int32 (narrow) or the reference's full int64 (wide)
+  using CountT = typename std::conditional<WIDE, int64_t, int32_t>::type;
   static constexpr int MS = 1;  // counts are in LDS, not in x[]
   static constexpr bool kCap = true; // the check costs nothing next to a pass over R reactions
-  // the narrow kernel's range is the smaller one: the host reruns such a run wide
+  // the narrow kernel's range is the smaller one: the host reruns such a run
+  // wide; the wide one flags an int64 overflow, or a count beyond the f64
+  // state arrays' exact range in a resumable run
   static constexpr uint32_t kRangeFlag = WIDE ? REBOP_FLAG_RANGE : REBOP_FLAG_NARROW;
-  static constexpr double kLimit = WIDE ? REBOP_EXACT_LIMIT : (double)REBOP_LDS_LIMIT;
+  static constexpr int64_t kLimit = WIDE ? (int64_t)REBOP_EXACT_LIMIT : (int64_t)REBOP_LDS_LIMIT;
   static constexpr int MR = MB; // cum[] holds the block checkpoints
   // job fields (wave-uniform)
   uint32_t nS, R, B, nblk;
@@ -87,6 +91,9 @@ struct LdsNet {
   // the job's reaction records in LDS, jump records in global memory
   const uint32_t *rec;
   const uint32_t *jrec;
+  // the run keeps its state in f64 arrays (resumable batch): counts must stay
+  // within |x| <= 2^52 there
+  bool state_f64;
   // per-lane flags
   bool linear;
   bool bad;
@@ -101,23 +108,37 @@ struct LdsNet {
     return rebop::grid_time(run, i);
   }
 
-  __device__ __forceinline__ void set_count(uint32_t s, double v) {
-    if (!(__builtin_fabs(v) <= kLimit)) {
+  // whether a count is inside this kernel's range
+  __device__ __forceinline__ bool fits(int64_t v) const {
+    if constexpr (WIDE) return !state_f64 || (v <= kLimit && v >= -kLimit);
+    else return v <= kLimit && v >= -kLimit;
+  }
+  __device__ __forceinline__ void set_count(uint32_t s, int64_t v) {
+    if (!fits(v)) {
       bad = true;
-      v = 0.0;
+      v = 0;
     }
     cnt(s) = (CountT)v;
-    if (v < 0.0) linear = true;
+    if (v < 0) linear = true;
   }
   __device__ __forceinline__ void load_x0(double (&)[MS], const int64_t *src) {
     linear = job_linear;
     bad = false;
-    for (uint32_t s = 0; s < nS; ++s) set_count(s, (double)src[s]);
+    for (uint32_t s = 0; s < nS; ++s) set_count(s, src[s]);
   }
+  // a resumable state: exact integers in f64 (|x| <= 2^52, as stored)
   __device__ __forceinline__ void load_state(double (&)[MS], const double *st, uint64_t stride) {
     linear = job_linear;
     bad = false;
-    for (uint32_t s = 0; s < nS; ++s) set_count(s, st[(uint64_t)s * stride]);
+    for (uint32_t s = 0; s < nS; ++s) {
+      const double v = st[(uint64_t)s * stride];
+      if (__builtin_fabs(v) <= REBOP_EXACT_LIMIT) {
+        set_count(s, (int64_t)v);
+      } else {
+        bad = true;
+        cnt(s) = 0;
+      }
+    }
   }
   __device__ __forceinline__ void store_state(const double (&)[MS], double *st, uint64_t stride) const {
     for (uint32_t s = 0; s < nS; ++s) st[(uint64_t)s * stride] = fcnt(s);
@@ -193,18 +214,16 @@ struct LdsNet {
     }
     double a = k[r];
     const uint32_t fe = fb[r + 1];
-    for (uint32_t f = fb[r]; f < fe; ++f) a = a * (fcnt(fs[f]) - fc[f]);
+    for (uint32_t f = fb[r]; f < fe; ++f) a = a * (double)((int64_t)cnt(fs[f]) - (int64_t)fc[f]);
     return a;
   }
 
   // factor (x[s] - c) of a record word whose count x is read, or exactly
-  // 1.0 when absent.  The narrow kernel takes the difference in int32
-  // (|x| <= 2^30, c < 4096) and converts once; the wide one subtracts exact
-  // integers in f64 (|x| <= 2^52): either way the reference's integer
-  // difference, converted
+  // 1.0 when absent: the reference's integer difference (int32 in the narrow
+  // kernel, |x| <= 2^30 and c < 4096; int64 in the wide one), converted once
   __device__ __forceinline__ double factor_val(uint32_t w, CountT x) const {
     if constexpr (WIDE) {
-      return (w & REBOP_REC_PRESENT) ? x - (double)((w >> 16) & 0xfffu) : 1.0;
+      return (w & REBOP_REC_PRESENT) ? (double)(x - (int64_t)((w >> 16) & 0xfffu)) : 1.0;
     } else {
       const int32_t v = x - (int32_t)((w >> 16) & 0xfffu);
       return (double)((w & REBOP_REC_PRESENT) ? v : 1);
@@ -366,10 +385,11 @@ struct LdsNet {
 
   __device__ __forceinline__ void add_count(uint32_t s, int32_t d) {
     if constexpr (WIDE) {
-      const double v = cnt(s) + (double)d; // exact: |x| <= 2^52, |d| < 2^23
+      int64_t v;
+      if (__builtin_add_overflow(cnt(s), (int64_t)d, &v)) bad = true; // the isize would overflow
       cnt(s) = v;
-      if (v < 0.0) linear = true;
-      if (!(__builtin_fabs(v) <= kLimit)) bad = true;
+      if (v < 0) linear = true;
+      if (!fits(v)) bad = true;
     } else {
       const int32_t v = (int32_t)((uint32_t)cnt(s) + (uint32_t)d);
       cnt(s) = v;
@@ -455,6 +475,7 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
     net.ssp = job.sys_species;
     net.smul = job.sys_mult;
     net.job_system = (job.flags & REBOP_JOB_SYSTEM) != 0u;
+    net.state_f64 = job.st_x != nullptr;
     net.xl = reinterpret_cast<typename LdsNet<MB, WIDE, EXPR>::CountT *>(lds_x) + threadIdx.x;
     net.rec = lrec;
     net.jrec = job.jrec;

@@ -179,11 +179,17 @@ def test_absorbing_and_empty_networks():
     assert (out[-1] == 0).all() and st.events == 300
 
 
-def test_range_guard_rejects_huge_counts():
-    g = GillespieNet([2**53, 0], seed=1)
+def test_huge_counts_run_on_int64_counts():
+    """2^53 + 1 is beyond the f64 counts of the register and specialised
+    kernels: the run takes the LDS kernel's int64 counts (about 9e15 events
+    per unit time, so a horizon of 2e-15)."""
+    g = GillespieNet([2**53 + 1, 0], seed=1)
     g.add_reaction(Rate.lma(1.0, [1, 0]), [-1, 1])
-    with pytest.raises(_native.RebopValueError):
-        g.run(1.0, 2, n_traj=4)
+    _, out, st = g.run(2e-15, 2, n_traj=4, with_stats=True)
+    assert st.kernel == _native.REBOP_KERNEL_LDS | _native.REBOP_KERNEL_WIDE
+    o = build_oracle((2, [(1.0, [1, 0], [-1, 1])], [2**53 + 1, 0]))
+    _, oout, _, _, _ = o.run_grid(1, [2**53 + 1, 0], 2e-15, 2, n_traj=4)
+    assert np.array_equal(out, oout) and out[-1, 1].min() > 0
 
 
 def test_python_api_sir_invariants():

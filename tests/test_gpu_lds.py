@@ -6,7 +6,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from rebop_amd import GillespieNet, Rate, _native, workloads
+from rebop_amd import _native, workloads
 from rebop_amd.engine import run_jobs
 from tests.networks import build_device, build_oracle, dimers, michaelis_menten, sir, synthetic
 
@@ -108,11 +108,12 @@ def test_lds_per_trajectory_x0_and_range_guard(nthreads):
     o = build_oracle(spec, sparse=True)
     _, oout, _, _, _ = o.run_grid(4, x0, 0.5, 8, n_traj=200, x0_per_traj=True, nthreads=nthreads)
     assert np.array_equal(out, oout)
-    # counts beyond the exactly simulated range (2^52) are refused
-    big = GillespieNet([2**53, 0] + [0] * 38, seed=1)
-    big.add_reaction(Rate.lma(1.0, [1] + [0] * 39), [-1, 1] + [0] * 38)
-    with pytest.raises(_native.RebopValueError):
-        big.run(1.0, 2, n_traj=4)
+    # counts beyond the f64 kernels' exact range (2^52) run on int64 counts
+    big = (40, [(1.0, [1] + [0] * 39, [-1, 1] + [0] * 38)], [2**53 + 1, 0] + [0] * 38)
+    times, bout, bst = build_device(big, 1).run(2e-15, 2, n_traj=4, with_stats=True)
+    assert bst.kernel == LDS | _native.REBOP_KERNEL_WIDE
+    _, boout, _, _, _ = build_oracle(big, sparse=True).run_grid(1, big[2], 2e-15, 2, n_traj=4)
+    assert np.array_equal(bout, boout) and bout[-1, 1].min() > 0
 
 
 def _wide_spec(x0a):
@@ -135,7 +136,7 @@ def _wide_spec(x0a):
 
 @pytest.mark.parametrize("x0a", [2**31 + 5, 2**30 - 3 * 2**20])
 def test_lds_counts_beyond_int32(x0a, nthreads):
-    """Populations beyond int32 run on the LDS kernel with f64 counts: chosen
+    """Populations beyond int32 run on the LDS kernel with int64 counts: chosen
     up front when the initial counts are beyond int32 (2^31 + 5), or by a rerun
     when a narrow run's count leaves int32 during the run (A starts at
     2^30 - 3 * 2^20 and its inflow carries it past 2^30).  Both equal the

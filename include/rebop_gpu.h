@@ -67,8 +67,8 @@ enum rebop_kernel_kind {
   REBOP_KERNEL_PLAIN = 1,      /* one trajectory per lane, lanes retire when done */
   REBOP_KERNEL_PERSISTENT = 2, /* lanes refill from an atomic trajectory queue */
   REBOP_KERNEL_LDS = 3,        /* counts in LDS: 256 bytes per species (int32; 512 with
-                                  REBOP_KERNEL_WIDE) and 16 per reaction must fit one
-                                  workgroup's LDS (160 KB) */
+                                  REBOP_KERNEL_WIDE, 128 with int16 counts) and 16 per
+                                  reaction must fit one workgroup's LDS (160 KB) */
   /* flag: the network compiled into the kernel (run-time compilation,
    * cached); combine with PLAIN or PERSISTENT */
   REBOP_KERNEL_SPECIALIZED = 16,
@@ -96,7 +96,16 @@ enum rebop_kernel_kind {
    * initial counts, or growth) is rerun there when its counters are read
    * (stats non-NULL), with every output rewritten; with stats NULL the
    * launch is asynchronous and such a run's outputs are not exact. */
-  REBOP_KERNEL_WIDE = 256
+  REBOP_KERNEL_WIDE = 256,
+  /* reported in rebop_stats.kernel: the LDS kernel ran with int16 counts
+   * (|x| <= 2^15 - 1; half the LDS of int32).  An LDS run whose counters are
+   * read starts there when the handle's initial counts fit it and the
+   * smaller LDS lets more waves share a CU (large networks: otherwise the
+   * registers bound the waves, and int32 runs as fast without the risk of a
+   * rerun); a launch that meets a larger count stops early and is rerun
+   * with int32 counts (then int64).  REBOP_LDS_COMPACT=0 in the environment
+   * never starts at int16, =1 does whenever the initial counts fit. */
+  REBOP_KERNEL_COMPACT = 512
 };
 
 /* Opaque network + initial state: rebop's `gillespie::Gillespie` (src/gillespie.rs:157-163). */

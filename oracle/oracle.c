@@ -806,6 +806,12 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
   for (int i = 0; i < 4; ++i) out[i] = r.w[i];
 }
 
+/* the per-trajectory form of the same block that the device kernels draw with */
+void orc_stream_block(uint64_t seed, uint64_t traj, uint64_t draw, uint32_t out[4]) {
+  const rebop_u32x4 r = rebop_stream_block(rebop_stream_of(seed, traj), draw);
+  for (int i = 0; i < 4; ++i) out[i] = r.w[i];
+}
+
 double orc_ln64(double x) { return rebop_ln64(x); }
 double orc_exp64(double x) { return rebop_exp64(x); }
 double orc_pow64(double x, double y) { return rebop_pow64(x, y); }

@@ -173,6 +173,7 @@ int orc_run_all_events_many(const orc_net *net, uint64_t seed, uint64_t traj_beg
 
 /* RNG primitives, exported so tests can pin them. */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void orc_stream_block(uint64_t seed, uint64_t traj, uint64_t draw, uint32_t out[4]);
 double orc_ln64(double x);
 /* the rate-expression math of rebop_expr.h */
 double orc_exp64(double x);

@@ -81,6 +81,7 @@ def load() -> ctypes.CDLL:
             [c_void_p, c_uint32, c_int, c_int, P(c_uint64), P(c_uint64), P(c_int)],
         ),
         "orc_philox4x32_10": (None, [P(c_uint32), P(c_uint32), P(c_uint32)]),
+        "orc_stream_block": (None, [c_uint64, c_uint64, c_uint64, P(c_uint32)]),
         "orc_ln64": (c_double, [c_double]),
         "orc_exp64": (c_double, [c_double]),
         "orc_pow64": (c_double, [c_double, c_double]),
@@ -344,6 +345,14 @@ def philox(ctr, key):
     k = (c_uint32 * 2)(*key)
     o = (c_uint32 * 4)()
     lib.orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def stream_block(seed, traj, draw):
+    """rebop_stream_block(rebop_stream_of(seed, traj), draw): the kernels' form."""
+    lib = load()
+    o = (c_uint32 * 4)()
+    lib.orc_stream_block(int(seed), int(traj), int(draw), o)
     return list(o)
 
 

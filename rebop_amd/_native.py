@@ -33,6 +33,7 @@ REBOP_KERNEL_GROUP2 = 32
 REBOP_KERNEL_GROUP4 = 64
 REBOP_KERNEL_WHOLE = 128
 REBOP_KERNEL_WIDE = 256
+REBOP_KERNEL_COMPACT = 512  # reported: the LDS kernel ran with int16 counts
 
 # rate-expression bytecode operations (token = op | arg << 8)
 REBOP_EXPR_CONST = 1

@@ -69,30 +69,42 @@ def embedded_headers_cpp() -> str:
     )
 
 
-def build(verbose: bool = False) -> Path:
-    """Compile every HIP source into ``librebop_gpu.so`` (always recompiles)."""
-    tmp = LIB.with_suffix(".so.tmp")
-    with tempfile.TemporaryDirectory() as td:
-        emb = Path(td) / "embedded_headers.cpp"
-        emb.write_text(embedded_headers_cpp())
-        cmd = [hipcc(), *FLAGS, "-shared", "-o", str(tmp), *[str(CSRC / s) for s in SOURCES],
-               str(emb), "-lhiprtc"]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stdout}\n{res.stderr}")
-    os.replace(tmp, LIB)
-    # the surrogate's kernels (dense layers on the matrix cores; contraction is
-    # allowed there: they are not part of the bit-exact SSA path)
-    sg_tmp = SG_LIB.with_suffix(".so.tmp")
-    sg_flags = [f for f in FLAGS if f != "-ffp-contract=off"]
-    cmd = [hipcc(), *sg_flags, "-shared", "-o", str(sg_tmp), *[str(CSRC / s) for s in SG_SOURCES]]
+def _run(cmd: list[str], verbose: bool) -> None:
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stdout}\n{res.stderr}")
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile every HIP source into ``librebop_gpu.so`` (always recompiles).
+
+    Each source is compiled to an object in its own hipcc process, several at
+    once, then the objects are linked; the surrogate's library is built
+    beside them."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    tmp = LIB.with_suffix(".so.tmp")
+    sg_tmp = SG_LIB.with_suffix(".so.tmp")
+    # the surrogate's kernels (dense layers on the matrix cores; contraction is
+    # allowed there: they are not part of the bit-exact SSA path)
+    sg_flags = [f for f in FLAGS if f != "-ffp-contract=off"]
+    jobs = min(8, os.cpu_count() or 1)
+    with tempfile.TemporaryDirectory() as td:
+        emb = Path(td) / "embedded_headers.cpp"
+        emb.write_text(embedded_headers_cpp())
+        srcs = [CSRC / s for s in SOURCES] + [emb]
+        objs = [Path(td) / f"{i}_{src.stem}.o" for i, src in enumerate(srcs)]
+        cmds = [[hipcc(), *FLAGS, "-c", "-o", str(o), str(src)] for src, o in zip(srcs, objs)]
+        cmds.append([hipcc(), *sg_flags, "-shared", "-o", str(sg_tmp),
+                     *[str(CSRC / s) for s in SG_SOURCES]])
+        with ThreadPoolExecutor(max_workers=jobs) as ex:
+            for f in [ex.submit(_run, c, verbose) for c in cmds]:
+                f.result()
+        _run([hipcc(), *FLAGS, "-shared", "-o", str(tmp), *[str(o) for o in objs], "-lhiprtc"],
+             verbose)
+    os.replace(tmp, LIB)
     os.replace(sg_tmp, SG_LIB)
     return LIB
 

@@ -39,11 +39,13 @@ hipError_t launch_reg(int c, const rebop_dev_net &net, const rebop_dev_run &run,
                       int threads, size_t lds, hipStream_t stream);
 
 int lds_checkpoints();
-hipError_t lds_occupancy(int threads, size_t lds, bool wide, bool expr, int *blocks_per_cu);
+hipError_t lds_occupancy(int threads, size_t lds, int width, bool expr, bool grec,
+                         int *blocks_per_cu);
 hipError_t launch_lds(const rebop_dev_job *jobs, const rebop_dev_item *items, uint32_t n_items,
                       unsigned long long *counters, unsigned long long *job_events,
                       unsigned long long *job_queue, int blocks, int threads, size_t lds,
-                      uint32_t rec_word_off, bool wide, bool expr, hipStream_t stream);
+                      uint32_t rec_word_off, int width, bool expr, bool grec,
+                      hipStream_t stream);
 
 } // namespace rebop
 
@@ -168,7 +170,8 @@ std::mutex g_job_mu;
 std::map<std::pair<int, hipStream_t>, JobBuffers> *g_job_bufs =
     new std::map<std::pair<int, hipStream_t>, JobBuffers>();
 
-// finish_stats: a narrow LDS launch saw a count beyond int32 (rerun it wide)
+// finish_stats: a compact or narrow LDS launch saw a count beyond int16 /
+// int32 (rerun it with the next wider counts)
 constexpr int kRetryWide = -1;
 // finish_stats: a register or specialised launch saw a count beyond the f64
 // counts' exact range (rerun it on the LDS kernel's int64 counts)
@@ -666,22 +669,103 @@ struct TableDest {
 
 // Uploads the job and item tables and enqueues the LDS kernel.
 // job_queue: one zeroed 64-bit trajectory counter per job.
+// REBOP_LDS_COMPACT: 0 never starts the LDS kernel with int16 counts, 1
+// does whenever the initial counts fit, unset (-1) where that runs more
+// waves per CU (compact_pays).
+int lds_compact_mode() {
+  const char *e = std::getenv("REBOP_LDS_COMPACT");
+  if (!e || !*e) return -1;
+  return std::strcmp(e, "0") == 0 ? 0 : 1;
+}
+
+// The LDS kernel's count width (bytes) a launch starts with: int64 when
+// asked (REBOP_KERNEL_WIDE), for a resumable batch (whose state a rerun
+// could not restore) or when the shared initial counts `x0` are beyond
+// int32; int16 when they fit it and a launch that meets a larger count can
+// be rerun (`rerun`: its counters are read) -- unless REBOP_LDS_COMPACT=0;
+// int32 otherwise (device-side initial counts, x0 NULL, included).  The
+// callers keep int16 only where it runs more waves (compact_pays).
+int lds_start_width(int kernel, uint32_t mode, bool rerun, const std::vector<int64_t> *x0) {
+  if ((kernel & REBOP_KERNEL_WIDE) || mode == REBOP_MODE_TARGET) return 8;
+  if (!x0) return 4;
+  int64_t m = 0;
+  for (int64_t v : *x0) {
+    if (v > REBOP_LDS_LIMIT || v < -REBOP_LDS_LIMIT) return 8;
+    m = std::max(m, v < 0 ? -v : v);
+  }
+  return rerun && lds_compact_mode() != 0 && m <= REBOP_LDS16_LIMIT ? 2 : 4;
+}
+
+// rebop_stats.kernel of an LDS launch of a count width
+int lds_kind(int width) {
+  return REBOP_KERNEL_LDS | (width == 8 ? REBOP_KERNEL_WIDE : width == 2 ? REBOP_KERNEL_COMPACT : 0);
+}
+
+// The LDS of one workgroup of an LDS launch: counts [S_max][64] of `width`
+// bytes (16-byte aligned), then the reaction records unless they are read in
+// place (grec).  `xbytes`: the counts' part.
+size_t lds_bytes(uint32_t smax, uint32_t rmax, int width, bool grec, size_t *xbytes = nullptr) {
+  const size_t xb = ((size_t)smax * kLdsThreads * (size_t)width + 15) & ~(size_t)15;
+  if (xbytes) *xbytes = xb;
+  return xb + (grec ? 0u : (size_t)rmax * REBOP_REC_WORDS * sizeof(uint32_t));
+}
+
+// Workgroups (waves) of the LDS kernel per CU for a shape, or 0 when its LDS
+// does not fit.
+int lds_per_cu(uint32_t smax, uint32_t rmax, int width, bool expr, bool grec) {
+  if (expr && grec) return 0;
+  const size_t lds = lds_bytes(smax, rmax, width, grec);
+  int n = 0;
+  if (lds > g_dev.lds_max ||
+      rebop::lds_occupancy(kLdsThreads, lds, width, expr, grec, &n) != hipSuccess)
+    return 0;
+  return n;
+}
+
+// Whether the records are read in place: when the LDS they would take
+// leaves fewer waves per CU (the registers bound most networks first, and
+// there the LDS copy is faster).
+bool lds_global_records(uint32_t smax, uint32_t rmax, int width, bool expr) {
+  return lds_per_cu(smax, rmax, width, expr, true) > lds_per_cu(smax, rmax, width, expr, false);
+}
+
+// The largest species and reaction counts of a set of jobs, and whether one
+// has rate expressions (the interpreting kernel).
+void job_shape(const std::vector<rebop_dev_job> &jobs, uint32_t *smax, uint32_t *rmax, bool *expr) {
+  *smax = 1;
+  *rmax = 1;
+  *expr = false;
+  for (auto &j : jobs) {
+    *smax = std::max(*smax, j.S);
+    *rmax = std::max(*rmax, j.R);
+    *expr = *expr || j.expr_begin != nullptr;
+  }
+}
+
+// Whether int16 counts let more waves share a CU than int32 counts.  Only
+// then are they worth the risk of a rerun: for most networks the registers,
+// not the LDS, bound the waves per CU, and the two widths run alike.
+bool compact_pays(const std::vector<rebop_dev_job> &jobs) {
+  uint32_t smax, rmax;
+  bool expr;
+  job_shape(jobs, &smax, &rmax, &expr);
+  auto per_cu = [&](int width) {
+    return std::max(lds_per_cu(smax, rmax, width, expr, false),
+                    lds_per_cu(smax, rmax, width, expr, true));
+  };
+  return lds_compact_mode() == 1 || per_cu(2) > per_cu(4);
+}
+
 int enqueue_lds(const std::vector<rebop_dev_job> &jobs, const std::vector<double> &cost,
                 const TableDest &dst, unsigned long long *counters,
-                unsigned long long *job_events, unsigned long long *job_queue, bool wide,
+                unsigned long long *job_events, unsigned long long *job_queue, int width,
                 hipStream_t stream) {
-  uint32_t smax = 1, rmax = 1;
-  bool expr = false; // some job has rate expressions: the interpreting kernel
-  for (auto &j : jobs) {
-    smax = std::max(smax, j.S);
-    rmax = std::max(rmax, j.R);
-    expr = expr || j.expr_begin != nullptr;
-  }
-  // counts [S_max][64] (int32, or f64 in the wide kernel), then the reaction
-  // records (16-byte aligned)
-  const size_t cbytes = wide ? sizeof(double) : sizeof(int32_t);
-  const size_t xbytes = ((size_t)smax * kLdsThreads * cbytes + 15) & ~(size_t)15;
-  const size_t lds = xbytes + (size_t)rmax * REBOP_REC_WORDS * sizeof(uint32_t);
+  uint32_t smax, rmax;
+  bool expr;
+  job_shape(jobs, &smax, &rmax, &expr);
+  size_t xbytes = 0;
+  const bool grec = lds_global_records(smax, rmax, width, expr);
+  const size_t lds = lds_bytes(smax, rmax, width, grec, &xbytes);
   if (lds > g_dev.lds_max)
     return fail(REBOP_ERR_UNSUPPORTED,
                 "LDS kernel: " + std::to_string(smax) + " species and " + std::to_string(rmax) +
@@ -707,7 +791,7 @@ int enqueue_lds(const std::vector<rebop_dev_job> &jobs, const std::vector<double
   HIP_TRY(hipMemcpyAsync(pj, jobs.data(), jbytes, hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(pi, items.data(), ibytes, hipMemcpyHostToDevice, stream));
   int per_cu = 0;
-  HIP_TRY(rebop::lds_occupancy(kLdsThreads, lds, wide, expr, &per_cu));
+  HIP_TRY(rebop::lds_occupancy(kLdsThreads, lds, width, expr, grec, &per_cu));
   if (per_cu < 1) per_cu = 1;
   const uint64_t waves_per_block = kLdsThreads / 64;
   uint64_t blocks = (uint64_t)per_cu * (uint64_t)g_dev.cus;
@@ -715,8 +799,8 @@ int enqueue_lds(const std::vector<rebop_dev_job> &jobs, const std::vector<double
   if (blocks > need) blocks = need;
   HIP_TRY(rebop::launch_lds((const rebop_dev_job *)pj, (const rebop_dev_item *)pi,
                             (uint32_t)items.size(), counters, job_events, job_queue, (int)blocks,
-                            kLdsThreads, lds, (uint32_t)(xbytes / sizeof(uint32_t)), wide, expr,
-                            stream));
+                            kLdsThreads, lds, (uint32_t)(xbytes / sizeof(uint32_t)), width, expr,
+                            grec, stream));
   return REBOP_OK;
 }
 
@@ -858,29 +942,24 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
       td.jobs = &g->djobs;
       td.items = &g->ditems;
     }
-    // int32 counts (narrow) unless asked for, a resumable batch (whose state a
-    // rerun could not restore), or shared initial counts beyond int32; a
-    // narrow run that meets a count beyond int32 is rerun wide (int64 counts)
-    // when its counters are read (stats)
-    bool wide = (kernel & REBOP_KERNEL_WIDE) || run.mode == REBOP_MODE_TARGET;
-    if (!run.x0_div)
-      for (int64_t v : g->x0) wide = wide || v > REBOP_LDS_LIMIT || v < -REBOP_LDS_LIMIT;
+    int width = lds_start_width(kernel, run.mode, stats != nullptr,
+                                run.x0_div ? nullptr : &g->x0);
+    if (width == 2 && !compact_pays(std::vector<rebop_dev_job>{job})) width = 4;
     for (;;) {
       EventPair ev;
       if (stats && !sc) HIP_TRY(ev.create());
       HIP_TRY(rebop::launch_reset_counters(ctrs, stream));
       if (ev.e0) HIP_TRY(hipEventRecord(ev.e0, stream));
       rc = enqueue_lds(std::vector<rebop_dev_job>{job}, std::vector<double>{1.0}, td, ctrs,
-                       nullptr, ctrs + REBOP_CTR_JOBQ, wide, stream);
+                       nullptr, ctrs + REBOP_CTR_JOBQ, width, stream);
       if (rc) return rc;
       if (!stats) return REBOP_OK;
       if (ev.e1) HIP_TRY(hipEventRecord(ev.e1, stream));
-      rc = finish_stats(stats, ctrs, ev, REBOP_KERNEL_LDS | (wide ? REBOP_KERNEL_WIDE : 0), stream,
-                        run.draw_cap);
+      rc = finish_stats(stats, ctrs, ev, lds_kind(width), stream, run.draw_cap);
       if (rc == kRetryExact) return fail(REBOP_ERR_DEVICE, "internal error: the LDS kernel flagged f64");
       if (rc != kRetryWide) return rc;
-      if (wide) return fail(REBOP_ERR_DEVICE, "internal error: the wide LDS kernel flagged int32");
-      wide = true;
+      if (width == 8) return fail(REBOP_ERR_DEVICE, "internal error: the wide LDS kernel flagged int32");
+      width *= 2;
     }
   }
   // lanes per trajectory: asked for, or (AUTO) when even 4 (2) lanes per
@@ -1589,14 +1668,17 @@ int rebop_run_jobs_device(const rebop_job *jobs, uint32_t n_jobs, void *stream_,
   TableDest td;
   td.jobs = &jb->djobs;
   td.items = &jb->ditems;
-  // int32 counts unless some job's shared initial counts are beyond int32; a
-  // run that meets a count beyond int32 is rerun with int64 counts when its
-  // counters are read (stats or events_per_job)
-  bool wide = false;
-  for (uint32_t j = 0; j < n_jobs; ++j)
-    if (!jobs[j].d_x0)
-      for (int64_t v : jobs[j].net->x0) wide = wide || v > REBOP_LDS_LIMIT || v < -REBOP_LDS_LIMIT;
+  // the counts' width: the narrowest that every job's shared initial counts
+  // fit (int16 only when the counters are read, so that a launch that meets
+  // a count beyond it can be rerun; device-side initial counts start at
+  // int32); a launch that meets a count beyond its width is rerun with the
+  // next wider counts when its counters are read (stats or events_per_job)
   const bool read = stats || jev;
+  int width = 2;
+  for (uint32_t j = 0; j < n_jobs; ++j)
+    width = std::max(width, lds_start_width(REBOP_KERNEL_AUTO, REBOP_MODE_GRID, read,
+                                            jobs[j].d_x0 ? nullptr : &jobs[j].net->x0));
+  if (width == 2 && !compact_pays(dj)) width = 4;
   rebop_stats local{};
   for (;;) {
     if (jev) HIP_TRY(rebop::launch_zero_u64(jev, n_jobs, stream));
@@ -1605,17 +1687,16 @@ int rebop_run_jobs_device(const rebop_job *jobs, uint32_t n_jobs, void *stream_,
     if (stats) HIP_TRY(ev.create());
     HIP_TRY(rebop::launch_reset_counters(ctrs, stream));
     if (ev.e0) HIP_TRY(hipEventRecord(ev.e0, stream));
-    rc = enqueue_lds(dj, cost, td, ctrs, jev, jq, wide, stream);
+    rc = enqueue_lds(dj, cost, td, ctrs, jev, jq, width, stream);
     if (rc) return rc;
     if (ev.e1) HIP_TRY(hipEventRecord(ev.e1, stream));
     if (!read) return REBOP_OK;
     if (jev)
       HIP_TRY(hipMemcpyAsync(events_per_job, jev, sizeof(unsigned long long) * n_jobs,
                              hipMemcpyDeviceToHost, stream));
-    rc = finish_stats(&local, ctrs, ev, REBOP_KERNEL_LDS | (wide ? REBOP_KERNEL_WIDE : 0), stream,
-                      cap_max);
-    if (rc == kRetryWide && !wide) {
-      wide = true;
+    rc = finish_stats(&local, ctrs, ev, lds_kind(width), stream, cap_max);
+    if (rc == kRetryWide && width < 8) {
+      width *= 2;
       continue;
     }
     if (rc == kRetryWide) return fail(REBOP_ERR_DEVICE, "internal error: the wide LDS kernel flagged int32");

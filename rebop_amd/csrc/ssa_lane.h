@@ -202,6 +202,7 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
   uint32_t i = 0, ov = 0;
   LaneTotals acc;
   bool done = true, started = false, spill = false;
+  rebop_stream st = rebop_stream_of(run.seed, 0ull);
 
   for (;;) {
     bool leave = false;
@@ -213,6 +214,7 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
         leave = true;
       } else {
         jl = nj;
+        st = rebop_stream_of(run.seed, run.traj_begin + jl);
         net.start(run, jl);
         if (run.st_x) {
           net.load_state(x, run.st_x + jl, n);
@@ -249,7 +251,7 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
     bool rec = true; // absorbing state: sample without drawing
     double tn = 0.0, u = 0.0;
     if (0.0 < tot) {
-      const rebop_u32x4 blk = rebop_draw_block(run.seed, run.traj_begin + jl, draw);
+      const rebop_u32x4 blk = rebop_stream_block(st, draw);
       draw += 1;
       tn = t + rebop_exp1_of_tab(blk, ln_tab) / tot;
       u = rebop_unif_of(blk);
@@ -520,6 +522,7 @@ __device__ __forceinline__ LaneTotals lane_events(Net &net, const rebop_dev_run 
   for (;;) {
     const uint64_t jl = next_trajectory(run, started, first, queue_offset);
     if (jl >= n) break;
+    const rebop_stream st = rebop_stream_of(run.seed, run.traj_begin + jl);
     net.start(run, jl);
     net.load_x0(x, run.x0_div ? run.x0 + (jl / run.x0_div) * S : run.x0);
     if (!net.in_range(x)) acc.flags |= Net::kRangeFlag;
@@ -548,7 +551,7 @@ __device__ __forceinline__ LaneTotals lane_events(Net &net, const rebop_dev_run 
         acc.flags |= REBOP_FLAG_RANGE; // time would stop (E / tot == 0)
         t = __builtin_inf();
       } else {
-        const rebop_u32x4 blk = rebop_draw_block(run.seed, run.traj_begin + jl, draw);
+        const rebop_u32x4 blk = rebop_stream_block(st, draw);
         draw += 1;
         t = t + rebop_exp1_of_tab(blk, ln_tab) / tot;
         net.affect(x, net.choose(x, cum, tot * rebop_unif_of(blk)));

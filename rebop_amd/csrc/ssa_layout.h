@@ -129,13 +129,15 @@ typedef struct {
 
 /*
  * One network of an LDS-kernel launch (a "job"), read at wave-uniform
- * addresses.  Counts are int32 in LDS; reactions are grouped in blocks of
- * `block` for the checkpointed cumulative sum.  The jump is sparse.
+ * addresses.  Counts are int16, int32 or int64 in LDS (the launch's width);
+ * reactions are grouped in blocks of `block` for the checkpointed cumulative
+ * sum.  The jump is sparse.
  */
 #define REBOP_JOB_LINEAR 1u /* cumulative sum may be non-monotone: full linear search */
 #define REBOP_JOB_SLOW 2u   /* some reaction's factors are not all in its record */
 #define REBOP_JOB_SYSTEM 4u /* define_system semantics: term products, first-match choice */
 #define REBOP_LDS_LIMIT 1073741824 /* |count| <= 2^30 in the narrow (int32) LDS kernel */
+#define REBOP_LDS16_LIMIT 32767    /* |count| <= 2^15 - 1 in the compact (int16) LDS kernel */
 
 /*
  * Reaction records of the LDS kernel, 4 words per reaction, copied into LDS

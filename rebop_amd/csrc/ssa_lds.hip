@@ -2,13 +2,16 @@
 //
 // Counts live in LDS, one column per lane (x[s] at lds[s * 64 + lane]), so
 // every access -- uniform or lane-varying species index -- maps lane l to
-// the same banks whatever the species: conflict-free.  The narrow kernel
-// keeps them as int32 (|x| <= 2^30, half the LDS, twice the workgroups per
-// CU); the wide one as int64, the reference's isize, for populations beyond
-// int32 and beyond the f64 counts of the register and specialised kernels
-// (|x| <= 2^52), with every addition checked for overflow.  The network (a "job") is read at
-// wave-uniform addresses through the scalar cache, except in the per-lane
-// block scan below.
+// the same banks whatever the species: conflict-free (int16 counts: lanes 2i
+// and 2i+1 share a bank, so lane-varying species cost at most a 2-way
+// conflict).  The compact kernel keeps them as int16 (|x| <= 2^15 - 1, a
+// quarter of the LDS of int64), the narrow one as int32 (|x| <= 2^30); LDS
+// per wave is what bounds the waves per CU, so the narrower counts run more
+// waves.  The wide one keeps int64, the reference's isize, for populations
+// beyond int32 and beyond the f64 counts of the register and specialised
+// kernels (|x| <= 2^52), with every addition checked for overflow.  The
+// network (a "job") is read at wave-uniform addresses through the scalar
+// cache, except in the per-lane block scan below.
 //
 // make_cumrates (src/gillespie.rs:357-364) runs as one uniform pass that
 // keeps the running sum at the end of every block of B reactions (cum[b],
@@ -53,17 +56,22 @@
 
 namespace rebop {
 
-template <int MB, bool WIDE, bool EXPR>
+template <int MB, int W, bool EXPR>
 struct LdsNet {
-  // counts: int32 (narrow) or the reference's full int64 (wide)
-  using CountT = typename std::conditional<WIDE, int64_t, int32_t>::type;
+  // counts of W bytes: int16 (compact), int32 (narrow) or the reference's
+  // full int64 (wide)
+  static constexpr bool WIDE = W == 8;
+  using CountT = typename std::conditional<
+      W == 8, int64_t, typename std::conditional<W == 4, int32_t, int16_t>::type>::type;
   static constexpr int MS = 1;  // counts are in LDS, not in x[]
   static constexpr bool kCap = true; // the check costs nothing next to a pass over R reactions
-  // the narrow kernel's range is the smaller one: the host reruns such a run
-  // wide; the wide one flags an int64 overflow, or a count beyond the f64
-  // state arrays' exact range in a resumable run
+  // a compact or narrow kernel's range is the smaller one: the host reruns
+  // such a run with wider counts; the wide one flags an int64 overflow, or a
+  // count beyond the f64 state arrays' exact range in a resumable run
   static constexpr uint32_t kRangeFlag = WIDE ? REBOP_FLAG_RANGE : REBOP_FLAG_NARROW;
-  static constexpr int64_t kLimit = WIDE ? (int64_t)REBOP_EXACT_LIMIT : (int64_t)REBOP_LDS_LIMIT;
+  static constexpr int64_t kLimit = W == 8   ? (int64_t)REBOP_EXACT_LIMIT
+                                    : W == 4 ? (int64_t)REBOP_LDS_LIMIT
+                                             : (int64_t)REBOP_LDS16_LIMIT;
   static constexpr int MR = MB; // cum[] holds the block checkpoints
   // job fields (wave-uniform)
   uint32_t nS, R, B, nblk;
@@ -219,13 +227,14 @@ struct LdsNet {
   }
 
   // factor (x[s] - c) of a record word whose count x is read, or exactly
-  // 1.0 when absent: the reference's integer difference (int32 in the narrow
-  // kernel, |x| <= 2^30 and c < 4096; int64 in the wide one), converted once
+  // 1.0 when absent: the reference's integer difference (int32 in the
+  // compact and narrow kernels, |x| <= 2^30 and c < 4096; int64 in the wide
+  // one), converted once
   __device__ __forceinline__ double factor_val(uint32_t w, CountT x) const {
     if constexpr (WIDE) {
       return (w & REBOP_REC_PRESENT) ? (double)(x - (int64_t)((w >> 16) & 0xfffu)) : 1.0;
     } else {
-      const int32_t v = x - (int32_t)((w >> 16) & 0xfffu);
+      const int32_t v = (int32_t)x - (int32_t)((w >> 16) & 0xfffu);
       return (double)((w & REBOP_REC_PRESENT) ? v : 1);
     }
   }
@@ -391,10 +400,11 @@ struct LdsNet {
       if (v < 0) linear = true;
       if (!fits(v)) bad = true;
     } else {
-      const int32_t v = (int32_t)((uint32_t)cnt(s) + (uint32_t)d);
-      cnt(s) = v;
+      // |x| <= 2^30 and |d| < 2^23: no int32 wrap before the check
+      const int32_t v = (int32_t)((uint32_t)(int32_t)cnt(s) + (uint32_t)d);
+      cnt(s) = (CountT)v;
       if (v < 0) linear = true;
-      if (v > REBOP_LDS_LIMIT || v < -REBOP_LDS_LIMIT) bad = true;
+      if (v > (int32_t)kLimit || v < -(int32_t)kLimit) bad = true;
     }
   }
 
@@ -417,8 +427,16 @@ struct LdsNet {
   }
 };
 
-template <int MB, bool WIDE, bool EXPR>
-__global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__restrict__ jobs,
+// At most 256 VGPRs, so that two waves share a SIMD: left to itself the
+// compiler takes a few more (the checkpoints alone hold 64) and one wave per
+// SIMD remains.  The int16 and int32 kernels fit without spilling.
+//
+// GREC: the reaction records are read where they lie in global memory
+// (through the caches) instead of from an LDS copy: that leaves more LDS
+// per CU for counts, which pays where the LDS, not the registers, bounds the
+// waves per CU (the host decides).
+template <int MB, int W, bool EXPR, bool GREC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) ssa_lds_kernel(const rebop_dev_job *__restrict__ jobs,
                                                       const rebop_dev_item *__restrict__ items,
                                                       uint32_t n_items,
                                                       unsigned long long *counters,
@@ -432,6 +450,12 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
   const uint32_t lane = __lane_id();
   LaneTotals total;
   for (;;) {
+    // a compact or narrow launch that met a count beyond its range is rerun
+    // wider by the host: take no further item (the rerun rewrites everything)
+    if (W != 8 && (__hip_atomic_load(&counters[REBOP_CTR_FLAGS], __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT) &
+                   REBOP_FLAG_NARROW))
+      break;
     // one item per wave; readfirstlane makes the index (and so every job
     // field and network read below) wave-uniform: scalar loads into SGPRs
     unsigned long long got = 0ull;
@@ -447,14 +471,18 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
     base = __shfl(base, 0);
     if (base >= job.n_traj) continue;
     // the job's reaction records into LDS (every item: a wave's consecutive
-    // items may belong to different jobs)
-    uint32_t *lrec = (uint32_t *)lds_x + rec_word_off;
-    __syncthreads(); // the previous item's reads are done
-    for (uint32_t w = threadIdx.x * 4u; w < job.R * REBOP_REC_WORDS; w += blockDim.x * 4u)
-      *(uint4 *)(lrec + w) = *(const uint4 *)(job.rec + w);
-    __syncthreads();
+    // items may belong to different jobs), or read in place
+    const uint32_t *lrec = job.rec;
+    if constexpr (!GREC) {
+      uint32_t *l = (uint32_t *)lds_x + rec_word_off;
+      __syncthreads(); // the previous item's reads are done
+      for (uint32_t w = threadIdx.x * 4u; w < job.R * REBOP_REC_WORDS; w += blockDim.x * 4u)
+        *(uint4 *)(l + w) = *(const uint4 *)(job.rec + w);
+      __syncthreads();
+      lrec = l;
+    }
 
-    LdsNet<MB, WIDE, EXPR> net;
+    LdsNet<MB, W, EXPR> net;
     net.nS = job.S;
     net.R = job.R;
     net.B = job.block;
@@ -476,7 +504,7 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
     net.smul = job.sys_mult;
     net.job_system = (job.flags & REBOP_JOB_SYSTEM) != 0u;
     net.state_f64 = job.st_x != nullptr;
-    net.xl = reinterpret_cast<typename LdsNet<MB, WIDE, EXPR>::CountT *>(lds_x) + threadIdx.x;
+    net.xl = reinterpret_cast<typename LdsNet<MB, W, EXPR>::CountT *>(lds_x) + threadIdx.x;
     net.rec = lrec;
     net.jrec = job.jrec;
     net.linear = net.job_linear;
@@ -519,6 +547,8 @@ __global__ void __launch_bounds__(256) ssa_lds_kernel(const rebop_dev_job *__res
     total.draws += acc.draws;
     total.trajectories += acc.trajectories;
     total.flags |= acc.flags;
+    if (W != 8 && __ballot((acc.flags & REBOP_FLAG_NARROW) != 0u) != 0ull && lane == 0)
+      atomicOr(&counters[REBOP_CTR_FLAGS], (unsigned long long)REBOP_FLAG_NARROW); // stop the others
   }
   flush_wave_totals(total, counters);
 }
@@ -530,39 +560,57 @@ constexpr int kLdsCheckpoints = 32;
 
 int lds_checkpoints() { return kLdsCheckpoints; }
 
-// EXPR: the variant that also interprets rate expressions (jobs with Rate::Expr
-// reactions; the others keep a kernel without the interpreter's stack)
-static const void *lds_kernel_ptr(bool wide, bool expr) {
-  if (expr)
-    return wide ? (const void *)&ssa_lds_kernel<kLdsCheckpoints, true, true>
-                : (const void *)&ssa_lds_kernel<kLdsCheckpoints, false, true>;
-  return wide ? (const void *)&ssa_lds_kernel<kLdsCheckpoints, true, false>
-              : (const void *)&ssa_lds_kernel<kLdsCheckpoints, false, false>;
+// width: bytes per count (2, 4 or 8).  EXPR: the variant that also
+// interprets rate expressions (jobs with Rate::Expr reactions; the others
+// keep a kernel without the interpreter's stack); it keeps the records in
+// LDS.  grec: records read in place (GREC).
+template <bool EXPR, bool GREC>
+static const void *lds_kernel_ptr_w(int width) {
+  return width == 8   ? (const void *)&ssa_lds_kernel<kLdsCheckpoints, 8, EXPR, GREC>
+         : width == 4 ? (const void *)&ssa_lds_kernel<kLdsCheckpoints, 4, EXPR, GREC>
+                      : (const void *)&ssa_lds_kernel<kLdsCheckpoints, 2, EXPR, GREC>;
 }
 
-hipError_t lds_occupancy(int threads, size_t lds, bool wide, bool expr, int *blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, lds_kernel_ptr(wide, expr),
-                                                      threads, lds);
+static const void *lds_kernel_ptr(int width, bool expr, bool grec) {
+  if (expr) return lds_kernel_ptr_w<true, false>(width);
+  return grec ? lds_kernel_ptr_w<false, true>(width) : lds_kernel_ptr_w<false, false>(width);
+}
+
+hipError_t lds_occupancy(int threads, size_t lds, int width, bool expr, bool grec,
+                         int *blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
+                                                      lds_kernel_ptr(width, expr, grec), threads,
+                                                      lds);
 }
 
 hipError_t launch_lds(const rebop_dev_job *jobs, const rebop_dev_item *items, uint32_t n_items,
                       unsigned long long *counters, unsigned long long *job_events,
                       unsigned long long *job_queue, int blocks, int threads, size_t lds,
-                      uint32_t rec_word_off, bool wide, bool expr, hipStream_t stream) {
+                      uint32_t rec_word_off, int width, bool expr, bool grec,
+                      hipStream_t stream) {
+  if (width != 2 && width != 4 && width != 8) return hipErrorInvalidValue;
+  if (expr && grec) return hipErrorInvalidValue; // the interpreting kernel keeps them in LDS
   if (lds > 64 * 1024) {
-    const hipError_t e = hipFuncSetAttribute(lds_kernel_ptr(wide, expr),
+    const hipError_t e = hipFuncSetAttribute(lds_kernel_ptr(width, expr, grec),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-#define REBOP_LDS_LAUNCH(W, E)                                                                  \
-  hipLaunchKernelGGL((ssa_lds_kernel<kLdsCheckpoints, W, E>), dim3(blocks), dim3(threads), lds, \
-                     stream, jobs, items, n_items, counters, job_events, job_queue, rec_word_off)
+#define REBOP_LDS_LAUNCH(W, E, G)                                                         \
+  hipLaunchKernelGGL((ssa_lds_kernel<kLdsCheckpoints, W, E, G>), dim3(blocks), dim3(threads), \
+                     lds, stream, jobs, items, n_items, counters, job_events, job_queue,     \
+                     rec_word_off)
   if (expr) {
-    if (wide) REBOP_LDS_LAUNCH(true, true);
-    else REBOP_LDS_LAUNCH(false, true);
+    if (width == 8) REBOP_LDS_LAUNCH(8, true, false);
+    else if (width == 4) REBOP_LDS_LAUNCH(4, true, false);
+    else REBOP_LDS_LAUNCH(2, true, false);
+  } else if (grec) {
+    if (width == 8) REBOP_LDS_LAUNCH(8, false, true);
+    else if (width == 4) REBOP_LDS_LAUNCH(4, false, true);
+    else REBOP_LDS_LAUNCH(2, false, true);
   } else {
-    if (wide) REBOP_LDS_LAUNCH(true, false);
-    else REBOP_LDS_LAUNCH(false, false);
+    if (width == 8) REBOP_LDS_LAUNCH(8, false, false);
+    else if (width == 4) REBOP_LDS_LAUNCH(4, false, false);
+    else REBOP_LDS_LAUNCH(2, false, false);
   }
 #undef REBOP_LDS_LAUNCH
   return hipGetLastError();

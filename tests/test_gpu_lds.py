@@ -12,6 +12,7 @@ from tests.networks import build_device, build_oracle, dimers, michaelis_menten,
 
 pytestmark = pytest.mark.gpu
 LDS = _native.REBOP_KERNEL_LDS
+COMPACT = _native.REBOP_KERNEL_COMPACT
 
 
 def _dev(g, tmax, nb, n, kernel, x0=None, traj_begin=0):
@@ -38,10 +39,15 @@ def _dev(g, tmax, nb, n, kernel, x0=None, traj_begin=0):
         ("syn64x256", synthetic(64, 256, 11, init_mean=30.0), 0.05, 10, 200),
     ],
 )
-def test_lds_kernel_bit_exact(name, spec, tmax, nb, n, nthreads):
+@pytest.mark.parametrize("width", ["int16", "int32"])
+def test_lds_kernel_bit_exact(name, spec, tmax, nb, n, width, nthreads, monkeypatch):
+    """int16 counts (REBOP_LDS_COMPACT=1: whenever the counts fit, also
+    where they would not run more waves) and int32 counts
+    (REBOP_LDS_COMPACT=0) both equal the oracle."""
+    monkeypatch.setenv("REBOP_LDS_COMPACT", "1" if width == "int16" else "0")
     g = build_device(spec, 77)
     out, st = _dev(g, tmax, nb, n, LDS, traj_begin=5)
-    assert st.kernel == LDS
+    assert st.kernel == (LDS | COMPACT if width == "int16" else LDS)
     o = build_oracle(spec, sparse=True)  # same results as dense, faster
     _, oout, _, oev, odr = o.run_grid(77, spec[2], tmax, nb, n_traj=n, traj_begin=5, nthreads=nthreads)
     assert np.array_equal(out, oout), np.argwhere(out != oout)[:5].tolist()
@@ -54,7 +60,7 @@ def test_synthetic_128x512_auto_uses_lds(nthreads):
     spec = workloads.synthetic_large()
     g = build_device(spec, 7)
     out, st = _dev(g, 0.05, 20, 64, _native.REBOP_KERNEL_AUTO)
-    assert st.kernel == LDS
+    assert st.kernel == LDS | COMPACT
     o = build_oracle(spec, sparse=True)
     _, oout, _, oev, _ = o.run_grid(7, spec[2], 0.05, 20, n_traj=64, nthreads=nthreads)
     assert np.array_equal(out, oout)
@@ -132,6 +138,79 @@ def _wide_spec(x0a):
         d[i + 1] = 1
         rx.append((0.3, o, d))
     return S, rx, [x0a, 0] + [20] * 38
+
+
+def _int16_spec(a0):
+    """100 species (int32 counts take 25.6 KB of LDS per wave, which bounds
+    the waves per CU; int16 counts lift that bound): A with an inflow in
+    jumps of 700 (so it passes 2^15 within the horizon when it starts near
+    30000), feeding B, which decays, and a chain on the others."""
+    S = 100
+    rx = [(3.0, [0] * S, [700] + [0] * (S - 1)),
+          (0.01, [1] + [0] * (S - 1), [-1, 1] + [0] * (S - 2)),
+          (0.5, [0, 1] + [0] * (S - 2), [0, -1] + [0] * (S - 2))]
+    for i in range(2, S - 1):
+        o = [0] * S
+        o[i] = 1
+        d = [0] * S
+        d[i] = -1
+        d[i + 1] = 1
+        rx.append((0.3, o, d))
+    return S, rx, [a0, 0] + [20] * (S - 2)
+
+
+@pytest.mark.parametrize("a0,kind", [(30000, LDS), (40000, LDS), (2**15 - 1, LDS), (100, LDS | COMPACT)])
+def test_lds_int16_counts_and_rerun(a0, kind, nthreads):
+    """int16 counts while every count stays within 2^15 - 1; a launch that
+    meets a larger count stops early and is rerun with int32 counts (A starts
+    near 30000 and its inflow carries it past); initial counts beyond int16
+    start at int32.  All equal the oracle."""
+    spec = _int16_spec(a0)
+    g = build_device(spec, 12)
+    out, st = _dev(g, 3.0, 12, 256, LDS)
+    assert st.kernel == kind
+    o = build_oracle(spec, sparse=True)
+    _, oout, _, oev, _ = o.run_grid(12, spec[2], 3.0, 12, n_traj=256, nthreads=nthreads)
+    assert np.array_equal(out, oout)
+    assert st.events == oev
+    assert (np.abs(oout).max() > 2**15 - 1) == (kind == LDS)
+
+
+def test_lds_int16_to_int64_in_two_reruns(nthreads, monkeypatch):
+    """Counts that fit int16 at the start and pass 2^30 during the run: the
+    compact launch and then the narrow one stop early, the wide one runs."""
+    monkeypatch.setenv("REBOP_LDS_COMPACT", "1")
+    S, rx, x0 = _wide_spec(20)
+    rx[0] = (600.0, rx[0][1], rx[0][2])  # inflow: ~3600 jumps of 2^20 by t = 6
+    spec = (S, rx, x0)
+    g = build_device(spec, 8)
+    out, st = _dev(g, 6.0, 12, 128, LDS)
+    assert st.kernel == LDS | _native.REBOP_KERNEL_WIDE
+    o = build_oracle(spec, sparse=True)
+    _, oout, _, oev, _ = o.run_grid(8, spec[2], 6.0, 12, n_traj=128, nthreads=nthreads)
+    assert np.array_equal(out, oout) and st.events == oev
+    assert np.abs(oout).max() > 2**30
+
+
+def test_lds_jobs_int16_rerun(nthreads):
+    """A fused batch starts compact when every job's counts fit int16; one
+    job passing 2^15 - 1 reruns the whole launch with int32 counts."""
+    import torch
+
+    specs = [_int16_spec(100), _int16_spec(30000)]
+    nets = [build_device(sp, 20 + i) for i, sp in enumerate(specs)]
+    bufs = [torch.zeros((13, sp[0], 128), dtype=torch.int64, device="cuda") for sp in specs]
+    st, ev = run_jobs([dict(net=g, tmax=3.0, nb_steps=12, n_traj=128, out_ptr=b.data_ptr())
+                       for g, b in zip(nets, bufs)], per_job_events=True)
+    torch.cuda.synchronize()
+    assert st.kernel == LDS
+    for i, sp in enumerate(specs):
+        _, oout, _, oev, _ = build_oracle(sp, sparse=True).run_grid(20 + i, sp[2], 3.0, 12, n_traj=128,
+                                                                    nthreads=nthreads)
+        assert np.array_equal(bufs[i].cpu().numpy(), oout) and int(ev[i]) == oev
+    st1, _ = run_jobs([dict(net=nets[0], tmax=3.0, nb_steps=12, n_traj=128, out_ptr=bufs[0].data_ptr())],
+                      per_job_events=True)
+    assert st1.kernel == LDS | COMPACT
 
 
 @pytest.mark.parametrize("x0a", [2**31 + 5, 2**30 - 3 * 2**20])
@@ -267,7 +346,7 @@ def test_autotune_picks_a_variant_that_stays_bit_exact(nthreads, tmp_path, monke
     assert res.n_tried >= 8 and res.ms > 0
     assert res.threads in (64, 128, 256) and 0 < res.occupancy <= 1.0
     out, st = _dev(g, 0.5, 50, 4096, _native.REBOP_KERNEL_AUTO)
-    assert st.kernel == res.kernel
+    assert st.kernel & ~COMPACT == res.kernel
     o = build_oracle(spec)
     _, oout, _, oev, _ = o.run_grid(31, spec[2], 0.5, 50, n_traj=4096, nthreads=nthreads)
     assert np.array_equal(out, oout) and st.events == oev

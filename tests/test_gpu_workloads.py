@@ -50,7 +50,7 @@ def test_synthetic_128x512_full_horizon(nthreads):
     out = torch.zeros((SYN_NB + 1, spec[0], n), dtype=torch.int64, device="cuda")
     st = g.run_device(SYN_TMAX, SYN_NB, n, out.data_ptr(), kernel=_native.REBOP_KERNEL_AUTO)
     torch.cuda.synchronize()
-    assert st.kernel == _native.REBOP_KERNEL_LDS
+    assert st.kernel == _native.REBOP_KERNEL_LDS | _native.REBOP_KERNEL_COMPACT  # counts < 2^15
     o = build_oracle(spec, sparse=True)
     _, oout, _, oev, odr = o.run_grid(7, spec[2], SYN_TMAX, SYN_NB, n_traj=n, nthreads=nthreads)
     got = out.cpu().numpy()
@@ -74,7 +74,7 @@ def test_synthetic_128x512_full_size(nthreads):
     try:
         st = g.run_device(SYN_TMAX, SYN_NB, n, out.data_ptr(), kernel=_native.REBOP_KERNEL_AUTO)
         torch.cuda.synchronize()
-        assert st.kernel == _native.REBOP_KERNEL_LDS
+        assert st.kernel == _native.REBOP_KERNEL_LDS | _native.REBOP_KERNEL_COMPACT
         assert st.trajectories == n
         assert st.events <= st.draws <= st.events + n * (SYN_NB + 1)
         x0 = torch.as_tensor(np.asarray(spec[2], dtype=np.int64), device="cuda")
@@ -117,7 +117,7 @@ def test_heterogeneous_4096_fused_launch(nthreads):
                      cost_hint=float(c)) for g, t, o, c in zip(nets, tmax, bufs, cost)]
         st, ev = run_jobs(jobs, per_job_events=True)
         torch.cuda.synchronize()
-        assert st.kernel == _native.REBOP_KERNEL_LDS
+        assert st.kernel & ~_native.REBOP_KERNEL_COMPACT == _native.REBOP_KERNEL_LDS
         assert st.trajectories == HET_N * HET_TRAJ
         assert int(ev.sum()) == st.events
         for i in idx:

@@ -44,6 +44,24 @@ def test_philox_matches_aten_engine(oracle):
         assert oracle.philox(ctr, key) == c["words"], c
 
 
+def test_stream_block_equals_philox(oracle):
+    """The kernels' per-trajectory form (round 0's trajectory product taken
+    once, keys recomputed per block) is the same function as the plain
+    Philox4x32-10 block, on the ATen vectors and on random counters with
+    high words set."""
+    data = json.loads((GOLDEN / "philox_aten.json").read_text())
+    for c in data["cases"][:64]:
+        assert oracle.stream_block(int(c["seed"]), int(c["traj"]), int(c["draw"])) == c["words"], c
+    rng = np.random.default_rng(11)
+    for _ in range(2000):
+        seed, traj, draw = (int(v) for v in rng.integers(0, 2**64, size=3, dtype=np.uint64))
+        if rng.random() < 0.5:
+            draw &= 0xFFFFFFFF
+        ctr = [draw & 0xFFFFFFFF, draw >> 32, traj & 0xFFFFFFFF, traj >> 32]
+        key = [seed & 0xFFFFFFFF, seed >> 32]
+        assert oracle.stream_block(seed, traj, draw) == oracle.philox(ctr, key)
+
+
 def test_philox_single_bit_avalanche(oracle):
     """A mistake in a round (dropped xor, swapped lanes) shows up as poor diffusion."""
     base = oracle.philox([1, 2, 3, 4], [5, 6])

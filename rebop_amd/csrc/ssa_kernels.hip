@@ -23,6 +23,7 @@ struct GenericNet {
   static constexpr bool kCap = CAP;
   // f64 counts: a run leaving |x| <= 2^52 is rerun on the int64 LDS kernel
   static constexpr uint32_t kRangeFlag = REBOP_FLAG_EXACT;
+  static constexpr uint32_t kStopFlag = 0u; // a run that is rerun finishes first
   rebop_dev_net net;
   const double *lds_diff; // [R][S]
   double kr[MR];          // this trajectory's rate constants

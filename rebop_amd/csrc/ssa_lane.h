@@ -33,6 +33,8 @@
 //   static constexpr bool kCap;  compile the draw-cap checks in
 //   static constexpr uint32_t kRangeFlag;  raised when in_range fails (REBOP_FLAG_EXACT
 //                                          marks f64 counts, checked every 2^31 draws)
+//   static constexpr uint32_t kStopFlag;   nonzero: once this flag is raised anywhere the
+//                                          launch will be rerun, so lanes stop early
 //   uint32_t S() const;
 //   void start(const rebop_dev_run &run, uint64_t j);    per-trajectory setup (rates)
 //   void load_x0(X &x, const int64_t *src);                counts from an int64 row
@@ -150,6 +152,14 @@ __device__ __forceinline__ uint64_t next_trajectory(const rebop_dev_run &run, bo
   return queue_offset + base + (uint64_t)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
+// Whether some lane of this launch raised `flag` (a launch that will be
+// rerun: the rest of its work would be thrown away).
+__device__ __forceinline__ bool launch_stopped(const rebop_dev_run &run, uint32_t flag) {
+  return (__hip_atomic_load(&run.counters[REBOP_CTR_FLAGS], __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT) &
+          flag) != 0ull;
+}
+
 // The draw count at which a trajectory that started this launch with d0
 // draws hits the watchdog (never, without a cap).
 __device__ __forceinline__ uint64_t cap_limit(const rebop_dev_run &run, uint64_t d0) {
@@ -202,6 +212,7 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
   uint32_t i = 0, ov = 0;
   LaneTotals acc;
   bool done = true, started = false, spill = false;
+  bool stop = false; // (kStopFlag) the launch will be rerun: take nothing more
   rebop_stream st = rebop_stream_of(run.seed, 0ull);
 
   for (;;) {
@@ -209,7 +220,8 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
     if (done) {
       // next trajectory: the lane's own first index, then (persistent
       // launches) indices from the device-wide queue
-      const uint64_t nj = next_trajectory(run, started, first, queue_offset);
+      if constexpr (Net::kStopFlag != 0u) stop = stop || launch_stopped(run, Net::kStopFlag);
+      const uint64_t nj = stop ? n : next_trajectory(run, started, first, queue_offset);
       if (nj >= n) {
         leave = true;
       } else {
@@ -267,6 +279,20 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
       if (!net.in_range(x)) acc.flags |= Net::kRangeFlag;
       if (Net::kRangeFlag == REBOP_FLAG_EXACT && draw - mark >= REBOP_DRAWS_PER_CHECK)
         acc.flags |= REBOP_FLAG_EXACT; // f64 counts: too many draws between two checks
+      if constexpr (Net::kStopFlag != 0u) {
+        // this lane met the flag: tell the others at once; else look every
+        // 8th sample whether another lane did; either way drop the trajectory
+        if (acc.flags & Net::kStopFlag) {
+          if (!stop) atomicOr(&run.counters[REBOP_CTR_FLAGS], (unsigned long long)Net::kStopFlag);
+          stop = true;
+        } else if ((i & 7u) == 7u) {
+          stop = launch_stopped(run, Net::kStopFlag);
+        }
+        if (stop) {
+          done = true;
+          continue;
+        }
+      }
       mark = draw;
       t = T;
       i += 1;

@@ -69,6 +69,7 @@ struct LdsNet {
   // such a run with wider counts; the wide one flags an int64 overflow, or a
   // count beyond the f64 state arrays' exact range in a resumable run
   static constexpr uint32_t kRangeFlag = WIDE ? REBOP_FLAG_RANGE : REBOP_FLAG_NARROW;
+  static constexpr uint32_t kStopFlag = WIDE ? 0u : REBOP_FLAG_NARROW;
   static constexpr int64_t kLimit = W == 8   ? (int64_t)REBOP_EXACT_LIMIT
                                     : W == 4 ? (int64_t)REBOP_LDS_LIMIT
                                              : (int64_t)REBOP_LDS16_LIMIT;

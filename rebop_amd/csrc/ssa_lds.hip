@@ -56,7 +56,14 @@
 
 namespace rebop {
 
-template <int MB, int W, bool EXPR>
+// the reaction records read in place at wave-uniform indices: through the
+// constant address space, so that they are scalar loads into SGPRs and the
+// record decoding runs on the scalar unit
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const __attribute__((address_space(4))) uint4 *rebop_crec_t;
+#endif
+
+template <int MB, int W, bool EXPR, bool GREC>
 struct LdsNet {
   // counts of W bytes: int16 (compact), int32 (narrow) or the reference's
   // full int64 (wide)
@@ -242,7 +249,11 @@ struct LdsNet {
   __device__ __forceinline__ double factor(uint32_t w) const {
     return factor_val(w, cnt(w & 0xffffu));
   }
+  // record r at a wave-uniform index (the pass over the reactions)
   __device__ __forceinline__ uint4 record(uint32_t r) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (GREC) return ((rebop_crec_t)(const void *)rec)[r];
+#endif
     return *(const uint4 *)(rec + (size_t)r * REBOP_REC_WORDS);
   }
 
@@ -258,7 +269,7 @@ struct LdsNet {
 
   // Rate::rate of reaction r when its factors are all in the record
   __device__ __forceinline__ double rate_fast(uint32_t r) const {
-    const uint4 q = *(const uint4 *)(rec + (size_t)r * REBOP_REC_WORDS);
+    const uint4 q = record(r);
     double a = rebop_f64_from_bits(((uint64_t)q.y << 32) | q.x);
     a = a * factor(q.z);
     a = a * factor(q.w);
@@ -483,7 +494,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) s
       lrec = l;
     }
 
-    LdsNet<MB, W, EXPR> net;
+    LdsNet<MB, W, EXPR, GREC> net;
     net.nS = job.S;
     net.R = job.R;
     net.B = job.block;
@@ -505,7 +516,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) s
     net.smul = job.sys_mult;
     net.job_system = (job.flags & REBOP_JOB_SYSTEM) != 0u;
     net.state_f64 = job.st_x != nullptr;
-    net.xl = reinterpret_cast<typename LdsNet<MB, W, EXPR>::CountT *>(lds_x) + threadIdx.x;
+    net.xl = reinterpret_cast<typename LdsNet<MB, W, EXPR, GREC>::CountT *>(lds_x) + threadIdx.x;
     net.rec = lrec;
     net.jrec = job.jrec;
     net.linear = net.job_linear;

@@ -57,8 +57,10 @@
 namespace rebop {
 
 // the reaction records read in place at wave-uniform indices: through the
-// constant address space, so that they are scalar loads into SGPRs and the
-// record decoding runs on the scalar unit
+// constant address space, so that the compiler may make them scalar loads
+// where it proves the index uniform (forcing that everywhere, with
+// readfirstlane, was slower: a scalar load and the LDS reads of the counts
+// it addresses share one wait counter)
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef const __attribute__((address_space(4))) uint4 *rebop_crec_t;
 #endif

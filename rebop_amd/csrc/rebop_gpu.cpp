@@ -347,8 +347,9 @@ int upload_net(rebop_gillespie *g) {
     for (uint32_t q = 0; q < nf && q < 2; ++q) {
       const uint32_t f = fb[r] + q;
       const uint32_t c = (uint32_t)fc[f];
-      if (c >= 4096u || fs[f] >= 65536u) slow_f = true;
-      w[2 + q] = (fs[f] & 0xffffu) | ((c & 0xfffu) << 16) | REBOP_REC_PRESENT;
+      if (c >= 4096u || fs[f] >= 65535u) slow_f = true;
+      // the factor's count row: species s at row s + 1 (row 0: ones)
+      w[2 + q] = ((fs[f] + 1u) & 0xffffu) | ((c & 0xfffu) << 16) | REBOP_REC_PRESENT;
     }
     if (slow_f) {
       w[3] |= REBOP_REC_SLOW_FACTORS;
@@ -701,11 +702,12 @@ int lds_kind(int width) {
   return REBOP_KERNEL_LDS | (width == 8 ? REBOP_KERNEL_WIDE : width == 2 ? REBOP_KERNEL_COMPACT : 0);
 }
 
-// The LDS of one workgroup of an LDS launch: counts [S_max][64] of `width`
-// bytes (16-byte aligned), then the reaction records unless they are read in
-// place (grec).  `xbytes`: the counts' part.
+// The LDS of one workgroup of an LDS launch: counts [1 + S_max][64] of
+// `width` bytes (row 0 holds ones, for absent factors; 16-byte aligned),
+// then the reaction records unless they are read in place (grec).
+// `xbytes`: the counts' part.
 size_t lds_bytes(uint32_t smax, uint32_t rmax, int width, bool grec, size_t *xbytes = nullptr) {
-  const size_t xb = ((size_t)smax * kLdsThreads * (size_t)width + 15) & ~(size_t)15;
+  const size_t xb = ((size_t)(smax + 1u) * kLdsThreads * (size_t)width + 15) & ~(size_t)15;
   if (xbytes) *xbytes = xb;
   return xb + (grec ? 0u : (size_t)rmax * REBOP_REC_WORDS * sizeof(uint32_t));
 }

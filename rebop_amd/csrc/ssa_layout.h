@@ -144,8 +144,9 @@ typedef struct {
  * when a wave starts an item of the job (read R times per event):
  *   w0, w1  rate constant k (f64 bits, low word first)
  *   w2, w3  the first two propensity factors (x[s] - c) in Rate::rate order:
- *           s in bits 0-15, c in bits 16-27, bit 31 = present (an absent
- *           factor multiplies by exactly 1.0)
+ *           the count row s + 1 in bits 0-15, c in bits 16-27, bit 31 =
+ *           present; an absent factor is 0 (row 0 of the counts holds ones,
+ *           c = 0: it multiplies by exactly 1.0)
  *   w3      bit 30: the factors are not all here (more than two, or c >=
  *           4096): use fac_begin/fac_s/fac_c for this reaction
  *           bit 29: the jump is not all in the jump record (more than four

@@ -103,8 +103,9 @@ struct LdsNet {
   const uint32_t *ssp;
   const uint32_t *smul;
   bool job_system;
-  // this lane's column of counts
+  // this lane's column of counts: xr at the row of ones, xl at species 0
   static constexpr uint32_t kBL = 64; // lanes per workgroup (one wave)
+  CountT *xr;
   CountT *xl;
   // the job's reaction records in LDS, jump records in global memory
   const uint32_t *rec;
@@ -117,6 +118,9 @@ struct LdsNet {
   bool bad;
 
   __device__ __forceinline__ CountT &cnt(uint32_t s) const { return xl[s * kBL]; }
+  // the count a record's factor word names: row 0 is a row of ones (an
+  // absent factor), row s + 1 is species s
+  __device__ __forceinline__ CountT rcnt(uint32_t w) const { return xr[(w & 0xffffu) * kBL]; }
   __device__ __forceinline__ double fcnt(uint32_t s) const { return (double)cnt(s); }
 
   __device__ __forceinline__ uint32_t S() const { return nS; }
@@ -236,21 +240,19 @@ struct LdsNet {
     return a;
   }
 
-  // factor (x[s] - c) of a record word whose count x is read, or exactly
-  // 1.0 when absent: the reference's integer difference (int32 in the
-  // compact and narrow kernels, |x| <= 2^30 and c < 4096; int64 in the wide
-  // one), converted once
+  // factor (x[s] - c) of a record word whose count x is read: the
+  // reference's integer difference (int32 in the compact and narrow kernels,
+  // |x| <= 2^30 and c < 4096; int64 in the wide one), converted once.  An
+  // absent factor names the row of ones with c = 0: exactly 1.0, with no
+  // test
   __device__ __forceinline__ double factor_val(uint32_t w, CountT x) const {
     if constexpr (WIDE) {
-      return (w & REBOP_REC_PRESENT) ? (double)(x - (int64_t)((w >> 16) & 0xfffu)) : 1.0;
+      return (double)(x - (int64_t)((w >> 16) & 0xfffu));
     } else {
-      const int32_t v = (int32_t)x - (int32_t)((w >> 16) & 0xfffu);
-      return (double)((w & REBOP_REC_PRESENT) ? v : 1);
+      return (double)((int32_t)x - (int32_t)((w >> 16) & 0xfffu));
     }
   }
-  __device__ __forceinline__ double factor(uint32_t w) const {
-    return factor_val(w, cnt(w & 0xffffu));
-  }
+  __device__ __forceinline__ double factor(uint32_t w) const { return factor_val(w, rcnt(w)); }
   // record r at a wave-uniform index (the pass over the reactions)
   __device__ __forceinline__ uint4 record(uint32_t r) const {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -311,10 +313,10 @@ struct LdsNet {
             q1 = record(n1);
             q2 = record(n2);
             q3 = record(n3);
-            const CountT x0a = cnt(c0.z & 0xffffu), x0b = cnt(c0.w & 0xffffu);
-            const CountT x1a = cnt(c1.z & 0xffffu), x1b = cnt(c1.w & 0xffffu);
-            const CountT x2a = cnt(c2.z & 0xffffu), x2b = cnt(c2.w & 0xffffu);
-            const CountT x3a = cnt(c3.z & 0xffffu), x3b = cnt(c3.w & 0xffffu);
+            const CountT x0a = rcnt(c0.z), x0b = rcnt(c0.w);
+            const CountT x1a = rcnt(c1.z), x1b = rcnt(c1.w);
+            const CountT x2a = rcnt(c2.z), x2b = rcnt(c2.w);
+            const CountT x3a = rcnt(c3.z), x3b = rcnt(c3.w);
             // keep the next records' reads ahead of the wait for these counts
             __builtin_amdgcn_sched_barrier(0);
             const double a0 = rate_of(c0, x0a, x0b), a1 = rate_of(c1, x1a, x1b);
@@ -459,9 +461,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) s
                                                       uint32_t rec_word_off) {
   // no static LDS: the dynamic size (counts + records) alone decides how
   // many workgroups share a CU (e.g. 4 x 40 KB for 128 species, 512 reactions)
-  extern __shared__ int32_t lds_x[]; // counts [S_max][blockDim.x], then the records
+  extern __shared__ int32_t lds_x[]; // counts [1 + S_max][blockDim.x], then the records
 
   const uint32_t lane = __lane_id();
+  using Net = LdsNet<MB, W, EXPR, GREC>;
+  // row 0: ones, the count an absent factor reads (never written again)
+  reinterpret_cast<typename Net::CountT *>(lds_x)[threadIdx.x] = (typename Net::CountT)1;
   LaneTotals total;
   for (;;) {
     // a compact or narrow launch that met a count beyond its range is rerun
@@ -518,7 +523,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) s
     net.smul = job.sys_mult;
     net.job_system = (job.flags & REBOP_JOB_SYSTEM) != 0u;
     net.state_f64 = job.st_x != nullptr;
-    net.xl = reinterpret_cast<typename LdsNet<MB, W, EXPR, GREC>::CountT *>(lds_x) + threadIdx.x;
+    net.xr = reinterpret_cast<typename Net::CountT *>(lds_x) + threadIdx.x;
+    net.xl = net.xr + Net::kBL;
     net.rec = lrec;
     net.jrec = job.jrec;
     net.linear = net.job_linear;

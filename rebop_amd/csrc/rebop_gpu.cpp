@@ -349,7 +349,7 @@ int upload_net(rebop_gillespie *g) {
       const uint32_t c = (uint32_t)fc[f];
       if (c >= 4096u || fs[f] >= 65535u) slow_f = true;
       // the factor's count row: species s at row s + 1 (row 0: ones)
-      w[2 + q] = ((fs[f] + 1u) & 0xffffu) | ((c & 0xfffu) << 16) | REBOP_REC_PRESENT;
+      w[2 + q] = ((fs[f] + 1u) << 16) | (c & 0xfffu) | REBOP_REC_PRESENT;
     }
     if (slow_f) {
       w[3] |= REBOP_REC_SLOW_FACTORS;

@@ -144,22 +144,22 @@ typedef struct {
  * when a wave starts an item of the job (read R times per event):
  *   w0, w1  rate constant k (f64 bits, low word first)
  *   w2, w3  the first two propensity factors (x[s] - c) in Rate::rate order:
- *           the count row s + 1 in bits 0-15, c in bits 16-27, bit 31 =
- *           present; an absent factor is 0 (row 0 of the counts holds ones,
- *           c = 0: it multiplies by exactly 1.0)
- *   w3      bit 30: the factors are not all here (more than two, or c >=
+ *           c in bits 0-11, bit 14 = present, the count row s + 1 in bits
+ *           16-31 (one shift gives the row); an absent factor is 0 (row 0 of
+ *           the counts holds ones, c = 0: it multiplies by exactly 1.0)
+ *   w3      bit 12: the factors are not all here (more than two, or c >=
  *           4096): use fac_begin/fac_s/fac_c for this reaction
- *           bit 29: the jump is not all in the jump record (more than four
+ *           bit 13: the jump is not all in the jump record (more than four
  *           entries): use jump_begin/jump_s/jump_d
  * and jump records, 4 words per reaction, read from global memory once per
  * event: up to four entries, species in bits 0-7, signed difference in bits
  * 8-31 (zero difference = unused).
  */
 #define REBOP_REC_WORDS 4u
-#define REBOP_REC_PRESENT 0x80000000u
+#define REBOP_REC_PRESENT 0x4000u
 #define REBOP_JREC_WORDS 4u
-#define REBOP_REC_SLOW_FACTORS 0x40000000u
-#define REBOP_REC_SLOW_JUMP 0x20000000u
+#define REBOP_REC_SLOW_FACTORS 0x1000u
+#define REBOP_REC_SLOW_JUMP 0x2000u
 
 typedef struct {
   uint32_t S, R, block, nblocks;

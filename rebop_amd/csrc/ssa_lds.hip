@@ -120,7 +120,7 @@ struct LdsNet {
   __device__ __forceinline__ CountT &cnt(uint32_t s) const { return xl[s * kBL]; }
   // the count a record's factor word names: row 0 is a row of ones (an
   // absent factor), row s + 1 is species s
-  __device__ __forceinline__ CountT rcnt(uint32_t w) const { return xr[(w & 0xffffu) * kBL]; }
+  __device__ __forceinline__ CountT rcnt(uint32_t w) const { return xr[(w >> 16) * kBL]; }
   __device__ __forceinline__ double fcnt(uint32_t s) const { return (double)cnt(s); }
 
   __device__ __forceinline__ uint32_t S() const { return nS; }
@@ -247,9 +247,9 @@ struct LdsNet {
   // test
   __device__ __forceinline__ double factor_val(uint32_t w, CountT x) const {
     if constexpr (WIDE) {
-      return (double)(x - (int64_t)((w >> 16) & 0xfffu));
+      return (double)(x - (int64_t)(w & 0xfffu));
     } else {
-      return (double)((int32_t)x - (int32_t)((w >> 16) & 0xfffu));
+      return (double)((int32_t)x - (int32_t)(w & 0xfffu));
     }
   }
   __device__ __forceinline__ double factor(uint32_t w) const { return factor_val(w, rcnt(w)); }

@@ -379,7 +379,9 @@ int upload_net(rebop_gillespie *g) {
   const size_t ojs = ojb + al(sizeof(uint32_t) * (R + 1));
   const size_t ojd = ojs + al(sizeof(uint32_t) * J);
   const size_t orc = ojd + al(sizeof(int32_t) * J);
-  const size_t ojr = orc + al(sizeof(uint32_t) * rec.size());
+  // four records of padding after the last: the LDS kernel's pass reads the
+  // next group of four ahead (never using what lies past R - 1)
+  const size_t ojr = orc + al(sizeof(uint32_t) * (rec.size() + 4u * REBOP_REC_WORDS));
   const size_t oeb = ojr + al(sizeof(uint32_t) * jrec.size());
   const size_t oec = oeb + al(sizeof(uint32_t) * (R + 1));
   const size_t oek = oec + al(sizeof(uint32_t) * EC);
@@ -704,12 +706,13 @@ int lds_kind(int width) {
 
 // The LDS of one workgroup of an LDS launch: counts [1 + S_max][64] of
 // `width` bytes (row 0 holds ones, for absent factors; 16-byte aligned),
-// then the reaction records unless they are read in place (grec).
+// then the reaction records and four of padding, unless they are read in
+// place (grec).
 // `xbytes`: the counts' part.
 size_t lds_bytes(uint32_t smax, uint32_t rmax, int width, bool grec, size_t *xbytes = nullptr) {
   const size_t xb = ((size_t)(smax + 1u) * kLdsThreads * (size_t)width + 15) & ~(size_t)15;
   if (xbytes) *xbytes = xb;
-  return xb + (grec ? 0u : (size_t)rmax * REBOP_REC_WORDS * sizeof(uint32_t));
+  return xb + (grec ? 0u : (size_t)(rmax + 4u) * REBOP_REC_WORDS * sizeof(uint32_t));
 }
 
 // Workgroups (waves) of the LDS kernel per CU for a shape, or 0 when its LDS

@@ -298,21 +298,20 @@ struct LdsNet {
       // (blocks hold a multiple of 4), software-pipelined: the records of the
       // next group are read while this group's counts are in flight, so a
       // group costs one LDS round trip (the sum still adds them one by one)
-      const uint32_t last = R - 1u;
-      uint4 q0 = record(0), q1 = record(1u < last ? 1u : last);
-      uint4 q2 = record(2u < last ? 2u : last), q3 = record(3u < last ? 3u : last);
+      // the record tables end with four records of padding, so the
+      // prefetch of the next group may read up to R + 3 (what it reads past
+      // R - 1 is never used), and the four reads share one base address
+      uint4 q0 = record(0), q1 = record(1u), q2 = record(2u), q3 = record(3u);
 #pragma unroll
       for (int b = 0; b < MB; ++b) { // static: cum[] stays in VGPRs
         if ((uint32_t)b < nblk) {
           const uint32_t re = (r + B < R) ? r + B : R;
           for (; r + 4 <= re; r += 4) {
             const uint4 c0 = q0, c1 = q1, c2 = q2, c3 = q3;
-            const uint32_t n0 = r + 4u < last ? r + 4u : last, n1 = r + 5u < last ? r + 5u : last;
-            const uint32_t n2 = r + 6u < last ? r + 6u : last, n3 = r + 7u < last ? r + 7u : last;
-            q0 = record(n0);
-            q1 = record(n1);
-            q2 = record(n2);
-            q3 = record(n3);
+            q0 = record(r + 4u);
+            q1 = record(r + 5u);
+            q2 = record(r + 6u);
+            q3 = record(r + 7u);
             const CountT x0a = rcnt(c0.z), x0b = rcnt(c0.w);
             const CountT x1a = rcnt(c1.z), x1b = rcnt(c1.w);
             const CountT x2a = rcnt(c2.z), x2b = rcnt(c2.w);

@@ -44,7 +44,7 @@ hipError_t lds_occupancy(int threads, size_t lds, int width, bool expr, bool gre
 hipError_t launch_lds(const rebop_dev_job *jobs, const rebop_dev_item *items, uint32_t n_items,
                       unsigned long long *counters, unsigned long long *job_events,
                       unsigned long long *job_queue, int blocks, int threads, size_t lds,
-                      uint32_t rec_word_off, int width, bool expr, bool grec,
+                      uint32_t rec_word_off, int width, bool expr, bool grec, bool rerun,
                       hipStream_t stream);
 
 } // namespace rebop
@@ -764,7 +764,7 @@ bool compact_pays(const std::vector<rebop_dev_job> &jobs) {
 int enqueue_lds(const std::vector<rebop_dev_job> &jobs, const std::vector<double> &cost,
                 const TableDest &dst, unsigned long long *counters,
                 unsigned long long *job_events, unsigned long long *job_queue, int width,
-                hipStream_t stream) {
+                bool rerun, hipStream_t stream) {
   uint32_t smax, rmax;
   bool expr;
   job_shape(jobs, &smax, &rmax, &expr);
@@ -805,7 +805,7 @@ int enqueue_lds(const std::vector<rebop_dev_job> &jobs, const std::vector<double
   HIP_TRY(rebop::launch_lds((const rebop_dev_job *)pj, (const rebop_dev_item *)pi,
                             (uint32_t)items.size(), counters, job_events, job_queue, (int)blocks,
                             kLdsThreads, lds, (uint32_t)(xbytes / sizeof(uint32_t)), width, expr,
-                            grec, stream));
+                            grec, rerun && width < 8, stream));
   return REBOP_OK;
 }
 
@@ -956,7 +956,7 @@ int launch_run(rebop_gillespie *g, rebop_dev_run run, int kernel, hipStream_t st
       HIP_TRY(rebop::launch_reset_counters(ctrs, stream));
       if (ev.e0) HIP_TRY(hipEventRecord(ev.e0, stream));
       rc = enqueue_lds(std::vector<rebop_dev_job>{job}, std::vector<double>{1.0}, td, ctrs,
-                       nullptr, ctrs + REBOP_CTR_JOBQ, width, stream);
+                       nullptr, ctrs + REBOP_CTR_JOBQ, width, stats != nullptr, stream);
       if (rc) return rc;
       if (!stats) return REBOP_OK;
       if (ev.e1) HIP_TRY(hipEventRecord(ev.e1, stream));
@@ -1692,7 +1692,7 @@ int rebop_run_jobs_device(const rebop_job *jobs, uint32_t n_jobs, void *stream_,
     if (stats) HIP_TRY(ev.create());
     HIP_TRY(rebop::launch_reset_counters(ctrs, stream));
     if (ev.e0) HIP_TRY(hipEventRecord(ev.e0, stream));
-    rc = enqueue_lds(dj, cost, td, ctrs, jev, jq, width, stream);
+    rc = enqueue_lds(dj, cost, td, ctrs, jev, jq, width, read, stream);
     if (rc) return rc;
     if (ev.e1) HIP_TRY(hipEventRecord(ev.e1, stream));
     if (!read) return REBOP_OK;

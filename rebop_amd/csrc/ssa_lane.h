@@ -34,7 +34,8 @@
 //   static constexpr uint32_t kRangeFlag;  raised when in_range fails (REBOP_FLAG_EXACT
 //                                          marks f64 counts, checked every 2^31 draws)
 //   static constexpr uint32_t kStopFlag;   nonzero: once this flag is raised anywhere the
-//                                          launch will be rerun, so lanes stop early
+//                                          launch may be rerun (bool may_stop: it will),
+//                                          so lanes stop early
 //   uint32_t S() const;
 //   void start(const rebop_dev_run &run, uint64_t j);    per-trajectory setup (rates)
 //   void load_x0(X &x, const int64_t *src);                counts from an int64 row
@@ -220,7 +221,8 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
     if (done) {
       // next trajectory: the lane's own first index, then (persistent
       // launches) indices from the device-wide queue
-      if constexpr (Net::kStopFlag != 0u) stop = stop || launch_stopped(run, Net::kStopFlag);
+      if constexpr (Net::kStopFlag != 0u)
+        stop = stop || (net.may_stop && launch_stopped(run, Net::kStopFlag));
       const uint64_t nj = stop ? n : next_trajectory(run, started, first, queue_offset);
       if (nj >= n) {
         leave = true;
@@ -280,9 +282,11 @@ __device__ __forceinline__ LaneTotals lane_loop(Net &net, const rebop_dev_run &r
       if (Net::kRangeFlag == REBOP_FLAG_EXACT && draw - mark >= REBOP_DRAWS_PER_CHECK)
         acc.flags |= REBOP_FLAG_EXACT; // f64 counts: too many draws between two checks
       if constexpr (Net::kStopFlag != 0u) {
-        // this lane met the flag: tell the others at once; else look every
-        // 8th sample whether another lane did; either way drop the trajectory
-        if (acc.flags & Net::kStopFlag) {
+        // (a launch that will be rerun) this lane met the flag: tell the
+        // others at once; else look every 8th sample whether another lane
+        // did; either way drop the trajectory
+        if (!net.may_stop) {
+        } else if (acc.flags & Net::kStopFlag) {
           if (!stop) atomicOr(&run.counters[REBOP_CTR_FLAGS], (unsigned long long)Net::kStopFlag);
           stop = true;
         } else if ((i & 7u) == 7u) {

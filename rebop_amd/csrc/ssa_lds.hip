@@ -113,6 +113,9 @@ struct LdsNet {
   // the run keeps its state in f64 arrays (resumable batch): counts must stay
   // within |x| <= 2^52 there
   bool state_f64;
+  // the host reruns this launch wider if a count leaves its range: lanes
+  // may then drop their work (kStopFlag)
+  bool may_stop;
   // per-lane flags
   bool linear;
   bool bad;
@@ -457,7 +460,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) s
                                                       unsigned long long *counters,
                                                       unsigned long long *job_events,
                                                       unsigned long long *job_queue,
-                                                      uint32_t rec_word_off) {
+                                                      uint32_t rec_word_off, uint32_t rerun) {
   // no static LDS: the dynamic size (counts + records) alone decides how
   // many workgroups share a CU (e.g. 4 x 40 KB for 128 species, 512 reactions)
   extern __shared__ int32_t lds_x[]; // counts [1 + S_max][blockDim.x], then the records
@@ -469,10 +472,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) s
   LaneTotals total;
   for (;;) {
     // a compact or narrow launch that met a count beyond its range is rerun
-    // wider by the host: take no further item (the rerun rewrites everything)
-    if (W != 8 && (__hip_atomic_load(&counters[REBOP_CTR_FLAGS], __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT) &
-                   REBOP_FLAG_NARROW))
+    // wider by the host (`rerun`: its counters are read): take no further
+    // item (the rerun rewrites everything)
+    if (W != 8 && rerun &&
+        (__hip_atomic_load(&counters[REBOP_CTR_FLAGS], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+         REBOP_FLAG_NARROW))
       break;
     // one item per wave; readfirstlane makes the index (and so every job
     // field and network read below) wave-uniform: scalar loads into SGPRs
@@ -528,6 +532,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) s
     net.jrec = job.jrec;
     net.linear = net.job_linear;
     net.bad = false;
+    net.may_stop = rerun != 0u;
 
     rebop_dev_run run{};
     run.seed = job.seed;
@@ -566,7 +571,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) s
     total.draws += acc.draws;
     total.trajectories += acc.trajectories;
     total.flags |= acc.flags;
-    if (W != 8 && __ballot((acc.flags & REBOP_FLAG_NARROW) != 0u) != 0ull && lane == 0)
+    if (W != 8 && rerun && __ballot((acc.flags & REBOP_FLAG_NARROW) != 0u) != 0ull && lane == 0)
       atomicOr(&counters[REBOP_CTR_FLAGS], (unsigned long long)REBOP_FLAG_NARROW); // stop the others
   }
   flush_wave_totals(total, counters);
@@ -605,7 +610,7 @@ hipError_t lds_occupancy(int threads, size_t lds, int width, bool expr, bool gre
 hipError_t launch_lds(const rebop_dev_job *jobs, const rebop_dev_item *items, uint32_t n_items,
                       unsigned long long *counters, unsigned long long *job_events,
                       unsigned long long *job_queue, int blocks, int threads, size_t lds,
-                      uint32_t rec_word_off, int width, bool expr, bool grec,
+                      uint32_t rec_word_off, int width, bool expr, bool grec, bool rerun,
                       hipStream_t stream) {
   if (width != 2 && width != 4 && width != 8) return hipErrorInvalidValue;
   if (expr && grec) return hipErrorInvalidValue; // the interpreting kernel keeps them in LDS
@@ -617,7 +622,7 @@ hipError_t launch_lds(const rebop_dev_job *jobs, const rebop_dev_item *items, ui
 #define REBOP_LDS_LAUNCH(W, E, G)                                                         \
   hipLaunchKernelGGL((ssa_lds_kernel<kLdsCheckpoints, W, E, G>), dim3(blocks), dim3(threads), \
                      lds, stream, jobs, items, n_items, counters, job_events, job_queue,     \
-                     rec_word_off)
+                     rec_word_off, rerun ? 1u : 0u)
   if (expr) {
     if (width == 8) REBOP_LDS_LAUNCH(8, true, false);
     else if (width == 4) REBOP_LDS_LAUNCH(4, true, false);

@@ -231,6 +231,27 @@ def test_lds_counts_beyond_int32(x0a, nthreads):
     assert np.abs(oout).max() > 2**30
 
 
+def test_lds_async_narrow_run_keeps_the_trajectories_in_range(nthreads):
+    """Without counters (stats NULL) nothing reruns a launch, so its lanes
+    do not drop their work when another trajectory leaves the count range:
+    the trajectories that stay in range equal the oracle."""
+    import torch
+
+    spec = _wide_spec(100)
+    n = 128
+    x0 = np.tile(np.asarray(spec[2], dtype=np.int64), (n, 1))
+    x0[64:, 0] = 2**30 - 3 * 2**20  # these pass 2^30: left wrong, as documented
+    g = build_device(spec, 14)
+    out = torch.zeros((13, spec[0], n), dtype=torch.int64, device="cuda")
+    xt = torch.as_tensor(x0, device="cuda")
+    st = g.run_device(6.0, 12, n, out.data_ptr(), kernel=LDS, x0_ptr=xt.data_ptr(), with_stats=False)
+    torch.cuda.synchronize()
+    assert st is None
+    o = build_oracle(spec, sparse=True)
+    _, oout, _, _, _ = o.run_grid(14, x0[:64], 6.0, 12, n_traj=64, x0_per_traj=True, nthreads=nthreads)
+    assert np.array_equal(out[:, :, :64].cpu().numpy(), oout)
+
+
 def test_lds_wide_per_trajectory_x0_and_jobs(nthreads):
     """Device-side initial counts (not visible to the host) start narrow and
     are rerun wide; so are fused job batches; the batch state runs wide."""

@@ -124,6 +124,42 @@ def test_counts_growing_past_2p52_are_rerun(kernel, nthreads):
     assert np.array_equal(out2, oout2)
 
 
+def test_events_growing_past_2p52_are_rerun():
+    """The every-event loop: the specialised kernel flags the first event
+    beyond 2^52 and both passes (counting, recording) are rerun on int64
+    counts; the records equal the oracle's."""
+    spec = _big(2**52 - 2**22, inflow=3.0, d0=0)
+    g = build_device(spec, 17)
+    ts, xs = g.run_all_events(2.0, n_traj=6)
+    o = build_oracle(spec)
+    grew = False
+    for j in range(6):
+        ot, ox = o.run_all_events(17, j, spec[2], 2.0)
+        assert np.array_equal(ts[j], ot)
+        assert np.array_equal(xs[j], ox)
+        grew = grew or int(ox[:, 0].max()) > 2**52
+    assert grew
+
+
+def test_fused_jobs_beyond_2p52(nthreads):
+    """A fused batch holding a job whose counts are beyond 2^52 runs with
+    int64 counts; every job equals the oracle."""
+    import torch
+
+    from rebop_amd.engine import run_jobs
+
+    specs = [_big(2**60 + 7), _big(5, d0=0)]
+    nets = [build_device(sp, 30 + i) for i, sp in enumerate(specs)]
+    bufs = [torch.zeros((9, sp[0], 96), dtype=torch.int64, device="cuda") for sp in specs]
+    st, ev = run_jobs([dict(net=g, tmax=2.0, nb_steps=8, n_traj=96, out_ptr=b.data_ptr())
+                       for g, b in zip(nets, bufs)], per_job_events=True)
+    torch.cuda.synchronize()
+    assert st.kernel == LDS_WIDE
+    for i, sp in enumerate(specs):
+        _, oout, _, oev, _ = build_oracle(sp).run_grid(30 + i, sp[2], 2.0, 8, n_traj=96, nthreads=nthreads)
+        assert np.array_equal(bufs[i].cpu().numpy(), oout) and int(ev[i]) == oev
+
+
 def test_int64_overflow_and_batch_limits():
     """An int64 overflow is an error (the reference's isize addition would
     wrap in a release build, panic in a debug one); resumable batches keep

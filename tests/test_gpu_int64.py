@@ -160,6 +160,19 @@ def test_fused_jobs_beyond_2p52(nthreads):
         assert np.array_equal(bufs[i].cpu().numpy(), oout) and int(ev[i]) == oev
 
 
+def test_python_api_counts_beyond_2p52():
+    """The reference's Python surface (Gillespie.run) with an initial count
+    f64 cannot hold: A -> B conserves A + B = 2^60 + 7 exactly."""
+    from rebop_amd import Gillespie
+
+    s = Gillespie()
+    s.add_reaction(2.0**-60, ["A"], ["B"])
+    ds = s.run({"A": 2**60 + 7}, tmax=5.0, nb_steps=10, rng=3)
+    A, B = np.asarray(ds.A, dtype=np.int64), np.asarray(ds.B, dtype=np.int64)
+    assert ((A + B) == 2**60 + 7).all()
+    assert B[-1] > 0 and A[0] == 2**60 + 7
+
+
 def test_int64_overflow_and_batch_limits():
     """An int64 overflow is an error (the reference's isize addition would
     wrap in a release build, panic in a debug one); resumable batches keep

@@ -56,13 +56,11 @@
 
 namespace rebop {
 
-// the reaction records read in place at wave-uniform indices: through the
-// constant address space, so that the compiler may make them scalar loads
-// where it proves the index uniform (forcing that everywhere, with
-// readfirstlane, was slower: a scalar load and the LDS reads of the counts
-// it addresses share one wait counter)
 #if defined(__HIP_DEVICE_COMPILE__)
-typedef const __attribute__((address_space(4))) uint4 *rebop_crec_t;
+// tables read in place as global memory: vector loads counted on the vector
+// memory counter only (a flat load also counts on the LDS one, so waiting
+// for a group's counts in LDS would wait for the next group's records too)
+typedef const __attribute__((address_space(1))) uint4 *rebop_grec_t;
 #endif
 
 template <int MB, int W, bool EXPR, bool GREC>
@@ -256,17 +254,17 @@ struct LdsNet {
     }
   }
   __device__ __forceinline__ double factor(uint32_t w) const { return factor_val(w, rcnt(w)); }
-  // record r at a wave-uniform index (the pass over the reactions)
+  // record r (read in place: as global memory)
   __device__ __forceinline__ uint4 record(uint32_t r) const {
 #if defined(__HIP_DEVICE_COMPILE__)
-    if constexpr (GREC) return ((rebop_crec_t)(const void *)rec)[r];
+    if constexpr (GREC) return ((rebop_grec_t)(const void *)rec)[r];
 #endif
     return *(const uint4 *)(rec + (size_t)r * REBOP_REC_WORDS);
   }
 
   // Rate::rate of reaction r, factors in stored order: k * f1 * f2
   __device__ __forceinline__ double rate(uint32_t r) const {
-    const uint4 q = *(const uint4 *)(rec + (size_t)r * REBOP_REC_WORDS);
+    const uint4 q = record(r);
     if (q.w & REBOP_REC_SLOW_FACTORS) return rate_slow(r);
     double a = rebop_f64_from_bits(((uint64_t)q.y << 32) | q.x);
     a = a * factor(q.z);
@@ -430,8 +428,12 @@ struct LdsNet {
   // define_system choice that matched no reaction) changes nothing
   __device__ __forceinline__ void affect(double (&)[MS], uint32_t kk) {
     if (kk >= R) return;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint4 j = ((rebop_grec_t)(const void *)jrec)[kk];
+#else
     const uint4 j = *(const uint4 *)(jrec + (size_t)kk * REBOP_JREC_WORDS);
-    if (rec[(size_t)kk * REBOP_REC_WORDS + 3] & REBOP_REC_SLOW_JUMP) {
+#endif
+    if (record(kk).w & REBOP_REC_SLOW_JUMP) {
       const uint32_t e1 = jb[kk + 1];
       for (uint32_t e = jb[kk]; e < e1; ++e) add_count(js[e], jd[e]);
       return;

@@ -323,8 +323,9 @@ class SystemNetwork(SingleNetwork):
 
 
 class ExprNetwork(SingleNetwork):
-    """The network with every rate a Rate::Expr (src/expr.rs), compiled into
-    the kernel; exprs[r] is reaction r's expression over the species names."""
+    """The network with Rate::Expr rates (src/expr.rs): exprs[r] is reaction
+    r's expression over the species names, or None where the reaction keeps
+    its mass-action rate from the spec."""
 
     def __init__(self, *a, names, exprs, **kw):
         super().__init__(*a, **kw)
@@ -334,18 +335,18 @@ class ExprNetwork(SingleNetwork):
         from rebop_amd import expr as _expr
 
         species = {n: i for i, n in enumerate(self.names)}
-        return [_expr.to_postfix(_expr.parse(e), species, {}) for e in self.exprs]
+        return [None if e is None else _expr.to_postfix(_expr.parse(e), species, {}) for e in self.exprs]
 
     def setup(self):
         import torch
 
-        from rebop_amd.engine import GillespieNet, RateExpr
+        from rebop_amd.engine import GillespieNet, Rate, RateExpr
 
         self.out = torch.empty((self.nb + 1, self.spec[0], self.n), dtype=torch.int64, device="cuda")
         self.stream = torch.cuda.current_stream().cuda_stream
         self.g = GillespieNet(self.spec[2], seed=0)
-        for (code, consts), (_, _, d) in zip(self._postfix(), self.spec[1]):
-            self.g.add_reaction(RateExpr.from_postfix(code, consts), d)
+        for pf, (k, o, d) in zip(self._postfix(), self.spec[1]):
+            self.g.add_reaction(Rate.lma(k, o) if pf is None else RateExpr.from_postfix(*pf), d)
 
     def cpu_make_net(self):
         from oracle import pyoracle
@@ -354,8 +355,11 @@ class ExprNetwork(SingleNetwork):
 
         def make(spec):
             net = pyoracle.OracleNet(spec[0])
-            for (code, consts), (_, _, d) in zip(pf, spec[1]):
-                net.add_expr(code, consts, d)
+            for p, (k, o, d) in zip(pf, spec[1]):
+                if p is None:
+                    net.add_lma(k, o, d)
+                else:
+                    net.add_expr(*p, d)
             return net
 
         return make
@@ -477,6 +481,14 @@ def make_workload(name: str, kernel: int) -> Workload:
                            "tmax=1.0, nb_steps=100", CPU_SAMPLE,
                            names=["gene", "mRNA", "protein", "dimer"],
                            exprs=["25*gene", "1000*mRNA", "0.001*protein*(protein-1)", "0.1*mRNA", "protein"])
+    if name == "big-expr":
+        spec, names, exprs = workloads.expression_large()
+        return ExprNetwork("big-expr", spec, 5.0, 50, 2**17, kernel,
+                           "mixed network beyond the specialised kernels' size: 48 species, 96 "
+                           "reactions (growth-bounded random mass action, seed 11), the 53 with "
+                           "one reactant molecule given Hill / exp / min / threshold Rate::Expr "
+                           "rates, 2^17 trajectories, tmax=5, nb_steps=50", 1024,
+                           names=names, exprs=exprs)
     if name == "sir-events":
         return EventsNetwork("sir-events", networks.sir(), 250.0, 10_000,
                              "SIR (3 species, 2 reactions), 10^4 trajectories, tmax=250, nb_steps=0 "
@@ -608,11 +620,12 @@ def main() -> int:
                     help="before timing, time every kernel variant and launch shape at the "
                          "workload's full size (rebop_autotune); AUTO steps then use the fastest")
     ap.add_argument("--workload", default="dimer", choices=["dimer", "sir", "syn", "het", "surrogate", "dimer-system",
-                                                              "dimer-expr", "sir-events"],
+                                                              "dimer-expr", "big-expr", "sir-events"],
                     help="dimer (the headline configuration), sir, syn (128x512), het (4096 "
                          "networks), surrogate (Michaelis-Menten sweep -> MLP -> W1 gate), "
                          "dimer-system (define_system! arithmetic), dimer-expr (Rate::Expr rates), "
-                         "sir-events (nb_steps = 0)")
+                         "big-expr (48x96 with Rate::Expr rates, LDS kernel), sir-events "
+                         "(nb_steps = 0)")
     ap.add_argument("--surrogate-points", type=int, default=100_000)
     ap.add_argument("--no-surrogate", action="store_true",
                     help="leave the surrogate half of the metric out of the default (dimer) line")

@@ -27,6 +27,38 @@ def synthetic_large(seed: int = 7):
     return networks.synthetic(128, 512, seed, init_mean=100.0)
 
 
+def expression_large(n_species: int = 48, n_reactions: int = 96, seed: int = 11):
+    """A mixed network beyond the specialised kernels' 32 species / 64
+    reactions: the growth-bounded random mass-action network of that size,
+    with every reaction whose only reactant is one molecule of X_a given a
+    Rate::Expr rate (src/expr.rs) in place of k * X_a, taking turns among a
+    Hill repression by another species X_b, an exponential damping by X_b,
+    min(X_a, X_b + 1) and a threshold max(X_a - 5, 0).  Each is 0 when X_a
+    is 0 and never negative, so counts stay non-negative.
+
+    Returns (spec, names, exprs): spec = (S, [(k, orders, diffs)], x0) is
+    the mass-action network (its orders give the operation count per event),
+    names[s] = "X<s>", exprs[r] the expression text or None for a reaction
+    that keeps its Rate::lma."""
+    S, rx, x0 = networks.synthetic(n_species, n_reactions, seed, init_mean=50.0, bounded=True)
+    rng = np.random.default_rng(seed + 1)
+    names = [f"X{s}" for s in range(S)]
+    exprs = []
+    turn = 0
+    for k, orders, _ in rx:
+        if sum(orders) != 1:
+            exprs.append(None)
+            continue
+        a, b = int(np.flatnonzero(orders)[0]), int(rng.integers(0, S))
+        c = repr(k)
+        exprs.append([f"{c} * X{a} / (1 + (X{b} / 40)^2)",
+                      f"{c} * X{a} * exp(-X{b} / 100)",
+                      f"{c} * min(X{a}, X{b} + 1)",
+                      f"{c} * max(X{a} - 5, 0)"][turn % 4])
+        turn += 1
+    return (S, rx, x0), names, exprs
+
+
 def heterogeneous_specs(n_networks: int, seed: int, s_range=(3, 64), r_range=(4, 256),
                         init_mean: float = 50.0):
     """Network specs (S, reactions, x0) of a heterogeneous batch."""

@@ -45,3 +45,35 @@ def test_events_cpu_baseline_counts_events(monkeypatch):
     r = bench.cpu_baseline_events(wl, wl.cpu_jobs()[1])
     assert r["value"] > 0 and r["cores"] >= 1 and r["kind"] == "port"
     assert r["philox_ln64_events_per_s_trajectories"] > 0
+
+
+def test_large_expression_workload():
+    """The mixed 48x96 workload: its Rate::Expr rates (in the oracle's
+    interpreter) are the mass-action rate times the stated factor, 0 when
+    the reactant is absent; reactions without an expression keep Rate::lma;
+    the network is beyond the specialised kernels' 32 / 64."""
+    import math
+
+    wl = bench.make_workload("big-expr", 0)
+    S, rx, x0 = wl.spec
+    assert S > 32 and len(rx) > 64
+    n_expr = sum(e is not None for e in wl.exprs)
+    assert n_expr == sum(sum(o) == 1 for _, o, _ in rx) and n_expr > 40
+    net = wl.cpu_make_net()(wl.spec)
+    lma = bench.oracle_lma_net(wl.spec)
+    rng = np.random.default_rng(5)
+    for x in (list(x0), [0] * S, rng.integers(0, 120, S).tolist()):
+        for r, ((k, o, _), e) in enumerate(zip(rx, wl.exprs)):
+            got = net.rate(r, x)
+            if e is None:
+                assert got == lma.rate(r, x)
+                continue
+            a = o.index(1)
+            b = int(e.rsplit("X", 1)[1].split(" ")[0].split("/")[0].split(")")[0])
+            xa, xb = float(x[a]), float(x[b])
+            want = {"/ (1": k * xa / (1 + (xb / 40) ** 2), "exp(": k * xa * math.exp(-xb / 100),
+                    "min(": k * min(xa, xb + 1), "max(": k * max(xa - 5, 0)}
+            key = next(t for t in want if t in e)
+            assert math.isclose(got, want[key], rel_tol=1e-14, abs_tol=0), (r, e, x[a], x[b])
+            if x[a] == 0:
+                assert got == 0.0

@@ -172,6 +172,26 @@ def test_large_expression_network_runs_on_the_lds_kernel(nthreads):
     assert e.value.code == K.REBOP_ERR_UNSUPPORTED
 
 
+def test_large_mixed_expression_workload_bit_exact(nthreads):
+    """The benchmark's 48x96 mixed network (Rate::Expr on 53 reactions,
+    Rate::lma on the rest) at a few hundred trajectories of its full horizon:
+    AUTO runs it on the LDS kernel, bit for bit against the oracle."""
+    from rebop_amd import workloads
+
+    (S, rx, x0), names, exprs = workloads.expression_large()
+    species = {n: i for i, n in enumerate(names)}
+    net = (S, x0, species, [(e, None, d) if e is not None else (k, o, d)
+                            for e, (k, o, d) in zip(exprs, rx)])
+    g, o = _build(net, 42)
+    n, tmax, nb = 384, 5.0, 50
+    out, st = _device_run(g, tmax, nb, n, K.REBOP_KERNEL_AUTO, traj_begin=5)
+    assert st.kernel & K.REBOP_KERNEL_LDS
+    _, oout, _, oev, odr = o.run_grid(42, x0, tmax, nb, n_traj=n, traj_begin=5, nthreads=nthreads)
+    assert np.array_equal(out, oout), np.argwhere(out != oout)[:5].tolist()
+    assert st.events == oev and st.draws == odr
+    assert oev > 1000 * n
+
+
 def test_expression_networks_in_a_fused_batch(nthreads):
     """Expression networks in one fused launch with a mass-action network:
     each job equals its own oracle run."""
